@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""LIJ sweep benchmark on one MI355X.
+
+Workload: the 1,536,000-element system (1,200 lines x 1,280 elements, 9x9 FP64
+blocks, diagonal shift 12, seed 4) from the seeded structured-grid generator,
+generated directly in HBM.  One step = one LIJ sweep (Thomas solve + U update,
+residual R = rhs + J U, its norm), factorisation excluded; the timed region
+launches K sweeps back to back with a HIP event pair around every kernel on the
+library's stream.  Prints ONE JSON line.
+
+Also measured (outside the timed region, reported as extra fields): the same
+sweep with FP32 factors, hipGraph launch throughput, time to relative residual
+1e-10 including factorisation for FP64 and FP32 factors, and the oracle's CPU
+throughput on a bounded slice of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, published
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nl", type=int, default=1200)
+    ap.add_argument("--nk", type=int, default=1280)
+    ap.add_argument("--shift", type=float, default=12.0)
+    ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extras", action="store_true", help="only the FP64 timed region")
+    return ap.parse_args()
+
+
+def cpu_baseline(tbs, args, nelem_full):
+    """The oracle (OpenMP over lines, all host cores) on a bounded slice of the
+    workload: the first lines of the same grid, same generator, 3 timed sweeps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_py import Oracle
+    import numpy as np
+    cores = os.cpu_count() or 1
+    cores = min(cores, 16)
+    nl = max(1, args.nl // 10)
+    m = tbs.structured_mesh(nl, args.nk)
+    jac = tbs.generate_host(args.seed, 9, m.nelem, m.nintface, args.shift)
+    o = Oracle()
+    o.lij_run(m, jac, 1, nthreads=cores)                 # warm-up
+    out = o.lij_run(m, jac, 3, nthreads=cores)
+    sweeps_per_s_sample = 3.0 / out["t_sweeps"]
+    scale = m.nelem / nelem_full
+    return {"value": sweeps_per_s_sample * scale, "unit": "sweeps/s", "cores": cores, "kind": "port",
+            "sample": f"{nl} x {args.nk} slice ({m.nelem} elements, {scale:.3f} of the workload), "
+                      f"3 sweeps after 1 warm-up, OpenMP over lines; sweeps/s scaled by the element ratio",
+            "t_sweep_sample_s": out["t_sweeps"] / 3.0, "t_factor_sample_s": out["t_factor"]}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is designed "
+                                   "(DESIGN.md) but not built", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import triblocksolve_amd as tbs
+
+    torch.cuda.init()
+    ctx = tbs.Context(0, 9)
+    stream = torch.cuda.ExternalStream(ctx.stream())
+    m = tbs.structured_mesh(args.nl, args.nk)
+    n, nif = m.nelem, m.nintface
+    z = lambda k: torch.empty(k, dtype=torch.float64, device="cuda")
+    D, O1, O2, rhs = z(81 * n), z(81 * nif), z(81 * nif), z(9 * n)
+    ctx.generate_device(args.seed, n, nif, args.shift, D, O1, O2, rhs)
+    ctx.sync()
+
+    def timed_sweeps(prec, steps, warmup):
+        plan = tbs.Plan(ctx, m, prec)
+        t0 = time.perf_counter()
+        plan.factor(D, O1, O2)
+        ctx.sync()
+        t_factor = time.perf_counter() - t0
+        info = plan.info()
+        plan.begin(rhs, None)
+        for _ in range(warmup):
+            plan.launch_thomas()
+            plan.launch_residual()
+        ctx.sync()
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+        torch.cuda.synchronize()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for s in range(steps):
+            ev[s][0].record(stream)
+            plan.launch_thomas()
+            ev[s][1].record(stream)
+            plan.launch_residual()
+            ev[s][2].record(stream)
+        ctx.sync()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        th = np.array([ev[s][0].elapsed_time(ev[s][1]) for s in range(steps)]) * 1e-3
+        rs = np.array([ev[s][1].elapsed_time(ev[s][2]) for s in range(steps)]) * 1e-3
+        # graph throughput (informational)
+        ctx.sync()
+        plan.launch_sweeps(steps, use_graph=True)   # capture + first launch
+        ctx.sync()
+        t1 = time.perf_counter()
+        plan.launch_sweeps(steps, use_graph=True)
+        ctx.sync()
+        t_graph = time.perf_counter() - t1
+        plan.close()
+        return dict(wall=wall, thomas=th, residual=rs, info=info, t_factor=t_factor, t_graph=t_graph)
+
+    def time_to_tol(prec, rtol=1e-10):
+        plan = tbs.Plan(ctx, m, prec)
+        U = torch.zeros(9 * n, dtype=torch.float64, device="cuda")
+        ctx.sync()
+        t0 = time.perf_counter()
+        plan.factor(D, O1, O2)
+        ctx.sync()
+        t1 = time.perf_counter()
+        hist, ns = plan.solve(rhs, U, 100, rtol)
+        ctx.sync()
+        t2 = time.perf_counter()
+        plan.close()
+        return dict(factor_s=t1 - t0, solve_s=t2 - t1, total_s=t2 - t0, sweeps=int(ns),
+                    final_rel_residual=float(hist[-1] / hist[0]), U=U)
+
+    r64 = timed_sweeps(tbs.TBS_FP64, args.steps, args.warmup)
+    info = r64["info"]
+    ms_per_step = r64["wall"] / args.steps * 1e3
+    sweeps_per_s = args.steps / r64["wall"]
+    model = info["alg_bytes_per_sweep"]
+    gbs = model / (r64["wall"] / args.steps) / 1e9
+    t_th, t_rs = float(r64["thomas"].mean()), float(r64["residual"].mean())
+    kern = {
+        "thomas": {"avg_ms": t_th * 1e3, "alg_bytes": info["thomas_bytes"],
+                   "GBps": info["thomas_bytes"] / t_th / 1e9},
+        "residual": {"avg_ms": t_rs * 1e3, "alg_bytes": info["residual_bytes"],
+                     "GBps": info["residual_bytes"] / t_rs / 1e9},
+    }
+    dom = "residual" if t_rs >= t_th else "thomas"
+    roof = {"bound": "hbm", "kernel": dom, "achieved": kern[dom]["GBps"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": kern[dom]["GBps"] / HBM_PEAK_GBS, "traffic": None}
+    out = {
+        "metric": "LIJ sweeps/s at 1.5M elem 9x9 (FP64)",
+        "value": sweeps_per_s,
+        "unit": "sweeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded structured-grid generator, generated in HBM)",
+        "config": {"workload": f"lij_{args.nl}x{args.nk}_nvar9_shift{args.shift:g}_seed{args.seed}",
+                   "nelem": n, "nline": args.nl, "line_len": args.nk, "nintface": nif,
+                   "precision": "fp64", "parallelism": "1 GPU"},
+        "gbps_model": gbs,
+        "frac_hbm_peak": gbs / HBM_PEAK_GBS,
+        "bytes_per_sweep_model": model,
+        "bytes_per_sweep_reference_model": tbs.reference_model_bytes(m),
+        "gbps_reference_model": tbs.reference_model_bytes(m) / (r64["wall"] / args.steps) / 1e9,
+        "kernels": kern,
+        "roofline": roof,
+        "graph_sweeps_per_s": args.steps / r64["t_graph"],
+        "factor_s": r64["t_factor"],
+    }
+    if not args.no_extras:
+        r32 = timed_sweeps(tbs.TBS_FP32, args.steps, args.warmup)
+        i32 = r32["info"]
+        out["fp32_factor"] = {
+            "sweeps_per_s": args.steps / r32["wall"],
+            "gbps_model": i32["alg_bytes_per_sweep"] / (r32["wall"] / args.steps) / 1e9,
+            "thomas_ms": float(r32["thomas"].mean()) * 1e3,
+            "residual_ms": float(r32["residual"].mean()) * 1e3,
+            "graph_sweeps_per_s": args.steps / r32["t_graph"],
+        }
+        a = time_to_tol(tbs.TBS_FP64)
+        b = time_to_tol(tbs.TBS_FP32)
+        err = float(torch.linalg.norm(b["U"] - a["U"]) / torch.linalg.norm(a["U"]))
+        for d in (a, b):
+            d.pop("U")
+        out["time_to_1e-10"] = {"fp64": a, "fp32_factor": b, "fp32_vs_fp64_U_relerr": err}
+    if not args.no_cpu:
+        cb = cpu_baseline(tbs, args, n)
+        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        out["cpu_baseline_detail"] = {k: cb[k] for k in ("t_sweep_sample_s", "t_factor_sample_s")}
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,231 @@
+/*
+ * triblock_hip.h -- C ABI of the MI355X (gfx950) line-implicit Jacobi (LIJ)
+ * block solver library, libtriblock_hip.so.
+ *
+ * Conventions
+ *   - Every entry point returns int: 0 (TBS_OK) on success, a negative
+ *     TBS_ERR_* code for an argument / state error, or a positive hipError_t
+ *     value when a HIP call failed.  tbs_last_error() gives a message for the
+ *     calling thread.  Nothing in the library calls exit().
+ *   - "device" pointers are HIP device allocations owned by the caller; the
+ *     library never frees them.  "host" pointers are ordinary host memory.
+ *   - Block arrays use the reference layout: entry (i,j) of block b at
+ *     i + nvar*j + nvar*nvar*b (column-major blocks, OKL @dim(NVAR,NVAR,n)).
+ *     Vectors: entry i of element e at i + nvar*e.  Indices are 0-based int32
+ *     (after the reference reader's 1 -> 0 shift); boundary faces are < 0.
+ *   - Launches are asynchronous on the context's stream, like OCCA's default
+ *     stream; tbs_ctx_synchronize() waits for them.
+ *   - A context is thread-compatible (one host thread at a time).
+ */
+#ifndef TRIBLOCK_HIP_H
+#define TRIBLOCK_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TBS_OK              0
+#define TBS_ERR_ARG        -1   /* bad argument (null pointer, size, nvar)   */
+#define TBS_ERR_STATE      -2   /* call out of order (e.g. solve before factor) */
+#define TBS_ERR_UNSUPPORTED -3  /* nvar / mode not built into this library    */
+#define TBS_ERR_NOMEM      -4   /* admission / allocation refused              */
+#define TBS_ERR_IO         -5   /* file missing, truncated or inconsistent     */
+#define TBS_ERR_NUMERIC    -6   /* non-finite residual or divergence guard     */
+
+#define TBS_FP64 0              /* factors and Thomas sweeps in FP64           */
+#define TBS_FP32 1              /* factors and Thomas sweeps in FP32; residual,
+                                   update and norm stay FP64                   */
+
+typedef struct tbs_ctx tbs_ctx;
+typedef struct tbs_plan tbs_plan;
+typedef struct tbs_batch tbs_batch;
+
+const char *tbs_last_error(void);
+const char *tbs_version(void);
+/* Block sizes (nvar) the drop-in kernels / the fused plan path are built for:
+ * writes up to cap values, returns the count. */
+int tbs_supported_nvar(int *out, int cap);
+int tbs_plan_supported_nvar(int *out, int cap);
+
+/* ---------------------------------------------------------------------- */
+/* Context: replaces Platform (include/Platform.hxx:25-92,                */
+/* src/Platform.cxx:11-78) -- device selection and the launch stream.     */
+/* ---------------------------------------------------------------------- */
+int tbs_ctx_create(int device_id, int nvar, tbs_ctx **out);
+int tbs_ctx_destroy(tbs_ctx *ctx);
+/* Use an existing HIP stream (hipStream_t) instead of the context's own. */
+int tbs_ctx_set_stream(tbs_ctx *ctx, void *hip_stream);
+void *tbs_ctx_stream(tbs_ctx *ctx);
+int tbs_ctx_synchronize(tbs_ctx *ctx);
+int tbs_ctx_nvar(tbs_ctx *ctx);
+/* Device memory helpers for hosts without another allocator. */
+int tbs_malloc(tbs_ctx *ctx, size_t bytes, void **dptr);
+int tbs_free(tbs_ctx *ctx, void *dptr);
+int tbs_memcpy_h2d(tbs_ctx *ctx, void *dst, const void *src, size_t bytes);
+int tbs_memcpy_d2h(tbs_ctx *ctx, void *dst, const void *src, size_t bytes);
+int tbs_memset(tbs_ctx *ctx, void *dst, int value, size_t bytes);
+
+/* ---------------------------------------------------------------------- */
+/* Drop-in kernels: one entry per active reference OKL kernel, same       */
+/* argument order, all pointers device pointers.                          */
+/* ---------------------------------------------------------------------- */
+/* B[n] = A[n], n < nelem*nvar*nvar.      src/okl/linesmoothLU_v5.okl:85-93  */
+int tbs_copyAtoBjac(tbs_ctx *ctx, int nelem, const double *A, double *B);
+/* B[n] = A[n], n < nelem*nvar.           src/okl/linesmoothLU_v5.okl:75-83  */
+int tbs_copyAtoB(tbs_ctx *ctx, int nelem, const double *A, double *B);
+/* B[n] += A[n], n < nelem*nvar.          src/okl/linesmoothLU_v5.okl:95-103 */
+int tbs_addAtoB(tbs_ctx *ctx, int nelem, const double *A, double *B);
+
+/* Packed in-line lower blocks by element id (host loop in the reference,
+ * here a device gather):  A[e] = (e==fc[2f]) ? jacO2[f] : jacO1[f] with
+ * f = lineface[m], for every line position k >= 1.
+ * Replaces Jacobian::assembleTriBlocks, src/Jacobian.cxx:76-117.          */
+int tbs_assembleTriBlocks(tbs_ctx *ctx, int nline, const int *linesize,
+                          const int *linepoint, const int *lines,
+                          const int *lineface, const int *fc,
+                          const double *jacO1, const double *jacO2,
+                          double *A);
+
+/* Block-Thomas factorisation, in place: Dia holds D on entry and the LU
+ * factors D' on exit; DinvC[elast] = D'^-1 C for every non-last element.
+ * Replaces lineLU, src/okl/lineLU_v2.okl:58-69.                            */
+int tbs_lineLU(tbs_ctx *ctx, int nelem, int nintfaces, int nlines,
+               const int *fc, const int *linesize, const int *linepoint,
+               const int *lines, const int *lineface, double *Dia,
+               const double *Of1, const double *Of2, double *DinvC);
+
+/* Thomas forward elimination + back substitution: dU = -M^-1 R per line.
+ * Replaces triblock_solveDU, src/okl/linesmoothLU_TriBlock_v3.okl:92-104. */
+int tbs_triblock_solveDU(tbs_ctx *ctx, int nelem, int nintfaces, int eftot,
+                         int nlines, int linelemtot, const int *linesize,
+                         const int *linepoint, const int *lines,
+                         const double *Dia, const double *DinvC,
+                         const double *A, double *dU, const double *R);
+
+/* Residual accumulation R(e) += D(e)U(e) + sum_faces Off(f)U(nb); the
+ * caller presets R = rhs (as src/main.cxx:217 does).
+ * Replaces triblock_lineRes, src/okl/linesmoothLU_TriBlock_v3.okl:228-243. */
+int tbs_triblock_lineRes(tbs_ctx *ctx, int nelem, int nintfaces, int eftot,
+                         int nlines, int linelemtot, const int *epoint,
+                         const int *ef, const int *fc, const int *linesize,
+                         const int *linepoint, const int *lines,
+                         const double *Dia, const double *Of1,
+                         const double *Of2, const double *U, double *R);
+
+/* ||v||_2 of n doubles (device), deterministic fixed-order reduction; the
+ * reference has no norm (build-defined convergence measure).              */
+int tbs_norm2(tbs_ctx *ctx, long n, const double *v, double *out_host);
+
+/* ---------------------------------------------------------------------- */
+/* Fused LIJ path ("plan"): the outer loop of src/main.cxx:180-229 as     */
+/* graph-captured HIP kernels over a device-resident, re-laid-out copy of  */
+/* the system.  Mesh topology is given as HOST arrays (the Mesh class's    */
+/* host vectors, include/Mesh.hxx:41-48); Jacobian values as DEVICE        */
+/* arrays in the reference layout (Jacobian::o_jacD/o_jacO1/o_jacO2).      */
+/* ---------------------------------------------------------------------- */
+typedef struct tbs_mesh_host {
+    int nvar, nelem, nintface, eftot, nline, nlineelem;
+    const int *epoint;     /* [nelem+1]      */
+    const int *ef;         /* [eftot]        */
+    const int *fc;         /* [2*nintface]   */
+    const int *linesize;   /* [nline]        */
+    const int *linepoint;  /* [nline+1]      */
+    const int *lines;      /* [nlineelem]    */
+    const int *lineface;   /* [nlineelem]    */
+} tbs_mesh_host;
+
+typedef struct tbs_plan_info {
+    int nvar, nelem, nline, max_faces, precision;
+    size_t device_bytes;          /* HBM held by the plan                      */
+    double alg_bytes_per_sweep;   /* algorithmic bytes of one sweep (model)    */
+    double ref_bytes_per_sweep;   /* the reference's own byte model,
+                                     src/main.cxx:146-175, per sweep          */
+    double thomas_bytes;          /* algorithmic bytes of one Thomas launch   */
+    double residual_bytes;        /* algorithmic bytes of one residual launch */
+} tbs_plan_info;
+
+/* Footprint the plan would need, without allocating (admission control). */
+int tbs_plan_footprint(const tbs_mesh_host *mesh, int precision, size_t *bytes);
+int tbs_plan_create(tbs_ctx *ctx, const tbs_mesh_host *mesh, int precision,
+                    tbs_plan **out);
+int tbs_plan_destroy(tbs_plan *plan);
+int tbs_plan_info_get(tbs_plan *plan, tbs_plan_info *info);
+/* Factorise: block-Thomas LU (lineLU semantics) + the sweep-form factors. */
+int tbs_plan_factor(tbs_plan *plan, const double *jacD, const double *jacO1,
+                    const double *jacO2);
+/* Solve J U = -rhs by LIJ sweeps from the initial U (device, reference
+ * layout, overwritten with the result).  Stops after max_sweeps or when
+ * ||R_k|| <= rtol * ||R_0||.  hist (host, >= max_sweeps+1) gets ||R_k||_2,
+ * k = 0..nsweeps.  R_final (device, may be NULL) gets rhs + J U.         */
+int tbs_plan_solve(tbs_plan *plan, const double *rhs, double *U,
+                   int max_sweeps, double rtol, double *hist, int *nsweeps,
+                   double *R_final);
+
+/* Low-level stepping, for timing and graph capture (all asynchronous). */
+int tbs_plan_begin(tbs_plan *plan, const double *rhs, const double *U0);
+int tbs_plan_launch_thomas(tbs_plan *plan);
+int tbs_plan_launch_residual(tbs_plan *plan);   /* residual + norm of sweep */
+int tbs_plan_launch_sweeps(tbs_plan *plan, int nsweeps, int use_graph);
+int tbs_plan_end(tbs_plan *plan, double *U, double *R_final);
+int tbs_plan_history(tbs_plan *plan, double *hist, int cap, int *count);
+
+/* ---------------------------------------------------------------------- */
+/* Batched solves: many independent plans (uneven line lengths) advanced   */
+/* by one launch per kernel per sweep, each with its own convergence test. */
+/* ---------------------------------------------------------------------- */
+int tbs_batch_create(tbs_ctx *ctx, tbs_plan *const *plans, int nplans,
+                     tbs_batch **out);
+int tbs_batch_destroy(tbs_batch *batch);
+/* rhs[i], U[i]: device pointers of plan i; hist: host [nplans][max_sweeps+1];
+ * nsweeps: host [nplans].                                                  */
+int tbs_batch_solve(tbs_batch *batch, const double *const *rhs,
+                    double *const *U, int max_sweeps, double rtol,
+                    int sweeps_per_check, double *hist, int *nsweeps);
+
+/* ---------------------------------------------------------------------- */
+/* Synthetic inputs (the reference reads files only; these generate the    */
+/* seeded structured-grid workloads).  Counter-based SplitMix64: value =   */
+/* a + (b-a)*u, u = (mix64(key + (i+1)*G) >> 11) * 2^-53.                  */
+/* ---------------------------------------------------------------------- */
+/* Structured NL x NK mesh (lines along the fast index) into caller arrays
+ * sized from tbs_structured_sizes; 0-based like the reader's output.      */
+int tbs_structured_sizes(int NL, int NK, int *nelem, int *nintface,
+                         int *eftot, int *nline, int *nlineelem);
+int tbs_structured_mesh(int NL, int NK, int *epoint, int *ef, int *fc,
+                        int *linesize, int *linepoint, int *lines,
+                        int *lineface);
+/* Fill a Jacobian system: D = U[-1,1] + shift*I, Of1/Of2 = U[-0.1,0.1],
+ * rhs = U[-1,1].  Host or device variant (device: on the ctx stream).    */
+int tbs_generate_host(uint64_t seed, int nvar, int nelem, int nintface,
+                      double shift, double *jacD, double *jacO1,
+                      double *jacO2, double *rhs);
+int tbs_generate_device(tbs_ctx *ctx, uint64_t seed, int nvar, int nelem,
+                        int nintface, double shift, double *jacD,
+                        double *jacO1, double *jacO2, double *rhs);
+
+/* Binary files of the reference (src/Mesh.cxx:11-73, read_mesh_data.F90;
+ * src/Jacobian.cxx:11-74, read_jacobian_data.F90).  Readers validate the
+ * header against the file size.  sizes[6] = nvar, nelem, nintface, eftot,
+ * nline, nlineelem.                                                       */
+int tbs_read_mesh_header(const char *path, int sizes[6]);
+int tbs_read_mesh(const char *path, int *epoint, int *ef, int *fc,
+                  int *linesize, int *linepoint, int *lines, int *lineface);
+int tbs_write_mesh(const char *path, const int sizes[6], const int *epoint,
+                   const int *ef, const int *fc, const int *linesize,
+                   const int *linepoint, const int *lines,
+                   const int *lineface);
+int tbs_read_jacobian_header(const char *path, int sizes[3]);
+int tbs_read_jacobian(const char *path, double *jacD, double *jacO1,
+                      double *jacO2, double *rhs, double *U0);
+int tbs_write_jacobian(const char *path, int nvar, int nelem, int nintface,
+                       const double *jacD, const double *jacO1,
+                       const double *jacO2, const double *rhs,
+                       const double *U0);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TRIBLOCK_HIP_H */

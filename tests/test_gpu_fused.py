@@ -1,0 +1,187 @@
+"""GPU: the fused sweep path (plan / batch) against the oracle through the C ABI.
+
+Tolerances (stated per result):
+  FP64  history: |h_k/h_0 - h'_k/h'_0| <= 1e-12 * max(h'_k/h'_0, 1e-3)   (h' = oracle)
+        solution: ||U - U'|| <= 1e-12 ||U'||
+  FP32 factors: converge to h_k/h_0 <= 1e-10 and ||U - U*|| <= 1e-9 ||U*||,
+        U* = FP64 oracle solution run to the round-off floor.
+"""
+import numpy as np
+import pytest
+
+import closed_forms as cf
+import synth_np
+from conftest import system
+from gpu_util import cross_lines, dev, host, ragged_lines, reversed_lines
+
+pytestmark = pytest.mark.gpu
+
+
+def relerr(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def ctxs(tbs):
+    cache = {}
+
+    def get(nv):
+        if nv not in cache:
+            cache[nv] = tbs.Context(0, nv)
+        return cache[nv]
+    yield get
+    for c in cache.values():
+        c.close()
+
+
+def solve_plan(tbs, ctx, m, jac, iters, rtol=0.0, prec=0, want_R=False):
+    import torch
+    plan = tbs.Plan(ctx, m, prec)
+    plan.factor(dev(jac["jacD"]), dev(jac["jacO1"]), dev(jac["jacO2"]))
+    U = torch.zeros(m.nvar * m.nelem, dtype=torch.float64, device="cuda")
+    R = torch.zeros_like(U) if want_R else None
+    hist, ns = plan.solve(dev(jac["rhs"]), U, iters, rtol, R_out=R)
+    out = dict(hist=hist, ns=ns, U=host(U), R=host(R) if want_R else None)
+    plan.close()
+    return out
+
+
+@pytest.mark.parametrize("shift", [20.0, 12.0])
+def test_cfg1_fp64_history_and_solution(tbs, ctxs, oracle, shift):
+    print(f"case cfg1 60x64 shift {shift}", flush=True)
+    m, jac = system(60, 64, 9, seed=1, shift=shift)
+    ref = oracle.lij_run(m, jac, 10)
+    out = solve_plan(tbs, ctxs(9), m, jac, 10, want_R=True)
+    assert out["ns"] == 10
+    ok, worst = cf.history_close(out["hist"], ref["hist"])
+    print(f"history worst {worst:.3e}; U relerr {relerr(out['U'], ref['U']):.3e}")
+    assert ok, worst
+    assert relerr(out["U"], ref["U"]) < 1e-12
+    # R_final is rhs + J U for the returned U
+    R_cf = jac["rhs"] + cf.assemble_J(m, jac) @ out["U"]
+    assert np.allclose(out["R"], R_cf, rtol=0, atol=1e-13 * np.abs(jac["rhs"]).max())
+
+
+VARIANTS = [
+    ("reversed_5x13_nv9", lambda: reversed_lines(system(5, 13, 9)[0])),
+    ("cross_7x9_nv5", lambda: cross_lines(7, 9, 5)),
+    ("ragged_nv9", lambda: ragged_lines([1, 2, 17, 64, 3, 100, 1, 65, 7, 8, 9], 9)),
+    ("grid_4x6_nv4", lambda: system(4, 6, 4)[0]),
+    ("grid_1x1_nv9", lambda: system(1, 1, 9)[0]),
+    ("grid_70x1_nv9", lambda: system(70, 1, 9)[0]),
+]
+
+
+@pytest.mark.parametrize("name,make", VARIANTS, ids=[v[0] for v in VARIANTS])
+def test_variants_fp64(tbs, ctxs, oracle, name, make):
+    print(f"case {name}", flush=True)
+    m = make()
+    jac = synth_np.generate(11, m.nvar, m.nelem, m.nintface, 2.0 * m.nvar + 2)
+    ref = oracle.lij_run(m, jac, 8)
+    out = solve_plan(tbs, ctxs(m.nvar), m, jac, 8)
+    ok, worst = cf.history_close(out["hist"], ref["hist"])
+    assert ok, worst
+    assert relerr(out["U"], ref["U"]) < 1e-12
+
+
+def test_fp32_factors_reach_fp64_accuracy(tbs, ctxs, oracle):
+    """FP32 factors and Thomas sweeps, FP64 residual/update/norm: same 1e-10 target."""
+    m, jac = system(60, 64, 9, seed=4, shift=12.0)
+    ref = oracle.lij_run(m, jac, 12)
+    o64 = solve_plan(tbs, ctxs(9), m, jac, 50, rtol=1e-10, prec=0)
+    o32 = solve_plan(tbs, ctxs(9), m, jac, 50, rtol=1e-10, prec=1)
+    print(f"sweeps fp64 {o64['ns']} fp32 {o32['ns']}; fp32 U err vs oracle {relerr(o32['U'], ref['U']):.3e}")
+    assert o64["hist"][-1] <= 1e-10 * o64["hist"][0]
+    assert o32["hist"][-1] <= 1e-10 * o32["hist"][0]
+    assert o32["ns"] <= o64["ns"] + 2
+    assert relerr(o32["U"], ref["U"]) < 1e-9
+
+
+def test_graph_replay_equals_eager(tbs, ctxs):
+    """Several replays of a captured K-sweep graph, with eager work between them,
+    give exactly the eager launches' history and solution."""
+    import torch
+    m, jac = system(30, 40, 9, seed=2)
+    ctx = ctxs(9)
+    res = {}
+    for mode in ("eager", "graph"):
+        plan = tbs.Plan(ctx, m)
+        plan.factor(dev(jac["jacD"]), dev(jac["jacO1"]), dev(jac["jacO2"]))
+        rhs = dev(jac["rhs"])
+        plan.begin(rhs, None)
+        for _ in range(3):
+            plan.launch_sweeps(2, use_graph=(mode == "graph"))
+            ctx.sync()
+            torch.ones(1000, device="cuda").sum().item()   # unrelated eager work
+        U = torch.zeros(9 * m.nelem, dtype=torch.float64, device="cuda")
+        plan.end(U)
+        ctx.sync()
+        res[mode] = (plan.history(), host(U))
+        plan.close()
+    assert np.array_equal(res["eager"][0], res["graph"][0])
+    assert np.array_equal(res["eager"][1], res["graph"][1])
+
+
+def test_deterministic(tbs, ctxs):
+    m, jac = system(40, 50, 9, seed=3)
+    a = solve_plan(tbs, ctxs(9), m, jac, 6)
+    b = solve_plan(tbs, ctxs(9), m, jac, 6)
+    assert np.array_equal(a["U"], b["U"]) and np.array_equal(a["hist"], b["hist"])
+
+
+def test_batch_equals_alone_and_oracle(tbs, ctxs, oracle):
+    """Systems of uneven lines advanced by one launch per kernel: each result is
+    bitwise the one it gets alone, and matches the oracle."""
+    import torch
+    ctx = ctxs(9)
+    specs = [system(20, 30, 9, seed=5)[0], reversed_lines(system(6, 17, 9)[0]),
+             ragged_lines([5, 96, 7, 130, 64], 9), system(3, 200, 9)[0]]
+    jacs = [synth_np.generate(21 + i, 9, m.nelem, m.nintface, 20.0 - 2 * i) for i, m in enumerate(specs)]
+    plans = []
+    for m, j in zip(specs, jacs):
+        p = tbs.Plan(ctx, m)
+        p.factor(dev(j["jacD"]), dev(j["jacO1"]), dev(j["jacO2"]))
+        plans.append(p)
+    batch = tbs.Batch(ctx, plans)
+    Us = [torch.zeros(9 * m.nelem, dtype=torch.float64, device="cuda") for m in specs]
+    hists, ns = batch.solve([dev(j["rhs"]) for j in jacs], Us, 40, rtol=1e-10, sweeps_per_check=3)
+    batch.close()
+    for i, (m, j) in enumerate(zip(specs, jacs)):
+        alone = solve_plan(tbs, ctx, m, j, 40, rtol=1e-10)
+        assert ns[i] == alone["ns"]
+        assert np.array_equal(hists[i], alone["hist"]), i
+        assert np.array_equal(host(Us[i]), alone["U"]), i
+        ref = oracle.lij_run(m, j, int(ns[i]))
+        assert cf.history_close(hists[i], ref["hist"])[0]
+        assert hists[i][-1] <= 1e-10 * hists[i][0]
+    for p in plans:
+        p.close()
+
+
+def test_bad_inputs_raise(tbs, ctxs):
+    m, jac = system(4, 5, 9)
+    bad = type(m)(m.nvar, m.epoint, m.ef, m.fc, m.linesize, m.linepoint, np.zeros_like(m.lines), m.lineface)
+    with pytest.raises(tbs.TbsError):
+        tbs.Plan(ctxs(9), bad)
+    p = tbs.Plan(ctxs(9), m)
+    import torch
+    with pytest.raises(tbs.TbsError) as ei:
+        p.solve(dev(jac["rhs"]), torch.zeros(9 * m.nelem, dtype=torch.float64, device="cuda"), 3)
+    assert ei.value.code == -2
+    p.close()
+    with pytest.raises(tbs.TbsError):
+        tbs.Context(0, 33)
+
+
+def test_divergence_is_reported(tbs, ctxs):
+    """Off-line blocks scaled so the iteration diverges: the solve stops with TBS_ERR_NUMERIC."""
+    import torch
+    m, jac = system(10, 12, 9, seed=6, shift=0.5)
+    for k in ("jacO1", "jacO2"):
+        jac[k] = jac[k] * 40.0
+    plan = tbs.Plan(ctxs(9), m)
+    plan.factor(dev(jac["jacD"]), dev(jac["jacO1"]), dev(jac["jacO2"]))
+    with pytest.raises(tbs.TbsError) as ei:
+        plan.solve(dev(jac["rhs"]), torch.zeros(9 * m.nelem, dtype=torch.float64, device="cuda"), 200, 1e-10)
+    assert ei.value.code == -6
+    plan.close()

@@ -1,0 +1,185 @@
+"""CPU: pin the oracle (oracle/oracle.cpp) against closed forms, special cases
+and brute force.  Runs anywhere (no GPU)."""
+import numpy as np
+import pytest
+
+import closed_forms as cf
+from conftest import system
+
+
+def _dd(rng, nv, shift=None):
+    A = rng.uniform(-1, 1, (nv, nv))
+    return A + (shift if shift is not None else 2.0 * nv) * np.eye(nv)
+
+
+@pytest.mark.parametrize("nv", [1, 2, 3, 5, 9, 12])
+def test_LU_reconstructs_and_solves(oracle, nv):
+    """LU2:17-27 Doolittle without pivoting: L (unit lower) @ U == A; solveLU == dense solve."""
+    rng = np.random.default_rng(nv)
+    for _ in range(5):
+        A = _dd(rng, nv)
+        F = oracle.LU(nv, A)
+        Lm = np.tril(F, -1) + np.eye(nv)
+        Um = np.triu(F)
+        assert np.allclose(Lm @ Um, A, rtol=0, atol=1e-13 * np.abs(A).max())
+        b = rng.uniform(-1, 1, nv)
+        x = oracle.solveLU(nv, F, b)
+        assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-13, atol=1e-15)
+
+
+def test_LU_is_not_pivoted(oracle):
+    """A matrix whose partial-pivoting LU would swap rows still factors in place."""
+    A = np.array([[1.0, 2.0], [3.0, 4.0]])
+    F = oracle.LU(2, A)
+    assert np.allclose(F, [[1.0, 2.0], [3.0, -2.0]])
+
+
+@pytest.mark.parametrize("NL,NK,nv", [(4, 6, 3), (3, 5, 9), (60, 64, 9)])
+def test_assemble_matches_line_structure(oracle, NL, NK, nv):
+    """src/Jacobian.cxx:76-117: A[e] is the block coupling e to its in-line predecessor."""
+    m, jac = system(NL, NK, nv)
+    Aout = oracle.assembleTriBlocks(m, jac["jacO1"], jac["jacO2"])
+    M = cf.assemble_M(m, jac).toarray()
+    for l in range(m.nline):
+        for k in range(1, m.linesize[l]):
+            e, ep = m.lines[m.linepoint[l] + k], m.lines[m.linepoint[l] + k - 1]
+            blk = Aout[nv * nv * e: nv * nv * (e + 1)].reshape(nv, nv, order="F")
+            assert np.array_equal(blk, M[nv * e: nv * e + nv, nv * ep: nv * ep + nv])
+
+
+@pytest.mark.parametrize("NL,NK,nv,shift", [(4, 6, 3, 6.0), (3, 5, 9, 20.0), (60, 64, 9, 20.0), (7, 33, 9, 12.0)])
+def test_thomas_equals_sparse_direct_solve(oracle, NL, NK, nv, shift):
+    """lineLU + solveDU solve M dU = -R (scipy splu of the assembled block-tridiagonal M)."""
+    import scipy.sparse.linalg as spla
+    m, jac = system(NL, NK, nv, seed=3, shift=shift)
+    Dia, DinvC = oracle.lineLU(m, jac["jacD"], jac["jacO1"], jac["jacO2"])
+    Aout = oracle.assembleTriBlocks(m, jac["jacO1"], jac["jacO2"])
+    R = np.random.default_rng(1).uniform(-1, 1, nv * m.nelem)
+    dU = oracle.solveDU(m, Dia, DinvC, Aout, R)
+    ref = -spla.splu(cf.assemble_M(m, jac).tocsc()).solve(R)
+    assert np.allclose(dU, ref, rtol=1e-12, atol=1e-14 * np.abs(ref).max())
+
+
+def test_factor_blocks_are_block_thomas(oracle):
+    """D'_k = D_k - A_k D'_{k-1}^{-1} C_{k-1} and DinvC[e_{k-1}] = D'_{k-1}^{-1} C_{k-1} (dense algebra)."""
+    nv = 3
+    m, jac = system(2, 5, nv, seed=5, shift=6.0)
+    Dia, DinvC = oracle.lineLU(m, jac["jacD"], jac["jacO1"], jac["jacO2"])
+    M = cf.assemble_M(m, jac).toarray()
+    blk = lambda a, e: a[nv * nv * e: nv * nv * (e + 1)].reshape(nv, nv, order="F")
+    for l in range(m.nline):
+        Dp = None
+        for k in range(m.linesize[l]):
+            e = m.lines[m.linepoint[l] + k]
+            D = M[nv * e: nv * e + nv, nv * e: nv * e + nv]
+            if k == 0:
+                Dp = D.copy()
+            else:
+                ep = m.lines[m.linepoint[l] + k - 1]
+                Ak = M[nv * e: nv * e + nv, nv * ep: nv * ep + nv]
+                Cp = M[nv * ep: nv * ep + nv, nv * e: nv * e + nv]
+                G = np.linalg.solve(Dp, Cp)
+                assert np.allclose(blk(DinvC, ep), G, atol=1e-13)
+                Dp = D - Ak @ G
+            F = blk(Dia, e)
+            assert np.allclose((np.tril(F, -1) + np.eye(nv)) @ np.triu(F), Dp, atol=1e-13)
+
+
+@pytest.mark.parametrize("NL,NK,nv", [(3, 5, 9), (60, 64, 9), (5, 8, 4)])
+def test_lineRes_equals_sparse_product(oracle, NL, NK, nv):
+    """lineRes with R preset to rhs gives rhs + J U (scipy assembly of the full J)."""
+    m, jac = system(NL, NK, nv, seed=2)
+    U = np.random.default_rng(2).uniform(-1, 1, nv * m.nelem)
+    R = oracle.lineRes(m, jac["jacD"], jac["jacO1"], jac["jacO2"], U, jac["rhs"])
+    ref = jac["rhs"] + cf.assemble_J(m, jac) @ U
+    assert np.allclose(R, ref, rtol=1e-13, atol=1e-14 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("NL,NK,nv,shift", [(60, 64, 9, 20.0), (30, 40, 9, 12.0), (6, 7, 3, 6.0)])
+def test_loop_history_equals_closed_form(oracle, NL, NK, nv, shift):
+    """src/main.cxx:197-229 loop: residual history equals the independent LIJ iteration."""
+    m, jac = system(NL, NK, nv, seed=1, shift=shift)
+    out = oracle.lij_run(m, jac, 10)
+    h_ref, U_ref, _ = cf.lij_closed_form(m, jac, 10)
+    ok, worst = cf.history_close(out["hist"], h_ref)
+    assert ok, worst
+    assert np.linalg.norm(out["U"] - U_ref) <= 1e-12 * np.linalg.norm(U_ref)
+
+
+def test_scalar_nvar1_is_textbook_thomas(oracle):
+    """NVAR = 1: block Thomas reduces to the scalar Thomas algorithm."""
+    m, jac = system(3, 17, 1, seed=7, shift=3.0)
+    Dia, DinvC = oracle.lineLU(m, jac["jacD"], jac["jacO1"], jac["jacO2"])
+    Aout = oracle.assembleTriBlocks(m, jac["jacO1"], jac["jacO2"])
+    R = np.random.default_rng(3).uniform(-1, 1, m.nelem)
+    dU = oracle.solveDU(m, Dia, DinvC, Aout, R)
+    M = cf.assemble_M(m, jac).toarray()
+    for l in range(m.nline):
+        idx = m.lines[m.linepoint[l]: m.linepoint[l] + m.linesize[l]]
+        sub = M[np.ix_(idx, idx)]
+        a = np.concatenate([[0.0], np.diag(sub, -1)])
+        c = np.concatenate([np.diag(sub, 1), [0.0]])
+        x = cf.thomas_scalar(a, np.diag(sub), c, -R[idx])
+        assert np.allclose(dU[idx], x, rtol=1e-13, atol=1e-15)
+
+
+def test_length_one_lines_are_point_jacobi(oracle):
+    """Every element its own line: dU = -D^{-1} R ([PAPER p.10 eq. 5] point-implicit Jacobi)."""
+    import synth_np
+    from triblocksolve_amd.solver import HostMesh
+    nv = 4
+    base = synth_np.structured(3, 4)
+    n = 12
+    mesh = HostMesh(nvar=nv, epoint=base["epoint"], ef=base["ef"], fc=base["fc"],
+                    linesize=np.ones(n, np.int32), linepoint=np.arange(n + 1, dtype=np.int32),
+                    lines=np.arange(n, dtype=np.int32), lineface=np.full(n, -2, np.int32))
+    jac = synth_np.generate(9, nv, n, mesh.nintface, 5.0)
+    Dia, DinvC = oracle.lineLU(mesh, jac["jacD"], jac["jacO1"], jac["jacO2"])
+    R = np.random.default_rng(4).uniform(-1, 1, nv * n)
+    dU = oracle.solveDU(mesh, Dia, DinvC, np.zeros_like(jac["jacD"]), R)
+    for e in range(n):
+        D = jac["jacD"][16 * e: 16 * e + 16].reshape(4, 4, order="F")
+        assert np.allclose(dU[4 * e: 4 * e + 4], -np.linalg.solve(D, R[4 * e: 4 * e + 4]), rtol=1e-13)
+
+
+def test_one_sweep_exact_without_offline_coupling(oracle):
+    """Zero cross-line blocks: J == M, so one LIJ sweep leaves only round-off."""
+    m, jac = system(12, 20, 9, seed=4)
+    fc = m.fc.reshape(-1, 2)
+    cross = np.nonzero(np.abs(fc[:, 0] - fc[:, 1]) != 1)[0]
+    for key in ("jacO1", "jacO2"):
+        blocks = jac[key].reshape(-1, 81)
+        blocks[cross] = 0.0
+    out = oracle.lij_run(m, jac, 2)
+    assert out["hist"][1] <= 1e-14 * out["hist"][0]
+
+
+def test_brute_force_error_propagation(oracle):
+    """Tiny grid: the error after k sweeps equals (I - M^{-1} J)^k e_0 (dense inverse)."""
+    nv = 2
+    m, jac = system(2, 3, nv, seed=6, shift=3.0)
+    J = cf.assemble_J(m, jac).toarray()
+    M = cf.assemble_M(m, jac).toarray()
+    x_star = np.linalg.solve(J, -jac["rhs"])
+    Gm = np.eye(J.shape[0]) - np.linalg.inv(M) @ J
+    for k in (1, 2, 3, 5):
+        out = oracle.lij_run(m, jac, k)
+        err = out["U"] - x_star
+        pred = np.linalg.matrix_power(Gm, k) @ (0 - x_star)
+        assert np.allclose(err, pred, atol=1e-13)
+
+
+def test_contraction_rate_matches_appendix(oracle):
+    """Shift 20 on the 60 x 64 grid contracts by about 0.012 per sweep (independent probe)."""
+    m, jac = system(60, 64, 9, seed=1, shift=20.0)
+    out = oracle.lij_run(m, jac, 5)
+    rates = out["hist"][1:4] / out["hist"][0:3]
+    assert np.all((rates > 0.005) & (rates < 0.03)), rates
+
+
+def test_openmp_equals_serial(oracle):
+    """Mode 4 (OpenMP over lines) is bitwise the serial mode 0 (no cross-line reductions)."""
+    m, jac = system(60, 64, 9, seed=1)
+    a = oracle.lij_run(m, jac, 4, nthreads=1)
+    b = oracle.lij_run(m, jac, 4, nthreads=4)
+    assert np.array_equal(a["U"], b["U"]) and np.array_equal(a["hist"], b["hist"])

@@ -1,0 +1,152 @@
+// ctx.hip -- device context (stream, device id, block size) and the shared
+// error / memory helpers.  Plays the role of the reference's Platform class
+// (include/Platform.hxx:25-92, src/Platform.cxx:11-78): it selects the device
+// and owns the launch queue; there is no JIT and no MPI.
+#include "tbs_common.hpp"
+
+#include <cstring>
+
+namespace tbs {
+
+static thread_local std::string g_last_error;
+
+void set_error(const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line) {
+    set_error("HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file, line);
+    return (int)e;
+}
+
+int use_ctx(tbs_ctx *ctx) {
+    TBS_REQUIRE(ctx != nullptr, TBS_ERR_ARG, "null context");
+    TBS_HIP(hipSetDevice(ctx->device));
+    return TBS_OK;
+}
+
+}  // namespace tbs
+
+extern "C" {
+
+const char *tbs_last_error(void) { return tbs::g_last_error.c_str(); }
+const char *tbs_version(void) { return "triblocksolve-mi355x 0.1 (gfx950)"; }
+
+int tbs_ctx_create(int device_id, int nvar, tbs_ctx **out) {
+    TBS_REQUIRE(out != nullptr, TBS_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    int supported[64];
+    int ns = tbs_supported_nvar(supported, 64);
+    bool ok = false;
+    for (int i = 0; i < ns; ++i) ok |= (supported[i] == nvar);
+    TBS_REQUIRE(ok, TBS_ERR_UNSUPPORTED, "block size nvar=%d is not built into this library", nvar);
+    int ndev = 0;
+    TBS_HIP(hipGetDeviceCount(&ndev));
+    TBS_REQUIRE(device_id >= 0 && device_id < ndev, TBS_ERR_ARG,
+                "device_id %d out of range (%d devices)", device_id, ndev);
+    TBS_HIP(hipSetDevice(device_id));
+    tbs_ctx *c = new tbs_ctx();
+    c->device = device_id;
+    c->nvar = nvar;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; return tbs::hip_fail(e, "hipStreamCreate", __FILE__, __LINE__); }
+    c->own_stream = true;
+    e = hipHostMalloc((void **)&c->red_host, sizeof(double), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return tbs::hip_fail(e, "hipHostMalloc", __FILE__, __LINE__);
+    }
+    *out = c;
+    return TBS_OK;
+}
+
+int tbs_ctx_destroy(tbs_ctx *ctx) {
+    if (!ctx) return TBS_OK;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    if (ctx->red_buf) hipFree(ctx->red_buf);
+    if (ctx->red_host) hipHostFree(ctx->red_host);
+    if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return TBS_OK;
+}
+
+int tbs_ctx_set_stream(tbs_ctx *ctx, void *hip_stream) {
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    if (ctx->own_stream && ctx->stream) {
+        TBS_HIP(hipStreamSynchronize(ctx->stream));
+        TBS_HIP(hipStreamDestroy(ctx->stream));
+    }
+    ctx->stream = (hipStream_t)hip_stream;
+    ctx->own_stream = false;
+    return TBS_OK;
+}
+
+void *tbs_ctx_stream(tbs_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int tbs_ctx_synchronize(tbs_ctx *ctx) {
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    TBS_HIP(hipStreamSynchronize(ctx->stream));
+    return TBS_OK;
+}
+
+int tbs_ctx_nvar(tbs_ctx *ctx) { return ctx ? ctx->nvar : TBS_ERR_ARG; }
+
+int tbs_malloc(tbs_ctx *ctx, size_t bytes, void **dptr) {
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    TBS_REQUIRE(dptr != nullptr, TBS_ERR_ARG, "null output pointer");
+    *dptr = nullptr;
+    if (bytes == 0) return TBS_OK;
+    hipError_t e = hipMalloc(dptr, bytes);
+    if (e == hipErrorOutOfMemory) {
+        hipGetLastError();
+        tbs::set_error("device allocation of %zu bytes refused", bytes);
+        return TBS_ERR_NOMEM;
+    }
+    TBS_HIP(e);
+    return TBS_OK;
+}
+
+int tbs_free(tbs_ctx *ctx, void *dptr) {
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    if (dptr) TBS_HIP(hipFree(dptr));
+    return TBS_OK;
+}
+
+int tbs_memcpy_h2d(tbs_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    if (bytes == 0) return TBS_OK;
+    TBS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    TBS_HIP(hipStreamSynchronize(ctx->stream));
+    return TBS_OK;
+}
+
+int tbs_memcpy_d2h(tbs_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    if (bytes == 0) return TBS_OK;
+    TBS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    TBS_HIP(hipStreamSynchronize(ctx->stream));
+    return TBS_OK;
+}
+
+int tbs_memset(tbs_ctx *ctx, void *dst, int value, size_t bytes) {
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    if (bytes == 0) return TBS_OK;
+    TBS_HIP(hipMemsetAsync(dst, value, bytes, ctx->stream));
+    return TBS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1189 @@
+// lij_fast.hip -- the fused LIJ sweep path ("plan" / "batch").
+//
+// One LIJ sweep of the reference (src/main.cxx:207-229) is
+//     dU = -M^-1 R  (Thomas per line),  U += dU,  R = rhs + J U.
+// Here it is two kernels plus a tiny norm/convergence kernel, captured in a
+// hipGraph:
+//   fres_kernel   residual R = rhs + D U + sum_faces Off U(nb) for a 64-element
+//                 group per wave (thread per element, SoA-64 block layout, all
+//                 block loads 512 B coalesced), the per-group sum of R^2, and
+//                 the Thomas right-hand side g = D'^-1 (-R) (W = D'^-1 is a
+//                 factor).  R itself is only written when asked for.
+//   fthomas_kernel one wave per line.  Forward: d_k = g_k - E_k d_{k-1} with
+//                 E_k = D'_k^-1 A_k; backward: d_k = d'_k - Gamma_k d_{k+1}
+//                 with Gamma_k = D'_k^-1 C_k (the reference's DinvC), and
+//                 U_k += d_k.  A window of SPW = floor(64/NV) consecutive steps
+//                 is spread over the wave, lane = step*NV + row; each step's
+//                 9x9 matvec uses the previous step's vector as uniform SGPR
+//                 operands (v_readlane), so the per-step dependency chain is one
+//                 FMA column plus the broadcast.  Factor windows are stored
+//                 [window][column j][lane], so each load is contiguous.
+//   ffinal_kernel per system: fixed-order sum of the group partials -> ||R||,
+//                 history, convergence / divergence flags.
+// These are algebraically the reference's sweep; rounding differs from the
+// reference's solveLU-per-step form by O(eps * cond(D')) because the factors
+// are applied as D'^-1 products (computed from the reference's LU factors by
+// its own solveLU).
+//
+// Many independent systems (a batch) run in one launch per kernel through
+// descriptor tables: lines -> (system, m0, n, window offset), groups ->
+// (system, group).  Every per-system quantity is computed from that system's
+// data alone and reduced in a fixed order, so a system's result is bitwise
+// the same alone and inside any batch.
+#include "tbs_common.hpp"
+#include "lane_ops.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+
+namespace {
+
+using tbs::rdlane;
+
+constexpr int kGroup = 64;      // elements per residual group (one wave)
+constexpr int kHistCap = 4096;  // sweeps of history kept per system
+
+template <typename TF>
+struct SysDev {
+    const double *blk;   // [ngroups][F+1][BS][64] residual blocks (slot 0 = D)
+    const int *nbr;      // [ngroups][F][64] neighbour internal index or -1
+    const TF *W;         // [ngroups][BS][64] D'^-1
+    const TF *EF;        // [windows][NV][SPW*NV] E_k rows, zero padded
+    const TF *GB;        // [windows][NV][SPW*NV] Gamma_k rows, zero padded
+    const double *rhs;   // [n][NV] internal order
+    double *U;           // [n][NV]
+    TF *g;               // [n][NV] Thomas right-hand side
+    TF *dUp;             // [n][NV] forward-sweep intermediate
+    double *R;           // [n][NV] residual (written on request)
+    double *gpart;       // [ngroups] sum of R^2 per group
+    double *hist;        // [kHistCap]
+    int *state;          // [0] = sweeps recorded, [1] = active, [2] = status
+    int n, ngroups, F, pad;
+};
+
+// ----------------------------------------------------------- residual --
+template <int NV, typename TF>
+__global__ __launch_bounds__(256) void fres_kernel(const SysDev<TF> *__restrict__ sys,
+                                                   const int2 *__restrict__ gtab, int ng,
+                                                   int want_R, int force) {
+    constexpr int BS = NV * NV;
+    const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wid >= ng) return;
+    const int lane = threadIdx.x & 63;
+    const int2 sg = gtab[wid];
+    const SysDev<TF> &S = sys[sg.x];
+    if (!force && !S.state[1]) return;
+    const int g = sg.y;
+    const int m = g * kGroup + lane;
+    const bool valid = m < S.n;
+    const int F = S.F;
+    const double *B = S.blk + (size_t)g * (F + 1) * BS * kGroup + lane;
+
+    double u[NV], acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) u[j] = valid ? S.U[(size_t)NV * m + j] : 0.0;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) acc[i] += B[(size_t)(i + NV * j) * kGroup] * u[j];
+    }
+    for (int f = 0; f < F; ++f) {
+        const int nb = S.nbr[((size_t)g * F + f) * kGroup + lane];
+        if (nb >= 0) {
+            const double *O = B + (size_t)(1 + f) * BS * kGroup;
+            double un[NV], t[NV];
+#pragma unroll
+            for (int j = 0; j < NV; ++j) un[j] = S.U[(size_t)NV * nb + j];
+#pragma unroll
+            for (int i = 0; i < NV; ++i) t[i] = 0.0;
+#pragma unroll
+            for (int j = 0; j < NV; ++j) {
+#pragma unroll
+                for (int i = 0; i < NV; ++i) t[i] += O[(size_t)(i + NV * j) * kGroup] * un[j];
+            }
+#pragma unroll
+            for (int i = 0; i < NV; ++i) acc[i] += t[i];
+        }
+    }
+    double r[NV];
+    double rr = 0.0;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        r[i] = valid ? S.rhs[(size_t)NV * m + i] + acc[i] : 0.0;
+        rr += r[i] * r[i];
+    }
+    const TF *W = S.W + (size_t)g * BS * kGroup + lane;
+    double gv[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) gv[i] = 0.0;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) gv[i] += (double)W[(size_t)(i + NV * j) * kGroup] * (-r[j]);
+    }
+    if (valid) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) S.g[(size_t)NV * m + i] = (TF)gv[i];
+        if (want_R) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) S.R[(size_t)NV * m + i] = r[i];
+        }
+    }
+    rr = tbs::wave_sum(rr);
+    if (lane == 0) S.gpart[g] = rr;
+}
+
+// -------------------------------------------------------------- Thomas --
+template <int NV, typename TF>
+__global__ __launch_bounds__(64) void fthomas_kernel(const SysDev<TF> *__restrict__ sys,
+                                                     const int4 *__restrict__ ltab, int nl) {
+    constexpr int SPW = 64 / NV;      // steps per window
+    constexpr int LW = SPW * NV;      // lanes carrying data
+    const int l = blockIdx.x;
+    if (l >= nl) return;
+    const int4 lt = ltab[l];          // {system, m0, n, window offset}
+    const SysDev<TF> &S = sys[lt.x];
+    if (!S.state[1]) return;
+    const int lane = threadIdx.x;
+    const int st_of_lane = lane / NV;
+    const int row = lane - st_of_lane * NV;
+    const bool lane_ok = lane < LW;
+    const int m0 = lt.y, n = lt.z;
+    const int nw = (n + SPW - 1) / SPW;
+    const TF *EF = S.EF + (size_t)lt.w * NV * LW;
+    const TF *GB = S.GB + (size_t)lt.w * NV * LW;
+
+    // ---- forward: d_k = g_k - E_k d_{k-1}
+    TF d[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) d[j] = TF(0);
+    TF ec[NV], en[NV];
+    TF gc, gn;
+    auto load_f = [&](int w, TF (&e)[NV], TF &gg) {
+        const int k = w * SPW + st_of_lane;
+        const bool ok = lane_ok && k < n;
+#pragma unroll
+        for (int j = 0; j < NV; ++j) e[j] = lane_ok ? EF[((size_t)w * NV + j) * LW + lane] : TF(0);
+        gg = ok ? S.g[(size_t)NV * (m0 + k) + row] : TF(0);
+    };
+    load_f(0, ec, gc);
+    for (int w = 0; w < nw; ++w) {
+        if (w + 1 < nw) load_f(w + 1, en, gn);
+        TF out = TF(0);
+#pragma unroll
+        for (int st = 0; st < SPW; ++st) {
+            TF x = TF(0);
+#pragma unroll
+            for (int j = 0; j < NV; ++j) x += ec[j] * d[j];
+            const TF v = gc - x;
+#pragma unroll
+            for (int j = 0; j < NV; ++j) d[j] = rdlane(v, st * NV + j);
+            out = (st_of_lane == st) ? v : out;
+        }
+        const int k = w * SPW + st_of_lane;
+        if (lane_ok && k < n) S.dUp[(size_t)NV * (m0 + k) + row] = out;
+        if (w + 1 < nw) {
+#pragma unroll
+            for (int j = 0; j < NV; ++j) ec[j] = en[j];
+            gc = gn;
+        }
+    }
+    __threadfence_block();
+
+    // ---- backward: d_k = d'_k - Gamma_k d_{k+1};  U_k += d_k
+#pragma unroll
+    for (int j = 0; j < NV; ++j) d[j] = TF(0);
+    TF pc, pn;
+    double uc, un;
+    auto load_b = [&](int w, TF (&e)[NV], TF &pp, double &uu) {
+        const int k = w * SPW + st_of_lane;
+        const bool ok = lane_ok && k < n;
+#pragma unroll
+        for (int j = 0; j < NV; ++j) e[j] = lane_ok ? GB[((size_t)w * NV + j) * LW + lane] : TF(0);
+        pp = ok ? S.dUp[(size_t)NV * (m0 + k) + row] : TF(0);
+        uu = ok ? S.U[(size_t)NV * (m0 + k) + row] : 0.0;
+    };
+    load_b(nw - 1, ec, pc, uc);
+    for (int w = nw - 1; w >= 0; --w) {
+        if (w > 0) load_b(w - 1, en, pn, un);
+        TF out = TF(0);
+#pragma unroll
+        for (int st = SPW - 1; st >= 0; --st) {
+            TF x = TF(0);
+#pragma unroll
+            for (int j = 0; j < NV; ++j) x += ec[j] * d[j];
+            const TF v = pc - x;
+#pragma unroll
+            for (int j = 0; j < NV; ++j) d[j] = rdlane(v, st * NV + j);
+            out = (st_of_lane == st) ? v : out;
+        }
+        const int k = w * SPW + st_of_lane;
+        if (lane_ok && k < n) S.U[(size_t)NV * (m0 + k) + row] = uc + (double)out;
+        if (w > 0) {
+#pragma unroll
+            for (int j = 0; j < NV; ++j) ec[j] = en[j];
+            pc = pn;
+            uc = un;
+        }
+    }
+}
+
+// ------------------------------------------------ norm / convergence --
+// One block per system: fixed-order sum of the group partials.
+template <typename TF>
+__global__ __launch_bounds__(256) void ffinal_kernel(const SysDev<TF> *__restrict__ sys, int nsys,
+                                                     int max_sweeps, double rtol, int force) {
+    __shared__ double s[256];
+    const int sidx = blockIdx.x;
+    if (sidx >= nsys) return;
+    const SysDev<TF> &S = sys[sidx];
+    if (!force && !S.state[1]) return;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < S.ngroups; i += 256) acc += S.gpart[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double h = sqrt(s[0]);
+        const int k = S.state[0];
+        if (k < kHistCap) S.hist[k] = h;
+        S.state[0] = k + 1;
+        int active = 1, status = 0;
+        if (!(h == h) || h > 1.0e300) { active = 0; status = 2; }          // non-finite
+        else if (k >= 1 && h <= rtol * S.hist[0]) { active = 0; status = 1; }  // converged
+        else if (k >= max_sweeps || k + 1 >= kHistCap) { active = 0; status = 3; }
+        else if (k >= 8 && h > 1.0e6 * S.hist[0]) { active = 0; status = 4; }  // diverging
+        S.state[1] = active;
+        S.state[2] = status;
+    }
+}
+
+template <typename TF>
+__global__ void reset_state_kernel(const SysDev<TF> *__restrict__ sys, int nsys) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nsys) {
+        sys[i].state[0] = 0;
+        sys[i].state[1] = 1;
+        sys[i].state[2] = 0;
+    }
+}
+
+// ------------------------------------------------------------ packing --
+// Residual blocks into SoA-64: src code per (internal element, slot):
+//   x = array (0 none, 1 jacD, 2 jacO1, 3 jacO2), y = block index.
+template <int NV>
+__global__ void pack_blocks_kernel(long total, int S, const int2 *__restrict__ src,
+                                   const double *__restrict__ D, const double *__restrict__ O1,
+                                   const double *__restrict__ O2, double *__restrict__ blk) {
+    constexpr int BS = NV * NV;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long)gridDim.x * blockDim.x) {
+        // t enumerates the destination in order -> coalesced writes
+        const int lane = (int)(t % kGroup);
+        const long r1 = t / kGroup;
+        const int q = (int)(r1 % BS);
+        const long r2 = r1 / BS;
+        const int slot = (int)(r2 % S);
+        const long g = r2 / S;
+        const long m = g * kGroup + lane;
+        const int2 c = src[m * S + slot];
+        double v = 0.0;
+        if (c.x == 1) v = D[(size_t)BS * c.y + q];
+        else if (c.x == 2) v = O1[(size_t)BS * c.y + q];
+        else if (c.x == 3) v = O2[(size_t)BS * c.y + q];
+        blk[t] = v;
+    }
+}
+
+// Sweep-form factors from the LU factors D' (reference layout) and DinvC:
+// thread per (internal element m, column c):  W(:,c) = solveLU(D', e_c),
+// E(:,c) = solveLU(D', A(:,c)) for k >= 1, Gamma(:,c) = DinvC(:,c) for
+// k <= n-2.  solveLU is src/okl/linesmoothLU_TriBlock_v3.okl:23-45.
+template <int NV, typename TF>
+__global__ void invform_kernel(int nle, const int *__restrict__ lines,
+                               const int *__restrict__ mline, const int *__restrict__ linepoint,
+                               const int *__restrict__ linesize, const int *__restrict__ woff,
+                               const double *__restrict__ Dp, const double *__restrict__ A,
+                               const double *__restrict__ DinvC, TF *__restrict__ W,
+                               TF *__restrict__ EF, TF *__restrict__ GB) {
+    constexpr int BS = NV * NV;
+    constexpr int SPW = 64 / NV;
+    constexpr int LW = SPW * NV;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nle * NV) return;
+    const int m = (int)(t / NV), c = (int)(t % NV);
+    const int e = lines[m];
+    const int l = mline[m];
+    const int k = m - linepoint[l];
+    const int n = linesize[l];
+    const double *LU = Dp + (size_t)BS * e;
+    double Lr[BS];
+#pragma unroll
+    for (int q = 0; q < BS; ++q) Lr[q] = LU[q];
+    auto solve = [&](const double (&b)[NV], double (&x)[NV]) {
+        double y[NV];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            double tot = 0.0;
+#pragma unroll
+            for (int j = 0; j < i; ++j) tot += Lr[i + NV * j] * y[j];
+            y[i] = b[i] - tot;
+        }
+#pragma unroll
+        for (int i = NV - 1; i >= 0; --i) {
+            double tot = 0.0;
+#pragma unroll
+            for (int j = i + 1; j < NV; ++j) tot += Lr[i + NV * j] * x[j];
+            x[i] = (y[i] - tot) / Lr[i + NV * i];
+        }
+    };
+    double b[NV], x[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) b[i] = (i == c) ? 1.0 : 0.0;
+    solve(b, x);
+    const int g = m / kGroup, lane = m % kGroup;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) W[((size_t)g * BS + i + NV * c) * kGroup + lane] = (TF)x[i];
+    const int w = k / SPW, st = k % SPW;
+    const size_t wbase = ((size_t)(woff[l] + w) * NV + c) * LW + (size_t)st * NV;
+    if (k >= 1) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) b[i] = A[(size_t)BS * e + i + NV * c];
+        solve(b, x);
+#pragma unroll
+        for (int i = 0; i < NV; ++i) EF[wbase + i] = (TF)x[i];
+    }
+    if (k <= n - 2) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) GB[wbase + i] = (TF)DinvC[(size_t)BS * e + i + NV * c];
+    }
+}
+
+// Vector gathers between reference order (element id) and internal order.
+__global__ void gather_vec_kernel(long total, int nv, const int *__restrict__ lines,
+                                  const double *__restrict__ src, double *__restrict__ dst) {
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long)gridDim.x * blockDim.x) {
+        const long m = t / nv;
+        const int i = (int)(t % nv);
+        dst[t] = src ? src[(size_t)nv * lines[m] + i] : 0.0;
+    }
+}
+__global__ void scatter_vec_kernel(long total, int nv, const int *__restrict__ lines,
+                                   const double *__restrict__ src, double *__restrict__ dst) {
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long)gridDim.x * blockDim.x) {
+        const long m = t / nv;
+        const int i = (int)(t % nv);
+        dst[(size_t)nv * lines[m] + i] = src[t];
+    }
+}
+__global__ void zero_kernel(long n, double *__restrict__ p) {
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+        p[t] = 0.0;
+}
+
+unsigned gs_grid(long n) {
+    long g = (n + 255) / 256;
+    return (unsigned)std::max(1L, std::min(g, 65536L));
+}
+
+}  // namespace
+
+// ======================================================================
+// Host side
+// ======================================================================
+
+struct tbs_plan {
+    tbs_ctx *ctx = nullptr;
+    int nv = 9, prec = TBS_FP64;
+    int nelem = 0, nintface = 0, nline = 0, nle = 0, F = 0, S = 1, ngroups = 0, nwin = 0;
+    bool factored = false;
+    std::vector<int> h_linesize, h_linepoint, h_woff;
+    // device: topology
+    int *d_lines = nullptr, *d_mline = nullptr, *d_linepoint = nullptr, *d_linesize = nullptr;
+    int *d_woff = nullptr;
+    int *d_fc = nullptr, *d_lineface = nullptr;  // for lineLU / assemble
+    int2 *d_src = nullptr;                       // packing source codes
+    // device: data
+    double *d_blk = nullptr;
+    int *d_nbr = nullptr;
+    void *d_W = nullptr, *d_EF = nullptr, *d_GB = nullptr;
+    double *d_rhs = nullptr, *d_U = nullptr, *d_R = nullptr, *d_gpart = nullptr, *d_hist = nullptr;
+    void *d_g = nullptr, *d_dUp = nullptr;
+    int *d_state = nullptr;
+    // the plan as a one-system batch
+    void *d_sys = nullptr;
+    int4 *d_ltab = nullptr;
+    int2 *d_gtab = nullptr;
+    std::map<int, hipGraphExec_t> graphs;
+    int max_sweeps = kHistCap - 1;
+    double rtol = 0.0;
+    size_t bytes = 0;
+    int *h_state = nullptr;  // pinned
+};
+
+struct tbs_batch {
+    tbs_ctx *ctx = nullptr;
+    std::vector<tbs_plan *> plans;
+    int nv = 9, prec = TBS_FP64;
+    void *d_sys = nullptr;
+    int4 *d_ltab = nullptr;
+    int2 *d_gtab = nullptr;
+    int nl = 0, ng = 0;
+    std::map<std::pair<int, std::pair<int, double>>, hipGraphExec_t> graphs;
+    int *h_state = nullptr;
+};
+
+namespace {
+
+template <typename TF>
+SysDev<TF> make_sysdev(const tbs_plan *p) {
+    SysDev<TF> s{};
+    s.blk = p->d_blk;
+    s.nbr = p->d_nbr;
+    s.W = (const TF *)p->d_W;
+    s.EF = (const TF *)p->d_EF;
+    s.GB = (const TF *)p->d_GB;
+    s.rhs = p->d_rhs;
+    s.U = p->d_U;
+    s.g = (TF *)p->d_g;
+    s.dUp = (TF *)p->d_dUp;
+    s.R = p->d_R;
+    s.gpart = p->d_gpart;
+    s.hist = p->d_hist;
+    s.state = p->d_state;
+    s.n = p->nle;
+    s.ngroups = p->ngroups;
+    s.F = p->F;
+    return s;
+}
+
+size_t tf_size(int prec) { return prec == TBS_FP32 ? sizeof(float) : sizeof(double); }
+
+int check_mesh(const tbs_mesh_host *m) {
+    TBS_REQUIRE(m != nullptr, TBS_ERR_ARG, "null mesh");
+    TBS_REQUIRE(m->nelem >= 1 && m->nline >= 1 && m->nintface >= 0 && m->eftot >= 0, TBS_ERR_ARG,
+                "bad mesh sizes");
+    TBS_REQUIRE(m->epoint && m->ef && m->linesize && m->linepoint && m->lines && m->lineface,
+                TBS_ERR_ARG, "null mesh array");
+    TBS_REQUIRE(m->nintface == 0 || m->fc, TBS_ERR_ARG, "null fc");
+    TBS_REQUIRE(m->nlineelem == m->nelem, TBS_ERR_ARG,
+                "the fused path needs every element in exactly one line (nlineelem %d != nelem %d)",
+                m->nlineelem, m->nelem);
+    TBS_REQUIRE(m->linepoint[0] == 0, TBS_ERR_ARG, "linepoint[0] must be 0");
+    for (int l = 0; l < m->nline; ++l)
+        TBS_REQUIRE(m->linesize[l] >= 1 && m->linepoint[l + 1] == m->linepoint[l] + m->linesize[l],
+                    TBS_ERR_ARG, "line %d: linepoint/linesize not contiguous", l);
+    TBS_REQUIRE(m->linepoint[m->nline] == m->nlineelem, TBS_ERR_ARG, "linepoint[nline] != nlineelem");
+    std::vector<char> seen(m->nelem, 0);
+    for (int i = 0; i < m->nlineelem; ++i) {
+        const int e = m->lines[i];
+        TBS_REQUIRE(e >= 0 && e < m->nelem && !seen[e], TBS_ERR_ARG, "lines[] is not a permutation");
+        seen[e] = 1;
+    }
+    TBS_REQUIRE(m->epoint[0] == 0 && m->epoint[m->nelem] == m->eftot, TBS_ERR_ARG, "bad epoint");
+    for (int e = 0; e < m->nelem; ++e)
+        TBS_REQUIRE(m->epoint[e + 1] >= m->epoint[e], TBS_ERR_ARG, "epoint not monotone");
+    for (int k = 0; k < m->eftot; ++k)
+        TBS_REQUIRE(m->ef[k] < m->nintface, TBS_ERR_ARG, "ef[%d] out of range", k);
+    for (int f = 0; f < 2 * m->nintface; ++f)
+        TBS_REQUIRE(m->fc[f] >= 0 && m->fc[f] < m->nelem, TBS_ERR_ARG, "fc[%d] out of range", f);
+    for (int l = 0; l < m->nline; ++l)
+        for (int k = 1; k < m->linesize[l]; ++k) {
+            const int mm = m->linepoint[l] + k;
+            const int f = m->lineface[mm];
+            TBS_REQUIRE(f >= 0 && f < m->nintface, TBS_ERR_ARG, "lineface[%d] not an interior face", mm);
+            const int e = m->lines[mm], ep = m->lines[mm - 1];
+            const int a = m->fc[2 * f], b = m->fc[2 * f + 1];
+            TBS_REQUIRE((a == e && b == ep) || (a == ep && b == e), TBS_ERR_ARG,
+                        "lineface[%d] does not join consecutive line elements", mm);
+        }
+    return TBS_OK;
+}
+
+struct Layout {
+    int F, S, ngroups, nwin;
+    std::vector<int> woff;
+};
+
+Layout layout_of(const tbs_mesh_host *m) {
+    Layout L;
+    int F = 0;
+    for (int e = 0; e < m->nelem; ++e) F = std::max(F, m->epoint[e + 1] - m->epoint[e]);
+    L.F = F;
+    L.S = F + 1;
+    L.ngroups = (m->nlineelem + kGroup - 1) / kGroup;
+    const int spw = 64 / m->nvar;
+    L.woff.resize(m->nline + 1);
+    int w = 0;
+    for (int l = 0; l < m->nline; ++l) {
+        L.woff[l] = w;
+        w += (m->linesize[l] + spw - 1) / spw;
+    }
+    L.woff[m->nline] = w;
+    L.nwin = w;
+    return L;
+}
+
+size_t footprint_of(const tbs_mesh_host *m, const Layout &L, int prec) {
+    const size_t nv = m->nvar, bs = nv * nv, tf = tf_size(prec);
+    const size_t spw = 64 / nv, lw = spw * nv;
+    const size_t G = (size_t)L.ngroups * kGroup;
+    size_t b = 0;
+    b += G * L.S * bs * 8;                // residual blocks
+    b += G * L.F * 4;                     // neighbour table
+    b += G * bs * tf;                     // W
+    b += 2 * (size_t)L.nwin * nv * lw * tf;  // E, Gamma windows
+    b += (size_t)m->nlineelem * nv * (8 * 3 + 2 * tf);  // rhs U R g dUp
+    b += (size_t)L.ngroups * 8 + kHistCap * 8 + 64;
+    b += (size_t)m->nlineelem * 4 * 2 + (size_t)(m->nline + 1) * 4 * 3;
+    b += (size_t)m->nline * 16 + (size_t)L.ngroups * 8;  // tables
+    return b;
+}
+
+// Algorithmic bytes of one sweep, BASELINE model: factor-side D', A, DinvC
+// (tf bytes each), D + Of1 + Of2 at FP64, 6 vector passes at FP64.
+double model_bytes(const tbs_mesh_host *m, int prec) {
+    const double bs = (double)m->nvar * m->nvar, nv = m->nvar;
+    const double tf = (double)tf_size(prec);
+    return tf * bs * (3.0 * m->nelem - 2.0 * m->nline) + 8.0 * bs * (m->nelem + 2.0 * m->nintface) +
+           6.0 * 8.0 * nv * m->nelem;
+}
+
+// The reference's own per-sweep byte model, src/main.cxx:149-175 / iters.
+double reference_model_bytes(const tbs_mesh_host *m) {
+    const double nv = m->nvar, bs = nv * nv;
+    const double du = 8.0 * (bs * m->nelem + bs * m->nelem + bs * m->nlineelem + 4.0 * nv * m->nelem +
+                             nv * m->nelem) +
+                      4.0 * (m->nline + (m->nline + 1.0) + 2.0 * m->nlineelem);
+    const double cp = 5.0 * nv * m->nelem * 8.0;
+    const double lr = 8.0 * (bs * m->nelem + 2.0 * bs * m->nintface + 5.0 * m->nelem * nv +
+                             2.0 * nv * m->nelem) +
+                      4.0 * ((m->nelem + 1.0) + m->eftot + 2.0 * m->nintface + m->nline +
+                             (m->nline + 1.0) + 2.0 * m->nlineelem);
+    return du + cp + lr;
+}
+
+template <typename T>
+int dalloc(tbs_plan *p, T **ptr, size_t count) {
+    *ptr = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)ptr, count * sizeof(T));
+    if (e == hipErrorOutOfMemory) {
+        hipGetLastError();
+        tbs::set_error("plan allocation of %zu bytes refused", count * sizeof(T));
+        return TBS_ERR_NOMEM;
+    }
+    TBS_HIP(e);
+    p->bytes += count * sizeof(T);
+    return TBS_OK;
+}
+
+#define TBS_TRY(x) do { int _rc = (x); if (_rc) return _rc; } while (0)
+
+void free_plan(tbs_plan *p) {
+    if (!p) return;
+    hipSetDevice(p->ctx->device);
+    for (auto &kv : p->graphs) hipGraphExecDestroy(kv.second);
+    void *ptrs[] = {p->d_lines, p->d_mline, p->d_linepoint, p->d_linesize, p->d_woff, p->d_fc,
+                    p->d_lineface, p->d_src, p->d_blk, p->d_nbr, p->d_W, p->d_EF, p->d_GB,
+                    p->d_rhs, p->d_U, p->d_R, p->d_gpart, p->d_hist, p->d_g, p->d_dUp,
+                    p->d_state, p->d_sys, p->d_ltab, p->d_gtab};
+    for (void *q : ptrs)
+        if (q) hipFree(q);
+    if (p->h_state) hipHostFree(p->h_state);
+    delete p;
+}
+
+template <int NV, typename TF>
+int launch_residual(hipStream_t st, const void *sys, const int2 *gtab, int ng, int want_R, int force) {
+    if (ng == 0) return TBS_OK;
+    hipLaunchKernelGGL((fres_kernel<NV, TF>), dim3((ng + 3) / 4), dim3(256), 0, st,
+                       (const SysDev<TF> *)sys, gtab, ng, want_R, force);
+    TBS_LAUNCHED("fres_kernel");
+    return TBS_OK;
+}
+
+template <int NV, typename TF>
+int launch_thomas(hipStream_t st, const void *sys, const int4 *ltab, int nl) {
+    if (nl == 0) return TBS_OK;
+    hipLaunchKernelGGL((fthomas_kernel<NV, TF>), dim3(nl), dim3(64), 0, st,
+                       (const SysDev<TF> *)sys, ltab, nl);
+    TBS_LAUNCHED("fthomas_kernel");
+    return TBS_OK;
+}
+
+template <typename TF>
+int launch_final(hipStream_t st, const void *sys, int nsys, int max_sweeps, double rtol, int force) {
+    hipLaunchKernelGGL((ffinal_kernel<TF>), dim3(nsys), dim3(256), 0, st,
+                       (const SysDev<TF> *)sys, nsys, max_sweeps, rtol, force);
+    TBS_LAUNCHED("ffinal_kernel");
+    return TBS_OK;
+}
+
+// Dispatch on (nvar, precision) for the fused kernels.
+#define TBS_PLAN_NVARS(X) X(4) X(5) X(9)
+
+int k_residual(int nv, int prec, hipStream_t st, const void *sys, const int2 *gtab, int ng,
+               int want_R, int force) {
+    switch (nv) {
+#define X(v) case v: return prec == TBS_FP32 ? launch_residual<v, float>(st, sys, gtab, ng, want_R, force) \
+                                         : launch_residual<v, double>(st, sys, gtab, ng, want_R, force);
+        TBS_PLAN_NVARS(X)
+#undef X
+    }
+    tbs::set_error("nvar %d not built for the fused path", nv);
+    return TBS_ERR_UNSUPPORTED;
+}
+int k_thomas(int nv, int prec, hipStream_t st, const void *sys, const int4 *ltab, int nl) {
+    switch (nv) {
+#define X(v) case v: return prec == TBS_FP32 ? launch_thomas<v, float>(st, sys, ltab, nl) \
+                                         : launch_thomas<v, double>(st, sys, ltab, nl);
+        TBS_PLAN_NVARS(X)
+#undef X
+    }
+    tbs::set_error("nvar %d not built for the fused path", nv);
+    return TBS_ERR_UNSUPPORTED;
+}
+int k_final(int prec, hipStream_t st, const void *sys, int nsys, int max_sweeps, double rtol, int force) {
+    return prec == TBS_FP32 ? launch_final<float>(st, sys, nsys, max_sweeps, rtol, force)
+                            : launch_final<double>(st, sys, nsys, max_sweeps, rtol, force);
+}
+int k_reset(int prec, hipStream_t st, const void *sys, int nsys) {
+    if (prec == TBS_FP32)
+        hipLaunchKernelGGL((reset_state_kernel<float>), dim3((nsys + 63) / 64), dim3(64), 0, st,
+                           (const SysDev<float> *)sys, nsys);
+    else
+        hipLaunchKernelGGL((reset_state_kernel<double>), dim3((nsys + 63) / 64), dim3(64), 0, st,
+                           (const SysDev<double> *)sys, nsys);
+    TBS_LAUNCHED("reset_state_kernel");
+    return TBS_OK;
+}
+
+bool plan_nvar_ok(int nv) {
+    switch (nv) {
+#define X(v) case v: return true;
+        TBS_PLAN_NVARS(X)
+#undef X
+    }
+    return false;
+}
+
+// One sweep: Thomas (+U update), residual, norm/convergence.
+int enqueue_sweep(int nv, int prec, hipStream_t st, const void *sys, int nsys, const int4 *ltab,
+                  int nl, const int2 *gtab, int ng, int max_sweeps, double rtol) {
+    TBS_TRY(k_thomas(nv, prec, st, sys, ltab, nl));
+    TBS_TRY(k_residual(nv, prec, st, sys, gtab, ng, 0, 0));
+    TBS_TRY(k_final(prec, st, sys, nsys, max_sweeps, rtol, 0));
+    return TBS_OK;
+}
+
+int capture_sweeps(hipStream_t st, int nsweeps, hipGraphExec_t *out,
+                   const std::function<int()> &one_sweep) {
+    TBS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = TBS_OK;
+    for (int i = 0; i < nsweeps && rc == TBS_OK; ++i) rc = one_sweep();
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &graph);
+    if (rc) { if (graph) hipGraphDestroy(graph); return rc; }
+    TBS_HIP(e);
+    e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    TBS_HIP(e);
+    return TBS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tbs_plan_supported_nvar(int *out, int cap) {
+    int n = 0;
+#define X(v) if (out && n < cap) out[n] = v; ++n;
+    TBS_PLAN_NVARS(X)
+#undef X
+    return n;
+}
+
+int tbs_plan_footprint(const tbs_mesh_host *mesh, int precision, size_t *bytes) {
+    TBS_REQUIRE(bytes, TBS_ERR_ARG, "null output");
+    TBS_TRY(check_mesh(mesh));
+    TBS_REQUIRE(precision == TBS_FP64 || precision == TBS_FP32, TBS_ERR_ARG, "bad precision");
+    Layout L = layout_of(mesh);
+    *bytes = footprint_of(mesh, L, precision);
+    return TBS_OK;
+}
+
+int tbs_plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_plan **out) {
+    TBS_TRY(tbs::use_ctx(ctx));
+    TBS_REQUIRE(out, TBS_ERR_ARG, "null output");
+    *out = nullptr;
+    TBS_TRY(check_mesh(m));
+    TBS_REQUIRE(m->nvar == ctx->nvar, TBS_ERR_ARG, "mesh nvar %d != context nvar %d", m->nvar, ctx->nvar);
+    TBS_REQUIRE(plan_nvar_ok(m->nvar), TBS_ERR_UNSUPPORTED, "nvar %d not built for the fused path", m->nvar);
+    TBS_REQUIRE(precision == TBS_FP64 || precision == TBS_FP32, TBS_ERR_ARG, "bad precision");
+    const int nv = m->nvar, BS = nv * nv;
+    Layout L = layout_of(m);
+    std::unique_ptr<tbs_plan, void (*)(tbs_plan *)> p(new tbs_plan(), free_plan);
+    p->ctx = ctx;
+    p->nv = nv;
+    p->prec = precision;
+    p->nelem = m->nelem;
+    p->nintface = m->nintface;
+    p->nline = m->nline;
+    p->nle = m->nlineelem;
+    p->F = L.F;
+    p->S = L.S;
+    p->ngroups = L.ngroups;
+    p->nwin = L.nwin;
+    p->h_linesize.assign(m->linesize, m->linesize + m->nline);
+    p->h_linepoint.assign(m->linepoint, m->linepoint + m->nline + 1);
+    p->h_woff = L.woff;
+    const size_t G = (size_t)L.ngroups * kGroup;
+    const size_t tf = tf_size(precision);
+    const int spw = 64 / nv, lw = spw * nv;
+
+    // host tables
+    std::vector<int> mline(p->nle), inv(p->nelem, -1);
+    for (int l = 0; l < p->nline; ++l)
+        for (int k = 0; k < m->linesize[l]; ++k) mline[m->linepoint[l] + k] = l;
+    for (int i = 0; i < p->nle; ++i) inv[m->lines[i]] = i;
+    std::vector<int2> src(G * L.S, make_int2(0, 0));
+    std::vector<int> nbr(G * std::max(L.F, 1), -1);
+    for (int mm = 0; mm < p->nle; ++mm) {
+        const int e = m->lines[mm];
+        const int g = mm / kGroup, lane = mm % kGroup;
+        src[(size_t)mm * L.S] = make_int2(1, e);
+        int s = 0;
+        for (int k2 = m->epoint[e]; k2 < m->epoint[e + 1]; ++k2, ++s) {
+            const int f = m->ef[k2];
+            if (f < 0) continue;
+            const int e1 = m->fc[2 * f], e2 = m->fc[2 * f + 1];
+            src[(size_t)mm * L.S + 1 + s] = make_int2(e == e1 ? 3 : 2, f);
+            nbr[((size_t)g * L.F + s) * kGroup + lane] = inv[e == e1 ? e2 : e1];
+        }
+    }
+    std::vector<int4> ltab(p->nline);
+    for (int l = 0; l < p->nline; ++l) ltab[l] = make_int4(0, m->linepoint[l], m->linesize[l], L.woff[l]);
+    std::vector<int2> gtab(L.ngroups);
+    for (int g = 0; g < L.ngroups; ++g) gtab[g] = make_int2(0, g);
+
+    tbs_plan *P = p.get();
+    TBS_TRY(dalloc(P, &P->d_lines, P->nle));
+    TBS_TRY(dalloc(P, &P->d_mline, P->nle));
+    TBS_TRY(dalloc(P, &P->d_linepoint, P->nline + 1));
+    TBS_TRY(dalloc(P, &P->d_linesize, P->nline));
+    TBS_TRY(dalloc(P, &P->d_woff, P->nline + 1));
+    TBS_TRY(dalloc(P, &P->d_fc, std::max(2 * m->nintface, 1)));
+    TBS_TRY(dalloc(P, &P->d_lineface, P->nle));
+    TBS_TRY(dalloc(P, &P->d_src, src.size()));
+    TBS_TRY(dalloc(P, &P->d_blk, G * L.S * BS));
+    TBS_TRY(dalloc(P, &P->d_nbr, nbr.size()));
+    TBS_TRY(dalloc(P, (char **)&P->d_W, G * BS * tf));
+    TBS_TRY(dalloc(P, (char **)&P->d_EF, (size_t)L.nwin * nv * lw * tf));
+    TBS_TRY(dalloc(P, (char **)&P->d_GB, (size_t)L.nwin * nv * lw * tf));
+    TBS_TRY(dalloc(P, &P->d_rhs, (size_t)P->nle * nv));
+    TBS_TRY(dalloc(P, &P->d_U, (size_t)P->nle * nv));
+    TBS_TRY(dalloc(P, &P->d_R, (size_t)P->nle * nv));
+    TBS_TRY(dalloc(P, (char **)&P->d_g, (size_t)P->nle * nv * tf));
+    TBS_TRY(dalloc(P, (char **)&P->d_dUp, (size_t)P->nle * nv * tf));
+    TBS_TRY(dalloc(P, &P->d_gpart, L.ngroups));
+    TBS_TRY(dalloc(P, &P->d_hist, kHistCap));
+    TBS_TRY(dalloc(P, &P->d_state, 4));
+    TBS_TRY(dalloc(P, &P->d_ltab, ltab.size()));
+    TBS_TRY(dalloc(P, &P->d_gtab, gtab.size()));
+    TBS_TRY(dalloc(P, (char **)&P->d_sys, precision == TBS_FP32 ? sizeof(SysDev<float>) : sizeof(SysDev<double>)));
+    TBS_HIP(hipHostMalloc((void **)&P->h_state, 4 * sizeof(int), hipHostMallocDefault));
+
+    hipStream_t st = ctx->stream;
+    auto up = [&](void *dst, const void *s, size_t bytes) -> int {
+        TBS_HIP(hipMemcpyAsync(dst, s, bytes, hipMemcpyHostToDevice, st));
+        return TBS_OK;
+    };
+    TBS_TRY(up(P->d_lines, m->lines, P->nle * 4));
+    TBS_TRY(up(P->d_mline, mline.data(), P->nle * 4));
+    TBS_TRY(up(P->d_linepoint, m->linepoint, (P->nline + 1) * 4));
+    TBS_TRY(up(P->d_linesize, m->linesize, P->nline * 4));
+    TBS_TRY(up(P->d_woff, L.woff.data(), (P->nline + 1) * 4));
+    if (m->nintface) TBS_TRY(up(P->d_fc, m->fc, (size_t)2 * m->nintface * 4));
+    TBS_TRY(up(P->d_lineface, m->lineface, P->nle * 4));
+    TBS_TRY(up(P->d_src, src.data(), src.size() * sizeof(int2)));
+    TBS_TRY(up(P->d_nbr, nbr.data(), nbr.size() * 4));
+    TBS_TRY(up(P->d_ltab, ltab.data(), ltab.size() * sizeof(int4)));
+    TBS_TRY(up(P->d_gtab, gtab.data(), gtab.size() * sizeof(int2)));
+    if (precision == TBS_FP32) {
+        SysDev<float> s = make_sysdev<float>(P);
+        TBS_TRY(up(P->d_sys, &s, sizeof(s)));
+    } else {
+        SysDev<double> s = make_sysdev<double>(P);
+        TBS_TRY(up(P->d_sys, &s, sizeof(s)));
+    }
+    TBS_HIP(hipStreamSynchronize(st));   // host vectors above go out of scope
+    *out = p.release();
+    return TBS_OK;
+}
+
+int tbs_plan_destroy(tbs_plan *plan) {
+    if (!plan) return TBS_OK;
+    hipSetDevice(plan->ctx->device);
+    hipStreamSynchronize(plan->ctx->stream);
+    free_plan(plan);
+    return TBS_OK;
+}
+
+int tbs_plan_info_get(tbs_plan *p, tbs_plan_info *info) {
+    TBS_REQUIRE(p && info, TBS_ERR_ARG, "null argument");
+    tbs_mesh_host mh{};
+    mh.nvar = p->nv;
+    mh.nelem = p->nelem;
+    mh.nintface = p->nintface;
+    mh.nline = p->nline;
+    mh.nlineelem = p->nle;
+    mh.eftot = 0;
+    info->nvar = p->nv;
+    info->nelem = p->nelem;
+    info->nline = p->nline;
+    info->max_faces = p->F;
+    info->precision = p->prec;
+    info->device_bytes = p->bytes;
+    info->alg_bytes_per_sweep = model_bytes(&mh, p->prec);
+    info->ref_bytes_per_sweep = 0.0;
+    // per-kernel algorithmic bytes of the fused form:
+    //   Thomas: E + Gamma factor blocks (tf), g in, d' out + in, U in + out
+    //   residual: D + Of1 + Of2 (FP64), W (tf), U, rhs in, g out
+    const double bs = (double)p->nv * p->nv, nv = p->nv, tf = (double)tf_size(p->prec);
+    const double n = p->nle;
+    info->thomas_bytes = tf * bs * 2.0 * (n - p->nline) + nv * n * (tf * 3.0 + 16.0);
+    info->residual_bytes = 8.0 * bs * (n + 2.0 * p->nintface) + tf * bs * n + nv * n * (16.0 + tf);
+    return TBS_OK;
+}
+
+int tbs_plan_factor(tbs_plan *p, const double *jacD, const double *jacO1, const double *jacO2) {
+    TBS_REQUIRE(p, TBS_ERR_ARG, "null plan");
+    TBS_TRY(tbs::use_ctx(p->ctx));
+    TBS_REQUIRE(jacD && (p->nintface == 0 || (jacO1 && jacO2)), TBS_ERR_ARG, "null Jacobian array");
+    tbs_ctx *ctx = p->ctx;
+    hipStream_t st = ctx->stream;
+    const int nv = p->nv, BS = nv * nv;
+    const size_t G = (size_t)p->ngroups * kGroup;
+    const size_t tf = tf_size(p->prec);
+    const int lw = (64 / nv) * nv;
+    // residual blocks
+    {
+        const long total = (long)G * p->S * BS;
+        switch (nv) {
+#define X(v) case v: hipLaunchKernelGGL(pack_blocks_kernel<v>, dim3(gs_grid(total)), dim3(256), 0, st, \
+                   total, p->S, p->d_src, jacD, jacO1, jacO2, p->d_blk); break;
+            TBS_PLAN_NVARS(X)
+#undef X
+        }
+        TBS_LAUNCHED("pack_blocks_kernel");
+    }
+    // LU factors D' and DinvC (reference semantics), and the packed A
+    double *Dp = nullptr, *DinvC = nullptr, *A = nullptr;
+    auto cleanup = [&]() {
+        if (Dp) hipFree(Dp);
+        if (DinvC) hipFree(DinvC);
+        if (A) hipFree(A);
+    };
+    const size_t nb = (size_t)p->nelem * BS * sizeof(double);
+    hipError_t e1 = hipMalloc((void **)&Dp, nb);
+    hipError_t e2 = hipMalloc((void **)&DinvC, nb);
+    hipError_t e3 = hipMalloc((void **)&A, nb);
+    if (e1 || e2 || e3) {
+        cleanup();
+        hipGetLastError();
+        tbs::set_error("factor workspace allocation (%zu bytes) refused", 3 * nb);
+        return TBS_ERR_NOMEM;
+    }
+    int rc = TBS_OK;
+    do {
+        if ((rc = tbs_copyAtoBjac(ctx, p->nelem, jacD, Dp))) break;
+        if (hipMemsetAsync(DinvC, 0, nb, st) || hipMemsetAsync(A, 0, nb, st)) { rc = TBS_ERR_STATE; break; }
+        if (p->nintface > 0) {
+            if ((rc = tbs_assembleTriBlocks(ctx, p->nline, p->d_linesize, p->d_linepoint, p->d_lines,
+                                            p->d_lineface, p->d_fc, jacO1, jacO2, A))) break;
+        }
+        if ((rc = tbs_lineLU(ctx, p->nelem, p->nintface, p->nline, p->d_fc, p->d_linesize,
+                             p->d_linepoint, p->d_lines, p->d_lineface, Dp, jacO1, jacO2, DinvC))) break;
+        if (hipMemsetAsync(p->d_EF, 0, (size_t)p->nwin * nv * lw * tf, st) ||
+            hipMemsetAsync(p->d_GB, 0, (size_t)p->nwin * nv * lw * tf, st) ||
+            hipMemsetAsync(p->d_W, 0, G * BS * tf, st)) { rc = TBS_ERR_STATE; break; }
+        const long threads = (long)p->nle * nv;
+        switch (nv) {
+#define X(v) case v: \
+            if (p->prec == TBS_FP32) hipLaunchKernelGGL((invform_kernel<v, float>), dim3((threads + 255) / 256), dim3(256), 0, st, \
+                p->nle, p->d_lines, p->d_mline, p->d_linepoint, p->d_linesize, p->d_woff, Dp, A, DinvC, \
+                (float *)p->d_W, (float *)p->d_EF, (float *)p->d_GB); \
+            else hipLaunchKernelGGL((invform_kernel<v, double>), dim3((threads + 255) / 256), dim3(256), 0, st, \
+                p->nle, p->d_lines, p->d_mline, p->d_linepoint, p->d_linesize, p->d_woff, Dp, A, DinvC, \
+                (double *)p->d_W, (double *)p->d_EF, (double *)p->d_GB); break;
+            TBS_PLAN_NVARS(X)
+#undef X
+        }
+        hipError_t le = hipGetLastError();
+        if (le) { rc = tbs::hip_fail(le, "invform_kernel", __FILE__, __LINE__); break; }
+        hipError_t se = hipStreamSynchronize(st);
+        if (se) { rc = tbs::hip_fail(se, "factor sync", __FILE__, __LINE__); break; }
+    } while (0);
+    cleanup();
+    if (rc == TBS_OK) p->factored = true;
+    return rc;
+}
+
+int tbs_plan_begin(tbs_plan *p, const double *rhs, const double *U0) {
+    TBS_REQUIRE(p, TBS_ERR_ARG, "null plan");
+    TBS_TRY(tbs::use_ctx(p->ctx));
+    TBS_REQUIRE(p->factored, TBS_ERR_STATE, "tbs_plan_begin before tbs_plan_factor");
+    TBS_REQUIRE(rhs, TBS_ERR_ARG, "null rhs");
+    hipStream_t st = p->ctx->stream;
+    const long total = (long)p->nle * p->nv;
+    hipLaunchKernelGGL(gather_vec_kernel, dim3(gs_grid(total)), dim3(256), 0, st, total, p->nv,
+                       p->d_lines, rhs, p->d_rhs);
+    TBS_LAUNCHED("gather rhs");
+    hipLaunchKernelGGL(gather_vec_kernel, dim3(gs_grid(total)), dim3(256), 0, st, total, p->nv,
+                       p->d_lines, U0, p->d_U);
+    TBS_LAUNCHED("gather U0");
+    TBS_TRY(k_reset(p->prec, st, p->d_sys, 1));
+    // R_0 = rhs + J U_0 (= rhs when U_0 = 0, the reference's start) and ||R_0||
+    TBS_TRY(k_residual(p->nv, p->prec, st, p->d_sys, p->d_gtab, p->ngroups, 0, 1));
+    TBS_TRY(k_final(p->prec, st, p->d_sys, 1, p->max_sweeps, p->rtol, 1));
+    return TBS_OK;
+}
+
+int tbs_plan_launch_thomas(tbs_plan *p) {
+    TBS_REQUIRE(p, TBS_ERR_ARG, "null plan");
+    TBS_TRY(tbs::use_ctx(p->ctx));
+    return k_thomas(p->nv, p->prec, p->ctx->stream, p->d_sys, p->d_ltab, p->nline);
+}
+
+int tbs_plan_launch_residual(tbs_plan *p) {
+    TBS_REQUIRE(p, TBS_ERR_ARG, "null plan");
+    TBS_TRY(tbs::use_ctx(p->ctx));
+    TBS_TRY(k_residual(p->nv, p->prec, p->ctx->stream, p->d_sys, p->d_gtab, p->ngroups, 0, 0));
+    return k_final(p->prec, p->ctx->stream, p->d_sys, 1, p->max_sweeps, p->rtol, 0);
+}
+
+int tbs_plan_launch_sweeps(tbs_plan *p, int nsweeps, int use_graph) {
+    TBS_REQUIRE(p && nsweeps >= 0, TBS_ERR_ARG, "bad arguments");
+    TBS_TRY(tbs::use_ctx(p->ctx));
+    hipStream_t st = p->ctx->stream;
+    auto one = [p, st]() {
+        return enqueue_sweep(p->nv, p->prec, st, p->d_sys, 1, p->d_ltab, p->nline, p->d_gtab,
+                             p->ngroups, p->max_sweeps, p->rtol);
+    };
+    if (!use_graph) {
+        for (int i = 0; i < nsweeps; ++i) TBS_TRY(one());
+        return TBS_OK;
+    }
+    if (nsweeps == 0) return TBS_OK;
+    auto it = p->graphs.find(nsweeps);
+    if (it == p->graphs.end()) {
+        hipGraphExec_t ge = nullptr;
+        TBS_TRY(capture_sweeps(st, nsweeps, &ge, one));
+        it = p->graphs.emplace(nsweeps, ge).first;
+    }
+    TBS_HIP(hipGraphLaunch(it->second, st));
+    return TBS_OK;
+}
+
+int tbs_plan_end(tbs_plan *p, double *U, double *R_final) {
+    TBS_REQUIRE(p, TBS_ERR_ARG, "null plan");
+    TBS_TRY(tbs::use_ctx(p->ctx));
+    hipStream_t st = p->ctx->stream;
+    const long total = (long)p->nle * p->nv;
+    if (U) {
+        hipLaunchKernelGGL(scatter_vec_kernel, dim3(gs_grid(total)), dim3(256), 0, st, total, p->nv,
+                           p->d_lines, p->d_U, U);
+        TBS_LAUNCHED("scatter U");
+    }
+    if (R_final) {
+        TBS_TRY(k_residual(p->nv, p->prec, st, p->d_sys, p->d_gtab, p->ngroups, 1, 1));
+        hipLaunchKernelGGL(scatter_vec_kernel, dim3(gs_grid(total)), dim3(256), 0, st, total, p->nv,
+                           p->d_lines, p->d_R, R_final);
+        TBS_LAUNCHED("scatter R");
+    }
+    return TBS_OK;
+}
+
+int tbs_plan_history(tbs_plan *p, double *hist, int cap, int *count) {
+    TBS_REQUIRE(p && count, TBS_ERR_ARG, "null argument");
+    TBS_TRY(tbs::use_ctx(p->ctx));
+    hipStream_t st = p->ctx->stream;
+    TBS_HIP(hipMemcpyAsync(p->h_state, p->d_state, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+    TBS_HIP(hipStreamSynchronize(st));
+    int k = std::min(p->h_state[0], kHistCap);
+    *count = k;
+    if (hist && cap > 0 && k > 0) {
+        TBS_HIP(hipMemcpy(hist, p->d_hist, std::min(k, cap) * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return TBS_OK;
+}
+
+int tbs_plan_solve(tbs_plan *p, const double *rhs, double *U, int max_sweeps, double rtol,
+                   double *hist, int *nsweeps, double *R_final) {
+    TBS_REQUIRE(p && U, TBS_ERR_ARG, "null argument");
+    TBS_REQUIRE(max_sweeps >= 0 && max_sweeps < kHistCap, TBS_ERR_ARG,
+                "max_sweeps %d outside [0, %d)", max_sweeps, kHistCap);
+    TBS_REQUIRE(rtol >= 0.0, TBS_ERR_ARG, "rtol < 0");
+    TBS_TRY(tbs::use_ctx(p->ctx));
+    if (p->max_sweeps != max_sweeps || p->rtol != rtol) {
+        for (auto &kv : p->graphs) hipGraphExecDestroy(kv.second);
+        p->graphs.clear();
+        p->max_sweeps = max_sweeps;
+        p->rtol = rtol;
+    }
+    TBS_TRY(tbs_plan_begin(p, rhs, U));
+    hipStream_t st = p->ctx->stream;
+    int done = 0;
+    const int chunk = 4;
+    while (done < max_sweeps) {
+        const int k = std::min(chunk, max_sweeps - done);
+        TBS_TRY(tbs_plan_launch_sweeps(p, k, 1));
+        done += k;
+        TBS_HIP(hipMemcpyAsync(p->h_state, p->d_state, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+        TBS_HIP(hipStreamSynchronize(st));
+        if (!p->h_state[1]) break;
+    }
+    TBS_TRY(tbs_plan_end(p, U, R_final));
+    int count = 0;
+    TBS_TRY(tbs_plan_history(p, hist, max_sweeps + 1, &count));
+    if (nsweeps) *nsweeps = count > 0 ? count - 1 : 0;
+    if (p->h_state[2] == 2 || p->h_state[2] == 4) {
+        tbs::set_error(p->h_state[2] == 2 ? "non-finite residual" : "residual diverging");
+        return TBS_ERR_NUMERIC;
+    }
+    return TBS_OK;
+}
+
+// ------------------------------------------------------------ batch --
+int tbs_batch_create(tbs_ctx *ctx, tbs_plan *const *plans, int nplans, tbs_batch **out) {
+    TBS_TRY(tbs::use_ctx(ctx));
+    TBS_REQUIRE(out && plans && nplans >= 1, TBS_ERR_ARG, "bad batch arguments");
+    *out = nullptr;
+    const int nv = plans[0]->nv, prec = plans[0]->prec;
+    for (int i = 0; i < nplans; ++i) {
+        TBS_REQUIRE(plans[i] && plans[i]->ctx == ctx, TBS_ERR_ARG, "plan %d not of this context", i);
+        TBS_REQUIRE(plans[i]->nv == nv && plans[i]->prec == prec, TBS_ERR_ARG,
+                    "plans in a batch must share nvar and precision");
+        TBS_REQUIRE(plans[i]->factored, TBS_ERR_STATE, "plan %d not factored", i);
+    }
+    std::unique_ptr<tbs_batch> b(new tbs_batch());
+    b->ctx = ctx;
+    b->plans.assign(plans, plans + nplans);
+    b->nv = nv;
+    b->prec = prec;
+    std::vector<int4> ltab;
+    std::vector<int2> gtab;
+    // lines longest-first so the longest chains start first
+    std::vector<std::pair<int, int4>> lines;
+    for (int s = 0; s < nplans; ++s) {
+        const tbs_plan *p = plans[s];
+        for (int l = 0; l < p->nline; ++l)
+            lines.push_back({p->h_linesize[l], make_int4(s, p->h_linepoint[l], p->h_linesize[l], p->h_woff[l])});
+        for (int g = 0; g < p->ngroups; ++g) gtab.push_back(make_int2(s, g));
+    }
+    std::stable_sort(lines.begin(), lines.end(),
+                     [](const std::pair<int, int4> &a, const std::pair<int, int4> &c) { return a.first > c.first; });
+    for (auto &x : lines) ltab.push_back(x.second);
+    b->nl = (int)ltab.size();
+    b->ng = (int)gtab.size();
+    const size_t ssz = prec == TBS_FP32 ? sizeof(SysDev<float>) : sizeof(SysDev<double>);
+    std::vector<char> sysh(ssz * nplans);
+    for (int s = 0; s < nplans; ++s) {
+        if (prec == TBS_FP32) { SysDev<float> d = make_sysdev<float>(plans[s]); std::memcpy(&sysh[ssz * s], &d, ssz); }
+        else { SysDev<double> d = make_sysdev<double>(plans[s]); std::memcpy(&sysh[ssz * s], &d, ssz); }
+    }
+    auto fail = [&](hipError_t e) { return tbs::hip_fail(e, "batch alloc", __FILE__, __LINE__); };
+    hipError_t e;
+    if ((e = hipMalloc(&b->d_sys, sysh.size()))) return fail(e);
+    if ((e = hipMalloc((void **)&b->d_ltab, ltab.size() * sizeof(int4)))) return fail(e);
+    if ((e = hipMalloc((void **)&b->d_gtab, gtab.size() * sizeof(int2)))) return fail(e);
+    if ((e = hipHostMalloc((void **)&b->h_state, 4 * sizeof(int), hipHostMallocDefault))) return fail(e);
+    TBS_HIP(hipMemcpy(b->d_sys, sysh.data(), sysh.size(), hipMemcpyHostToDevice));
+    TBS_HIP(hipMemcpy(b->d_ltab, ltab.data(), ltab.size() * sizeof(int4), hipMemcpyHostToDevice));
+    TBS_HIP(hipMemcpy(b->d_gtab, gtab.data(), gtab.size() * sizeof(int2), hipMemcpyHostToDevice));
+    *out = b.release();
+    return TBS_OK;
+}
+
+int tbs_batch_destroy(tbs_batch *b) {
+    if (!b) return TBS_OK;
+    hipSetDevice(b->ctx->device);
+    hipStreamSynchronize(b->ctx->stream);
+    for (auto &kv : b->graphs) hipGraphExecDestroy(kv.second);
+    if (b->d_sys) hipFree(b->d_sys);
+    if (b->d_ltab) hipFree(b->d_ltab);
+    if (b->d_gtab) hipFree(b->d_gtab);
+    if (b->h_state) hipHostFree(b->h_state);
+    delete b;
+    return TBS_OK;
+}
+
+int tbs_batch_solve(tbs_batch *b, const double *const *rhs, double *const *U, int max_sweeps,
+                    double rtol, int sweeps_per_check, double *hist, int *nsweeps) {
+    TBS_REQUIRE(b && rhs && U, TBS_ERR_ARG, "null argument");
+    TBS_REQUIRE(max_sweeps >= 0 && max_sweeps < kHistCap, TBS_ERR_ARG, "bad max_sweeps");
+    TBS_TRY(tbs::use_ctx(b->ctx));
+    const int ns = (int)b->plans.size();
+    hipStream_t st = b->ctx->stream;
+    if (sweeps_per_check < 1) sweeps_per_check = 1;
+    for (int s = 0; s < ns; ++s) {
+        tbs_plan *p = b->plans[s];
+        TBS_REQUIRE(rhs[s] && U[s], TBS_ERR_ARG, "null vector for system %d", s);
+        const long total = (long)p->nle * p->nv;
+        hipLaunchKernelGGL(gather_vec_kernel, dim3(gs_grid(total)), dim3(256), 0, st, total, p->nv,
+                           p->d_lines, rhs[s], p->d_rhs);
+        hipLaunchKernelGGL(gather_vec_kernel, dim3(gs_grid(total)), dim3(256), 0, st, total, p->nv,
+                           p->d_lines, (const double *)U[s], p->d_U);
+        TBS_LAUNCHED("batch gather");
+    }
+    TBS_TRY(k_reset(b->prec, st, b->d_sys, ns));
+    TBS_TRY(k_residual(b->nv, b->prec, st, b->d_sys, b->d_gtab, b->ng, 0, 1));
+    TBS_TRY(k_final(b->prec, st, b->d_sys, ns, max_sweeps, rtol, 1));
+    auto key = std::make_pair(sweeps_per_check, std::make_pair(max_sweeps, rtol));
+    auto it = b->graphs.find(key);
+    if (it == b->graphs.end()) {
+        hipGraphExec_t ge = nullptr;
+        auto one = [b, st, ns, max_sweeps, rtol]() {
+            return enqueue_sweep(b->nv, b->prec, st, b->d_sys, ns, b->d_ltab, b->nl, b->d_gtab, b->ng,
+                                 max_sweeps, rtol);
+        };
+        TBS_TRY(capture_sweeps(st, sweeps_per_check, &ge, one));
+        it = b->graphs.emplace(key, ge).first;
+    }
+    int done = 0;
+    std::vector<int> st_h(4 * ns);
+    while (done < max_sweeps) {
+        TBS_HIP(hipGraphLaunch(it->second, st));
+        done += sweeps_per_check;
+        bool any = false;
+        for (int s = 0; s < ns; ++s) {
+            TBS_HIP(hipMemcpyAsync(&st_h[4 * s], b->plans[s]->d_state, 4 * sizeof(int),
+                                   hipMemcpyDeviceToHost, st));
+        }
+        TBS_HIP(hipStreamSynchronize(st));
+        for (int s = 0; s < ns; ++s) any |= st_h[4 * s + 1] != 0;
+        if (!any) break;
+    }
+    for (int s = 0; s < ns; ++s) {
+        TBS_TRY(tbs_plan_end(b->plans[s], U[s], nullptr));
+        int count = 0;
+        TBS_TRY(tbs_plan_history(b->plans[s], hist ? hist + (size_t)s * (max_sweeps + 1) : nullptr,
+                                 max_sweeps + 1, &count));
+        if (nsweeps) nsweeps[s] = count > 0 ? count - 1 : 0;
+    }
+    return TBS_OK;
+}
+
+}  // extern "C"

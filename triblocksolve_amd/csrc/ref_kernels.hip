@@ -1,0 +1,513 @@
+// ref_kernels.hip -- drop-in HIP kernels for the reference's active OKL
+// kernels, operating on the reference's own layouts (column-major NV x NV
+// blocks by element / face id, vectors by element id).
+//
+// Mapping to gfx950: the reference launches 9-lane workgroups and runs its
+// triangular solves in one lane.  Here one wave64 owns a line; lane i < NV
+// holds block row i in registers and the values the other rows need are
+// broadcast with v_readlane (uniform SGPR operands), so no LDS round trip
+// sits on the per-step dependency chain.  Every block entry is computed with
+// the reference's sequence of operations (same accumulation order, multiply
+// by the reciprocal pivot in LU, divide by the U diagonal in solveLU); the
+// compiler contracts a*b+c into FMA as the reference's HIP backend does.
+#include "tbs_common.hpp"
+#include "lane_ops.hpp"
+
+namespace {
+
+using tbs::rdlane;
+
+// Row-parallel Doolittle LU (src/okl/lineLU_v2.okl:17-27): lane i holds row
+// i of the block in r[]; on exit r[] holds row i of the packed L\U factors.
+template <int NV>
+__device__ __forceinline__ void lu_rows(double (&r)[NV], int i) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        double rowj[NV];
+#pragma unroll
+        for (int k = j; k < NV; ++k) rowj[k] = rdlane(r[k], j);
+        const double pivot = 1.0 / rowj[j];
+        if (i > j) {
+            r[j] = r[j] * pivot;
+#pragma unroll
+            for (int k = j + 1; k < NV; ++k) r[k] -= r[j] * rowj[k];
+        }
+    }
+}
+
+// Row-parallel solveLU (src/okl/linesmoothLU_TriBlock_v3.okl:23-45): lane i
+// holds row i of the L\U factors in a[] and b_i; returns x_i.  Sums run in
+// the reference's ascending-j order.
+template <int NV>
+__device__ __forceinline__ double solve_rows(const double (&a)[NV], double b, int i) {
+    double y[NV];
+    double tot = 0.0;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        y[j] = rdlane(b - tot, j);           // lane j: b_j - sum_{j'<j} L(j,j') y_j'
+        if (i > j) tot += a[j] * y[j];
+    }
+    double diag = a[0];
+#pragma unroll
+    for (int j = 1; j < NV; ++j) diag = (i == j) ? a[j] : diag;
+    double x[NV];
+    double xi = 0.0;
+#pragma unroll
+    for (int j = NV - 1; j >= 0; --j) {
+        double t = 0.0;
+#pragma unroll
+        for (int jj = j + 1; jj < NV; ++jj) t += a[jj] * x[jj];
+        const double v = (y[j] - t) / diag;
+        x[j] = rdlane(v, j);
+        xi = (i == j) ? x[j] : xi;
+    }
+    return xi;
+}
+
+// ---------------------------------------------------------------- lineLU --
+// src/okl/lineLU_v2.okl:58-182.  One wave per line.
+template <int NV>
+__global__ __launch_bounds__(64) void lineLU_kernel(int nlines, const int *__restrict__ fc,
+                                                    const int *__restrict__ linesize,
+                                                    const int *__restrict__ linepoint,
+                                                    const int *__restrict__ lines,
+                                                    const int *__restrict__ lineface,
+                                                    double *__restrict__ Dia,
+                                                    const double *__restrict__ Of1,
+                                                    const double *__restrict__ Of2,
+                                                    double *__restrict__ DinvC) {
+    constexpr int BS = NV * NV;
+    __shared__ double sDl[BS];   // D'_{k-1} (packed L\U)
+    __shared__ double sC[BS];
+    __shared__ double sA[BS];
+    __shared__ double sG[BS];    // Gamma = D'_{k-1}^{-1} C
+    __shared__ double sDe[BS];   // raw D_k
+    const int l = blockIdx.x;
+    if (l >= nlines) return;
+    const int t = threadIdx.x;
+    const int i = t;             // block row for t < NV
+    const int n = linesize[l];
+    const int m0 = linepoint[l];
+    if (n <= 0) return;
+
+    double r[NV];
+    {
+        const int e0 = lines[m0];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) r[k] = (i < NV) ? Dia[(size_t)BS * e0 + i + NV * k] : 1.0;
+        lu_rows<NV>(r, i);                                           // :91
+        if (i < NV) {
+#pragma unroll
+            for (int k = 0; k < NV; ++k) {
+                Dia[(size_t)BS * e0 + i + NV * k] = r[k];
+                sDl[i + NV * k] = r[k];
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = 1; k < n; ++k) {
+        const int e = lines[m0 + k];
+        const int f = lineface[m0 + k];
+        const int e1 = fc[2 * f];
+        const int elast = lines[m0 + k - 1];
+        const double *Ap = (e == e1) ? Of2 : Of1;                    // :116
+        const double *Cp = (e == e1) ? Of1 : Of2;                    // :117
+        for (int q = t; q < BS; q += 64) {                           // :119-139
+            sC[q] = Cp[(size_t)BS * f + q];
+            sA[q] = Ap[(size_t)BS * f + q];
+            sDe[q] = Dia[(size_t)BS * e + q];
+        }
+        __syncthreads();
+        if (t < NV) {                                                // :142-144
+            const int c = t;
+            double y[NV], x[NV];
+#pragma unroll
+            for (int ii = 0; ii < NV; ++ii) {
+                double tot = 0.0;
+#pragma unroll
+                for (int j = 0; j < ii; ++j) tot += sDl[ii + NV * j] * y[j];
+                y[ii] = sC[ii + NV * c] - tot;
+            }
+#pragma unroll
+            for (int ii = NV - 1; ii >= 0; --ii) {
+                double tot = 0.0;
+#pragma unroll
+                for (int j = ii + 1; j < NV; ++j) tot += sDl[ii + NV * j] * x[j];
+                x[ii] = (y[ii] - tot) / sDl[ii + NV * ii];
+            }
+#pragma unroll
+            for (int ii = 0; ii < NV; ++ii) sG[ii + NV * c] = x[ii];
+        }
+        __syncthreads();
+        // Alpha = A Gamma, D - Alpha (:147-162), then LU (:165)
+#pragma unroll
+        for (int kk = 0; kk < NV; ++kk) {
+            double tot = 0.0;
+            if (i < NV) {
+#pragma unroll
+                for (int j = 0; j < NV; ++j) tot += sA[i + NV * j] * sG[j + NV * kk];
+                r[kk] = sDe[i + NV * kk] - tot;
+            } else {
+                r[kk] = 1.0;
+            }
+        }
+        lu_rows<NV>(r, i);
+        __syncthreads();                                             // sDl reads done
+        if (i < NV) {                                                // :169-178
+#pragma unroll
+            for (int kk = 0; kk < NV; ++kk) {
+                Dia[(size_t)BS * e + i + NV * kk] = r[kk];
+                sDl[i + NV * kk] = r[kk];
+            }
+        }
+        for (int q = t; q < BS; q += 64) DinvC[(size_t)BS * elast + q] = sG[q];
+        __syncthreads();
+    }
+}
+
+// -------------------------------------------------------------- solveDU --
+// src/okl/linesmoothLU_TriBlock_v3.okl:92-226.  One wave per line.
+template <int NV>
+__global__ __launch_bounds__(64) void solveDU_kernel(int nlines, const int *__restrict__ linesize,
+                                                     const int *__restrict__ linepoint,
+                                                     const int *__restrict__ lines,
+                                                     const double *__restrict__ Dia,
+                                                     const double *__restrict__ DinvC,
+                                                     const double *__restrict__ A,
+                                                     double *__restrict__ dU,
+                                                     const double *__restrict__ R) {
+    constexpr int BS = NV * NV;
+    const int l = blockIdx.x;
+    if (l >= nlines) return;
+    const int i = threadIdx.x;
+    const int ir = (i < NV) ? i : 0;   // lanes >= NV shadow row 0 (results unused)
+    const int n = linesize[l];
+    const int m0 = linepoint[l];
+    if (n <= 0) return;
+
+    double prev[NV];
+    {
+        const int e0 = lines[m0];
+        double d[NV];
+#pragma unroll
+        for (int j = 0; j < NV; ++j) d[j] = Dia[(size_t)BS * e0 + ir + NV * j];
+        const double S = -R[(size_t)NV * e0 + ir];                    // :144
+        const double x = solve_rows<NV>(d, S, i);                     // :153-155
+        if (i < NV) dU[(size_t)NV * e0 + i] = x;                      // :157
+#pragma unroll
+        for (int j = 0; j < NV; ++j) prev[j] = rdlane(x, j);
+    }
+    for (int k = 1; k < n; ++k) {                                     // :163-196
+        const int e = lines[m0 + k];
+        double a[NV], d[NV];
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            a[j] = A[(size_t)BS * e + ir + NV * j];
+            d[j] = Dia[(size_t)BS * e + ir + NV * j];
+        }
+        const double rv = R[(size_t)NV * e + ir];
+        double tot = 0.0;
+#pragma unroll
+        for (int j = 0; j < NV; ++j) tot += a[j] * prev[j];
+        const double S = -rv - tot;                                   // :188
+        const double x = solve_rows<NV>(d, S, i);
+        if (i < NV) dU[(size_t)NV * e + i] = x;
+#pragma unroll
+        for (int j = 0; j < NV; ++j) prev[j] = rdlane(x, j);
+    }
+    for (int k = n - 2; k >= 0; --k) {                                // :199-223
+        const int e = lines[m0 + k];
+        double g[NV];
+#pragma unroll
+        for (int j = 0; j < NV; ++j) g[j] = DinvC[(size_t)BS * e + ir + NV * j];
+        double tot = 0.0;
+#pragma unroll
+        for (int j = 0; j < NV; ++j) tot += g[j] * prev[j];          // prev = dU(elast)
+        const double v = dU[(size_t)NV * e + ir] - tot;
+        if (i < NV) dU[(size_t)NV * e + i] = v;
+#pragma unroll
+        for (int j = 0; j < NV; ++j) prev[j] = rdlane(v, j);
+    }
+}
+
+// -------------------------------------------------------------- lineRes --
+// src/okl/linesmoothLU_TriBlock_v3.okl:228-336.  One thread per (element,
+// row); grid.y = line, grid.x strides over the line's positions.
+template <int NV>
+__global__ void lineRes_kernel(int nlines, const int *__restrict__ epoint,
+                               const int *__restrict__ ef, const int *__restrict__ fc,
+                               const int *__restrict__ linesize,
+                               const int *__restrict__ linepoint,
+                               const int *__restrict__ lines, const double *__restrict__ Dia,
+                               const double *__restrict__ Of1, const double *__restrict__ Of2,
+                               const double *__restrict__ U, double *__restrict__ R) {
+    constexpr int BS = NV * NV;
+    constexpr int EPB = (256 / NV) > 0 ? (256 / NV) : 1;   // elements per block
+    const int l = blockIdx.y;
+    const int kl = threadIdx.x / NV, i = threadIdx.x % NV;
+    if (l >= nlines || kl >= EPB) return;
+    const int n = linesize[l];
+    const int m0 = linepoint[l];
+    for (int k = blockIdx.x * EPB + kl; k < n; k += gridDim.x * EPB) {
+        const int e = lines[m0 + k];
+        double tot = 0.0;                                             // :276-293
+#pragma unroll
+        for (int j = 0; j < NV; ++j) tot += Dia[(size_t)BS * e + i + NV * j] * U[(size_t)NV * e + j];
+        double sR = tot;
+        for (int k2 = epoint[e]; k2 < epoint[e + 1]; ++k2) {          // :299-328
+            const int f = ef[k2];
+            if (f >= 0) {
+                const int e1 = fc[2 * f + 0];
+                const int e2 = fc[2 * f + 1];
+                const double *off = (e == e1) ? Of2 : Of1;
+                const int nb = (e == e1) ? e2 : e1;
+                double t2 = 0.0;
+#pragma unroll
+                for (int j = 0; j < NV; ++j) t2 += off[(size_t)BS * f + i + NV * j] * U[(size_t)NV * nb + j];
+                sR += t2;
+            }
+        }
+        R[(size_t)NV * e + i] += sR;                                  // :331
+    }
+}
+
+// ------------------------------------------------------ assembleTriBlocks --
+// src/Jacobian.cxx:76-117 as a device gather; grid.y = line.
+template <int NV>
+__global__ void assemble_kernel(int nline, const int *__restrict__ linesize,
+                                const int *__restrict__ linepoint,
+                                const int *__restrict__ lines,
+                                const int *__restrict__ lineface, const int *__restrict__ fc,
+                                const double *__restrict__ O1, const double *__restrict__ O2,
+                                double *__restrict__ A) {
+    constexpr int BS = NV * NV;
+    const int l = blockIdx.y;
+    if (l >= nline) return;
+    const int n = linesize[l];
+    const int m0 = linepoint[l];
+    const long total = (long)(n - 1) * BS;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)gridDim.x * blockDim.x) {
+        const int k = 1 + (int)(idx / BS);
+        const int q = (int)(idx % BS);
+        const int m = m0 + k;
+        const int f = lineface[m];
+        const int e = lines[m];
+        const double *src = (e == fc[2 * f]) ? O2 : O1;
+        A[(size_t)BS * e + q] = src[(size_t)BS * f + q];
+    }
+}
+
+// -------------------------------------------------------- copy / add --
+__global__ void copy_kernel(long n, const double *__restrict__ A, double *__restrict__ B) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        B[i] = A[i];
+}
+__global__ void add_kernel(long n, const double *__restrict__ A, double *__restrict__ B) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        B[i] += A[i];
+}
+
+// -------------------------------------------------------------- norm --
+// Deterministic two-level sum of squares: block b reduces the fixed slice
+// [b*chunk, (b+1)*chunk) in a fixed order; one block then sums the partials.
+constexpr int kNormBlocks = 1024;
+__global__ __launch_bounds__(256) void sumsq_partial(long n, long chunk, const double *__restrict__ v,
+                                                     double *__restrict__ part) {
+    __shared__ double s[256];
+    const long b0 = (long)blockIdx.x * chunk;
+    const long b1 = (b0 + chunk < n) ? b0 + chunk : n;
+    double acc = 0.0;
+    for (long i = b0 + threadIdx.x; i < b1; i += 256) acc += v[i] * v[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+__global__ __launch_bounds__(256) void sum_final(int nb, const double *__restrict__ part,
+                                                 double *__restrict__ out) {
+    __shared__ double s[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256) acc += part[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sqrt(s[0]);
+}
+
+unsigned grid_for(long n) {
+    long g = (n + 255) / 256;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    return (unsigned)g;
+}
+
+}  // namespace
+
+// Block sizes built into the drop-in kernels.
+#define TBS_REF_NVARS(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
+
+extern "C" {
+
+int tbs_supported_nvar(int *out, int cap) {
+    int n = 0;
+#define X(v) if (out && n < cap) out[n] = v; ++n;
+    TBS_REF_NVARS(X)
+#undef X
+    return n;
+}
+
+int tbs_copyAtoBjac(tbs_ctx *ctx, int nelem, const double *A, double *B) {
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    TBS_REQUIRE(nelem >= 0, TBS_ERR_ARG, "nelem < 0");
+    const long n = (long)nelem * ctx->nvar * ctx->nvar;
+    if (n == 0) return TBS_OK;
+    TBS_REQUIRE(A && B, TBS_ERR_ARG, "null array");
+    hipLaunchKernelGGL(copy_kernel, dim3(grid_for(n)), dim3(256), 0, ctx->stream, n, A, B);
+    TBS_LAUNCHED("copyAtoBjac");
+    return TBS_OK;
+}
+
+int tbs_copyAtoB(tbs_ctx *ctx, int nelem, const double *A, double *B) {
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    TBS_REQUIRE(nelem >= 0, TBS_ERR_ARG, "nelem < 0");
+    const long n = (long)nelem * ctx->nvar;
+    if (n == 0) return TBS_OK;
+    TBS_REQUIRE(A && B, TBS_ERR_ARG, "null array");
+    hipLaunchKernelGGL(copy_kernel, dim3(grid_for(n)), dim3(256), 0, ctx->stream, n, A, B);
+    TBS_LAUNCHED("copyAtoB");
+    return TBS_OK;
+}
+
+int tbs_addAtoB(tbs_ctx *ctx, int nelem, const double *A, double *B) {
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    TBS_REQUIRE(nelem >= 0, TBS_ERR_ARG, "nelem < 0");
+    const long n = (long)nelem * ctx->nvar;
+    if (n == 0) return TBS_OK;
+    TBS_REQUIRE(A && B, TBS_ERR_ARG, "null array");
+    hipLaunchKernelGGL(add_kernel, dim3(grid_for(n)), dim3(256), 0, ctx->stream, n, A, B);
+    TBS_LAUNCHED("addAtoB");
+    return TBS_OK;
+}
+
+int tbs_assembleTriBlocks(tbs_ctx *ctx, int nline, const int *linesize, const int *linepoint,
+                          const int *lines, const int *lineface, const int *fc,
+                          const double *jacO1, const double *jacO2, double *A) {
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    TBS_REQUIRE(nline >= 0 && nline <= 65535, TBS_ERR_ARG, "nline %d outside [0, 65535]", nline);
+    if (nline == 0) return TBS_OK;
+    TBS_REQUIRE(linesize && linepoint && lines && lineface && fc && jacO1 && jacO2 && A,
+                TBS_ERR_ARG, "null array");
+    switch (ctx->nvar) {
+#define X(v) case v: hipLaunchKernelGGL(assemble_kernel<v>, dim3(16, nline), dim3(256), 0, ctx->stream, \
+                   nline, linesize, linepoint, lines, lineface, fc, jacO1, jacO2, A); break;
+        TBS_REF_NVARS(X)
+#undef X
+        default: TBS_REQUIRE(false, TBS_ERR_UNSUPPORTED, "nvar %d", ctx->nvar);
+    }
+    TBS_LAUNCHED("assembleTriBlocks");
+    return TBS_OK;
+}
+
+int tbs_lineLU(tbs_ctx *ctx, int nelem, int nintfaces, int nlines, const int *fc,
+               const int *linesize, const int *linepoint, const int *lines,
+               const int *lineface, double *Dia, const double *Of1, const double *Of2,
+               double *DinvC) {
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    TBS_REQUIRE(nelem >= 0 && nintfaces >= 0 && nlines >= 0, TBS_ERR_ARG, "negative size");
+    if (nlines == 0) return TBS_OK;
+    TBS_REQUIRE(linesize && linepoint && lines && lineface && Dia && DinvC, TBS_ERR_ARG,
+                "null array");
+    TBS_REQUIRE(nintfaces == 0 || (fc && Of1 && Of2), TBS_ERR_ARG, "null face array");
+    switch (ctx->nvar) {
+#define X(v) case v: hipLaunchKernelGGL(lineLU_kernel<v>, dim3(nlines), dim3(64), 0, ctx->stream, \
+                   nlines, fc, linesize, linepoint, lines, lineface, Dia, Of1, Of2, DinvC); break;
+        TBS_REF_NVARS(X)
+#undef X
+        default: TBS_REQUIRE(false, TBS_ERR_UNSUPPORTED, "nvar %d", ctx->nvar);
+    }
+    TBS_LAUNCHED("lineLU");
+    return TBS_OK;
+}
+
+int tbs_triblock_solveDU(tbs_ctx *ctx, int nelem, int nintfaces, int eftot, int nlines,
+                         int linelemtot, const int *linesize, const int *linepoint,
+                         const int *lines, const double *Dia, const double *DinvC,
+                         const double *A, double *dU, const double *R) {
+    (void)nintfaces; (void)eftot; (void)linelemtot;
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    TBS_REQUIRE(nelem >= 0 && nlines >= 0, TBS_ERR_ARG, "negative size");
+    if (nlines == 0) return TBS_OK;
+    TBS_REQUIRE(linesize && linepoint && lines && Dia && DinvC && A && dU && R, TBS_ERR_ARG,
+                "null array");
+    switch (ctx->nvar) {
+#define X(v) case v: hipLaunchKernelGGL(solveDU_kernel<v>, dim3(nlines), dim3(64), 0, ctx->stream, \
+                   nlines, linesize, linepoint, lines, Dia, DinvC, A, dU, R); break;
+        TBS_REF_NVARS(X)
+#undef X
+        default: TBS_REQUIRE(false, TBS_ERR_UNSUPPORTED, "nvar %d", ctx->nvar);
+    }
+    TBS_LAUNCHED("triblock_solveDU");
+    return TBS_OK;
+}
+
+int tbs_triblock_lineRes(tbs_ctx *ctx, int nelem, int nintfaces, int eftot, int nlines,
+                         int linelemtot, const int *epoint, const int *ef, const int *fc,
+                         const int *linesize, const int *linepoint, const int *lines,
+                         const double *Dia, const double *Of1, const double *Of2,
+                         const double *U, double *R) {
+    (void)nelem; (void)eftot; (void)linelemtot;
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    TBS_REQUIRE(nlines >= 0 && nlines <= 65535, TBS_ERR_ARG, "nlines %d outside [0, 65535]", nlines);
+    if (nlines == 0) return TBS_OK;
+    TBS_REQUIRE(epoint && ef && linesize && linepoint && lines && Dia && U && R, TBS_ERR_ARG,
+                "null array");
+    TBS_REQUIRE(nintfaces == 0 || (fc && Of1 && Of2), TBS_ERR_ARG, "null face array");
+    const int nv = ctx->nvar;
+    const int epb = (256 / nv) > 0 ? 256 / nv : 1;
+    switch (nv) {
+#define X(v) case v: hipLaunchKernelGGL(lineRes_kernel<v>, dim3(8, nlines), dim3(epb * v), 0, ctx->stream, \
+                   nlines, epoint, ef, fc, linesize, linepoint, lines, Dia, Of1, Of2, U, R); break;
+        TBS_REF_NVARS(X)
+#undef X
+        default: TBS_REQUIRE(false, TBS_ERR_UNSUPPORTED, "nvar %d", nv);
+    }
+    TBS_LAUNCHED("triblock_lineRes");
+    return TBS_OK;
+}
+
+int tbs_norm2(tbs_ctx *ctx, long n, const double *v, double *out_host) {
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    TBS_REQUIRE(n >= 0 && out_host, TBS_ERR_ARG, "bad norm arguments");
+    if (n == 0) { *out_host = 0.0; return TBS_OK; }
+    TBS_REQUIRE(v, TBS_ERR_ARG, "null vector");
+    if (!ctx->red_buf) TBS_HIP(hipMalloc((void **)&ctx->red_buf, (kNormBlocks + 1) * sizeof(double)));
+    const long chunk = (n + kNormBlocks - 1) / kNormBlocks;
+    hipLaunchKernelGGL(sumsq_partial, dim3(kNormBlocks), dim3(256), 0, ctx->stream, n, chunk, v,
+                       ctx->red_buf);
+    TBS_LAUNCHED("sumsq_partial");
+    hipLaunchKernelGGL(sum_final, dim3(1), dim3(256), 0, ctx->stream, kNormBlocks, ctx->red_buf,
+                       ctx->red_buf + kNormBlocks);
+    TBS_LAUNCHED("sum_final");
+    TBS_HIP(hipMemcpyAsync(ctx->red_host, ctx->red_buf + kNormBlocks, sizeof(double),
+                           hipMemcpyDeviceToHost, ctx->stream));
+    TBS_HIP(hipStreamSynchronize(ctx->stream));
+    *out_host = *ctx->red_host;
+    return TBS_OK;
+}
+
+}  // extern "C"

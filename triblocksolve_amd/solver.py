@@ -1,0 +1,340 @@
+"""Host-side helpers over the C ABI: meshes, synthetic systems, contexts, plans.
+
+Naming follows the reference host API (src/Mesh.cxx, src/Jacobian.cxx,
+src/main.cxx): meshes carry epoint/ef/fc/linesize/linepoint/lines/lineface
+(0-based), Jacobians jacD/jacO1/jacO2/rhs in the reference block layout.
+Device arrays are torch tensors on a HIP device; all arithmetic happens in
+libtriblock_hip.so.
+"""
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _capi as A
+
+
+def _ptr(x):
+    """c_void_p of a numpy array or a torch tensor (None -> NULL)."""
+    if x is None:
+        return C.c_void_p(None)
+    if isinstance(x, np.ndarray):
+        if not x.flags["C_CONTIGUOUS"]:
+            raise ValueError("array must be C-contiguous")
+        return C.c_void_p(x.ctypes.data)
+    if hasattr(x, "data_ptr"):
+        if not x.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return C.c_void_p(x.data_ptr())
+    if isinstance(x, int):
+        return C.c_void_p(x)
+    raise TypeError(f"cannot take a pointer of {type(x)}")
+
+
+MESH_FIELDS = ("epoint", "ef", "fc", "linesize", "linepoint", "lines", "lineface")
+
+
+@dataclass
+class HostMesh:
+    """Mesh connectivity as host int32 arrays (the Mesh class of src/Mesh.cxx)."""
+    nvar: int
+    epoint: np.ndarray
+    ef: np.ndarray
+    fc: np.ndarray
+    linesize: np.ndarray
+    linepoint: np.ndarray
+    lines: np.ndarray
+    lineface: np.ndarray
+
+    @property
+    def nelem(self):
+        return int(self.epoint.size - 1)
+
+    @property
+    def nintface(self):
+        return int(self.fc.size // 2)
+
+    @property
+    def eftot(self):
+        return int(self.ef.size)
+
+    @property
+    def nline(self):
+        return int(self.linesize.size)
+
+    @property
+    def nlineelem(self):
+        return int(self.lines.size)
+
+    @property
+    def max_line_nelem(self):
+        return int(self.linesize.max()) if self.linesize.size else 0
+
+    def sizes(self):
+        return [self.nvar, self.nelem, self.nintface, self.eftot, self.nline, self.nlineelem]
+
+    def as_struct(self):
+        for f in MESH_FIELDS:
+            a = getattr(self, f)
+            if a.dtype != np.int32 or not a.flags["C_CONTIGUOUS"]:
+                setattr(self, f, np.ascontiguousarray(a, dtype=np.int32))
+        s = A.MeshHost(self.nvar, self.nelem, self.nintface, self.eftot, self.nline, self.nlineelem,
+                       *[_ptr(getattr(self, f)) for f in MESH_FIELDS])
+        s._keep = self  # keep arrays alive
+        return s
+
+    def to_device(self, device="cuda"):
+        import torch
+        return {f: torch.from_numpy(np.ascontiguousarray(getattr(self, f))).to(device) for f in MESH_FIELDS}
+
+
+def structured_mesh(NL, NK, nvar=9):
+    """NL lines x NK elements structured grid (the ex05 topology rule)."""
+    lib = A.load()
+    s = (C.c_int * 5)()
+    A.check(lib.tbs_structured_sizes(NL, NK, *[C.byref(s, 4 * i) for i in range(5)]))
+    nelem, nif, eftot, nline, nle = list(s)
+    arrs = dict(epoint=np.empty(nelem + 1, np.int32), ef=np.empty(eftot, np.int32),
+                fc=np.empty(2 * nif, np.int32), linesize=np.empty(nline, np.int32),
+                linepoint=np.empty(nline + 1, np.int32), lines=np.empty(nle, np.int32),
+                lineface=np.empty(nle, np.int32))
+    A.check(lib.tbs_structured_mesh(NL, NK, *[_ptr(arrs[f]) for f in MESH_FIELDS]))
+    return HostMesh(nvar=nvar, **arrs)
+
+
+StructuredMesh = structured_mesh
+
+
+def generate_host(seed, nvar, nelem, nintface, shift):
+    """Seeded synthetic Jacobian on the host (product generator)."""
+    bs = nvar * nvar
+    out = dict(jacD=np.empty(bs * nelem), jacO1=np.empty(bs * nintface),
+               jacO2=np.empty(bs * nintface), rhs=np.empty(nvar * nelem))
+    A.check(A.load().tbs_generate_host(seed, nvar, nelem, nintface, float(shift),
+                                       *[_ptr(out[k]) for k in ("jacD", "jacO1", "jacO2", "rhs")]))
+    return out
+
+
+def read_mesh(path):
+    lib = A.load()
+    h = np.empty(6, np.int32)
+    A.check(lib.tbs_read_mesh_header(str(path).encode(), _ptr(h)))
+    nvar, nelem, nif, eftot, nline, nle = [int(v) for v in h]
+    arrs = dict(epoint=np.empty(nelem + 1, np.int32), ef=np.empty(eftot, np.int32),
+                fc=np.empty(2 * nif, np.int32), linesize=np.empty(nline, np.int32),
+                linepoint=np.empty(nline + 1, np.int32), lines=np.empty(nle, np.int32),
+                lineface=np.empty(nle, np.int32))
+    A.check(lib.tbs_read_mesh(str(path).encode(), *[_ptr(arrs[f]) for f in MESH_FIELDS]))
+    return HostMesh(nvar=nvar, **arrs)
+
+
+def write_mesh(path, mesh):
+    h = np.array(mesh.sizes(), np.int32)
+    mesh.as_struct()
+    A.check(A.load().tbs_write_mesh(str(path).encode(), _ptr(h), *[_ptr(getattr(mesh, f)) for f in MESH_FIELDS]))
+
+
+def read_jacobian(path):
+    lib = A.load()
+    h = np.empty(3, np.int32)
+    A.check(lib.tbs_read_jacobian_header(str(path).encode(), _ptr(h)))
+    nvar, nelem, nif = [int(v) for v in h]
+    bs = nvar * nvar
+    out = dict(jacD=np.empty(bs * nelem), jacO1=np.empty(bs * nif), jacO2=np.empty(bs * nif),
+               rhs=np.empty(nvar * nelem), U0=np.empty(nvar * nelem))
+    A.check(lib.tbs_read_jacobian(str(path).encode(), *[_ptr(out[k]) for k in ("jacD", "jacO1", "jacO2", "rhs", "U0")]))
+    return nvar, nelem, nif, out
+
+
+def write_jacobian(path, nvar, nelem, nintface, jac, U0=None):
+    A.check(A.load().tbs_write_jacobian(str(path).encode(), nvar, nelem, nintface,
+                                        *[_ptr(np.ascontiguousarray(jac[k])) for k in ("jacD", "jacO1", "jacO2", "rhs")],
+                                        _ptr(U0)))
+
+
+def model_bytes(mesh, factor_bytes=8):
+    """Algorithmic HBM bytes of one LIJ sweep: D', A, DinvC (factor precision),
+    D, Of1, Of2 (FP64) and six FP64 vector passes."""
+    bs = mesh.nvar * mesh.nvar
+    return (factor_bytes * bs * (3 * mesh.nelem - 2 * mesh.nline) + 8 * bs * (mesh.nelem + 2 * mesh.nintface)
+            + 6 * 8 * mesh.nvar * mesh.nelem)
+
+
+def reference_model_bytes(mesh):
+    """The reference's own per-sweep byte model, src/main.cxx:149-175 divided by iters."""
+    nv, bs = mesh.nvar, mesh.nvar * mesh.nvar
+    du = 8 * (bs * mesh.nelem + bs * mesh.nelem + bs * mesh.nlineelem + 4 * nv * mesh.nelem + nv * mesh.nelem) \
+        + 4 * (mesh.nline + mesh.nline + 1 + 2 * mesh.nlineelem)
+    cp = 5 * nv * mesh.nelem * 8
+    lr = 8 * (bs * mesh.nelem + 2 * bs * mesh.nintface + 5 * mesh.nelem * nv + 2 * nv * mesh.nelem) \
+        + 4 * (mesh.nelem + 1 + mesh.eftot + 2 * mesh.nintface + mesh.nline + mesh.nline + 1 + 2 * mesh.nlineelem)
+    return du + cp + lr
+
+
+class Context:
+    """Device context (the Platform of include/Platform.hxx): device id, block size, stream."""
+
+    def __init__(self, device=0, nvar=9):
+        self._lib = A.load()
+        h = C.c_void_p()
+        A.check(self._lib.tbs_ctx_create(device, nvar, C.byref(h)))
+        self.handle = h
+        self.nvar = nvar
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            self._lib.tbs_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        A.check(self._lib.tbs_ctx_synchronize(self.handle))
+
+    def stream(self):
+        return self._lib.tbs_ctx_stream(self.handle)
+
+    # --- drop-in kernels (device tensors) ---
+    def copyAtoBjac(self, nelem, a, b):
+        A.check(self._lib.tbs_copyAtoBjac(self.handle, nelem, _ptr(a), _ptr(b)))
+
+    def copyAtoB(self, nelem, a, b):
+        A.check(self._lib.tbs_copyAtoB(self.handle, nelem, _ptr(a), _ptr(b)))
+
+    def addAtoB(self, nelem, a, b):
+        A.check(self._lib.tbs_addAtoB(self.handle, nelem, _ptr(a), _ptr(b)))
+
+    def assembleTriBlocks(self, m, jacO1, jacO2, Aout):
+        A.check(self._lib.tbs_assembleTriBlocks(self.handle, m["linesize"].numel(), _ptr(m["linesize"]),
+                                                _ptr(m["linepoint"]), _ptr(m["lines"]), _ptr(m["lineface"]),
+                                                _ptr(m["fc"]), _ptr(jacO1), _ptr(jacO2), _ptr(Aout)))
+
+    def lineLU(self, nelem, nintface, m, Dia, Of1, Of2, DinvC):
+        A.check(self._lib.tbs_lineLU(self.handle, nelem, nintface, m["linesize"].numel(), _ptr(m["fc"]),
+                                     _ptr(m["linesize"]), _ptr(m["linepoint"]), _ptr(m["lines"]),
+                                     _ptr(m["lineface"]), _ptr(Dia), _ptr(Of1), _ptr(Of2), _ptr(DinvC)))
+
+    def solveDU(self, nelem, nintface, m, Dia, DinvC, Amat, dU, R):
+        A.check(self._lib.tbs_triblock_solveDU(self.handle, nelem, nintface, m["ef"].numel(), m["linesize"].numel(),
+                                               m["lines"].numel(), _ptr(m["linesize"]), _ptr(m["linepoint"]),
+                                               _ptr(m["lines"]), _ptr(Dia), _ptr(DinvC), _ptr(Amat), _ptr(dU), _ptr(R)))
+
+    def lineRes(self, nelem, nintface, m, Dia, Of1, Of2, U, R):
+        A.check(self._lib.tbs_triblock_lineRes(self.handle, nelem, nintface, m["ef"].numel(), m["linesize"].numel(),
+                                               m["lines"].numel(), _ptr(m["epoint"]), _ptr(m["ef"]), _ptr(m["fc"]),
+                                               _ptr(m["linesize"]), _ptr(m["linepoint"]), _ptr(m["lines"]),
+                                               _ptr(Dia), _ptr(Of1), _ptr(Of2), _ptr(U), _ptr(R)))
+
+    def norm2(self, n, v):
+        out = C.c_double()
+        A.check(self._lib.tbs_norm2(self.handle, n, _ptr(v), C.byref(out)))
+        return out.value
+
+    def generate_device(self, seed, nelem, nintface, shift, jacD, jacO1, jacO2, rhs):
+        A.check(self._lib.tbs_generate_device(self.handle, seed, self.nvar, nelem, nintface, float(shift),
+                                              _ptr(jacD), _ptr(jacO1), _ptr(jacO2), _ptr(rhs)))
+
+
+class Plan:
+    """Fused LIJ solver for one system: re-laid-out, factored, graph-captured sweeps."""
+
+    def __init__(self, ctx, mesh, precision=A.TBS_FP64):
+        self.ctx = ctx
+        self.mesh = mesh
+        self._ms = mesh.as_struct()
+        h = C.c_void_p()
+        A.check(ctx._lib.tbs_plan_create(ctx.handle, C.byref(self._ms), precision, C.byref(h)))
+        self.handle = h
+        self.precision = precision
+
+    @staticmethod
+    def footprint(mesh, precision=A.TBS_FP64):
+        ms = mesh.as_struct()
+        out = C.c_size_t()
+        A.check(A.load().tbs_plan_footprint(C.byref(ms), precision, C.byref(out)))
+        return out.value
+
+    def close(self):
+        if self.handle:
+            self.ctx._lib.tbs_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        inf = A.PlanInfo()
+        A.check(self.ctx._lib.tbs_plan_info_get(self.handle, C.byref(inf)))
+        return {k: getattr(inf, k) for k, _ in A.PlanInfo._fields_}
+
+    def factor(self, jacD, jacO1, jacO2):
+        A.check(self.ctx._lib.tbs_plan_factor(self.handle, _ptr(jacD), _ptr(jacO1), _ptr(jacO2)))
+
+    def solve(self, rhs, U, max_sweeps, rtol=0.0, R_out=None):
+        hist = np.zeros(max_sweeps + 1)
+        ns = C.c_int()
+        A.check(self.ctx._lib.tbs_plan_solve(self.handle, _ptr(rhs), _ptr(U), max_sweeps, float(rtol),
+                                             _ptr(hist), C.byref(ns), _ptr(R_out)))
+        return hist[: ns.value + 1], ns.value
+
+    def begin(self, rhs, U0=None):
+        A.check(self.ctx._lib.tbs_plan_begin(self.handle, _ptr(rhs), _ptr(U0)))
+
+    def launch_thomas(self):
+        A.check(self.ctx._lib.tbs_plan_launch_thomas(self.handle))
+
+    def launch_residual(self):
+        A.check(self.ctx._lib.tbs_plan_launch_residual(self.handle))
+
+    def launch_sweeps(self, n, use_graph=True):
+        A.check(self.ctx._lib.tbs_plan_launch_sweeps(self.handle, n, 1 if use_graph else 0))
+
+    def end(self, U=None, R=None):
+        A.check(self.ctx._lib.tbs_plan_end(self.handle, _ptr(U), _ptr(R)))
+
+    def history(self, cap=4096):
+        hist = np.zeros(cap)
+        cnt = C.c_int()
+        A.check(self.ctx._lib.tbs_plan_history(self.handle, _ptr(hist), cap, C.byref(cnt)))
+        return hist[: min(cnt.value, cap)]
+
+
+class Batch:
+    """Many independent plans advanced together (one launch per kernel per sweep)."""
+
+    def __init__(self, ctx, plans):
+        self.ctx = ctx
+        self.plans = list(plans)
+        arr = (C.c_void_p * len(self.plans))(*[p.handle.value for p in self.plans])
+        h = C.c_void_p()
+        A.check(ctx._lib.tbs_batch_create(ctx.handle, arr, len(self.plans), C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self.ctx._lib.tbs_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, rhs_list, U_list, max_sweeps, rtol=0.0, sweeps_per_check=4):
+        n = len(self.plans)
+        rp = (C.c_void_p * n)(*[_ptr(r).value for r in rhs_list])
+        up = (C.c_void_p * n)(*[_ptr(u).value for u in U_list])
+        hist = np.zeros((n, max_sweeps + 1))
+        ns = np.zeros(n, np.int32)
+        A.check(self.ctx._lib.tbs_batch_solve(self.handle, rp, up, max_sweeps, float(rtol), sweeps_per_check,
+                                              _ptr(hist), _ptr(ns)))
+        return [hist[i, : ns[i] + 1] for i in range(n)], ns
