@@ -68,15 +68,12 @@ def main():
                                    "(DESIGN.md) but not built", "n_gpus": args.gpus}))
         sys.exit(2)
     import numpy as np
-    import torch
     import triblocksolve_amd as tbs
 
-    torch.cuda.init()
     ctx = tbs.Context(0, 9)
-    stream = torch.cuda.ExternalStream(ctx.stream())
     m = tbs.structured_mesh(args.nl, args.nk)
     n, nif = m.nelem, m.nintface
-    z = lambda k: torch.empty(k, dtype=torch.float64, device="cuda")
+    z = lambda k: ctx.array(size=k, zero=False)
     D, O1, O2, rhs = z(81 * n), z(81 * nif), z(81 * nif), z(9 * n)
     ctx.generate_device(args.seed, n, nif, args.shift, D, O1, O2, rhs)
     ctx.sync()
@@ -93,21 +90,22 @@ def main():
             plan.launch_thomas()
             plan.launch_residual()
         ctx.sync()
-        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
-        torch.cuda.synchronize()
+        ev = [[ctx.event() for _ in range(3)] for _ in range(steps)]
         ctx.sync()
         t0 = time.perf_counter()
         for s in range(steps):
-            ev[s][0].record(stream)
+            ctx.record(ev[s][0])
             plan.launch_thomas()
-            ev[s][1].record(stream)
+            ctx.record(ev[s][1])
             plan.launch_residual()
-            ev[s][2].record(stream)
+            ctx.record(ev[s][2])
         ctx.sync()
-        torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        th = np.array([ev[s][0].elapsed_time(ev[s][1]) for s in range(steps)]) * 1e-3
-        rs = np.array([ev[s][1].elapsed_time(ev[s][2]) for s in range(steps)]) * 1e-3
+        th = np.array([ctx.elapsed_ms(ev[s][0], ev[s][1]) for s in range(steps)]) * 1e-3
+        rs = np.array([ctx.elapsed_ms(ev[s][1], ev[s][2]) for s in range(steps)]) * 1e-3
+        for trio in ev:
+            for e in trio:
+                ctx.destroy_event(e)
         # graph throughput (informational)
         ctx.sync()
         plan.launch_sweeps(steps, use_graph=True)   # capture + first launch
@@ -121,7 +119,7 @@ def main():
 
     def time_to_tol(prec, rtol=1e-10):
         plan = tbs.Plan(ctx, m, prec)
-        U = torch.zeros(9 * n, dtype=torch.float64, device="cuda")
+        U = ctx.array(size=9 * n)
         ctx.sync()
         t0 = time.perf_counter()
         plan.factor(D, O1, O2)
@@ -188,7 +186,8 @@ def main():
         }
         a = time_to_tol(tbs.TBS_FP64)
         b = time_to_tol(tbs.TBS_FP32)
-        err = float(torch.linalg.norm(b["U"] - a["U"]) / torch.linalg.norm(a["U"]))
+        ua, ub = a["U"].numpy(), b["U"].numpy()
+        err = float(np.linalg.norm(ub - ua) / np.linalg.norm(ua))
         for d in (a, b):
             d.pop("U")
         out["time_to_1e-10"] = {"fp64": a, "fp32_factor": b, "fp32_vs_fp64_U_relerr": err}

@@ -60,6 +60,8 @@ int tbs_ctx_destroy(tbs_ctx *ctx);
 int tbs_ctx_set_stream(tbs_ctx *ctx, void *hip_stream);
 void *tbs_ctx_stream(tbs_ctx *ctx);
 int tbs_ctx_synchronize(tbs_ctx *ctx);
+/* Wait for all work on the context's device (every stream). */
+int tbs_device_synchronize(tbs_ctx *ctx);
 int tbs_ctx_nvar(tbs_ctx *ctx);
 /* Device memory helpers for hosts without another allocator. */
 int tbs_malloc(tbs_ctx *ctx, size_t bytes, void **dptr);
@@ -67,6 +69,11 @@ int tbs_free(tbs_ctx *ctx, void *dptr);
 int tbs_memcpy_h2d(tbs_ctx *ctx, void *dst, const void *src, size_t bytes);
 int tbs_memcpy_d2h(tbs_ctx *ctx, void *dst, const void *src, size_t bytes);
 int tbs_memset(tbs_ctx *ctx, void *dst, int value, size_t bytes);
+/* Timing events on the context's stream (hipEvent_t behind an opaque handle). */
+int tbs_event_create(tbs_ctx *ctx, void **ev);
+int tbs_event_destroy(tbs_ctx *ctx, void *ev);
+int tbs_event_record(tbs_ctx *ctx, void *ev);
+int tbs_event_elapsed_ms(void *ev_start, void *ev_stop, float *ms);
 
 /* ---------------------------------------------------------------------- */
 /* Drop-in kernels: one entry per active reference OKL kernel, same       */

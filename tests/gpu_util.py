@@ -5,15 +5,29 @@ import synth_np
 from triblocksolve_amd.solver import HostMesh
 
 
+_MEM = []
+
+
+def mctx():
+    """Context used for test allocations (HBM pointers are valid for every context on the device)."""
+    if not _MEM:
+        import triblocksolve_amd as tbs
+        _MEM.append(tbs.Context(0, 9))
+    return _MEM[0]
+
+
 def dev(a):
-    import torch
-    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    """Host array -> DeviceBuffer (library-allocated HBM)."""
+    return mctx().array(np.ascontiguousarray(a))
 
 
-def host(t):
-    import torch
-    torch.cuda.synchronize()
-    return t.detach().cpu().numpy()
+def zeros(n, dtype=np.float64):
+    return mctx().array(size=n, dtype=dtype)
+
+
+def host(buf):
+    """DeviceBuffer -> host numpy copy, after a device-wide synchronisation."""
+    return buf.numpy()
 
 
 def reversed_lines(mesh):

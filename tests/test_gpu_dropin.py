@@ -6,7 +6,7 @@ import pytest
 import closed_forms as cf
 import synth_np
 from conftest import system
-from gpu_util import cross_lines, dev, host, ragged_lines, reversed_lines
+from gpu_util import cross_lines, dev, host, mctx, ragged_lines, reversed_lines, zeros
 
 pytestmark = pytest.mark.gpu
 
@@ -55,7 +55,7 @@ def test_kernels_match_oracle(ctxs, oracle, name, m):
     print(f"case {name}", flush=True)
     ctx = ctxs(m.nvar)
     jac = _jac(m)
-    d = m.to_device()
+    d = m.to_device(mctx())
     D, O1, O2 = dev(jac["jacD"]), dev(jac["jacO1"]), dev(jac["jacO2"])
     nv, n = m.nvar, m.nelem
     # copies / add: exact
@@ -99,13 +99,12 @@ def test_kernels_match_oracle(ctxs, oracle, name, m):
 
 def run_dropin_loop(ctx, m, jac, iters):
     """src/main.cxx:180-229 with the drop-in kernels; returns (hist, U)."""
-    import torch
-    d = m.to_device()
+    d = m.to_device(mctx())
     nv, n = m.nvar, m.nelem
     D, O1, O2, rhs = dev(jac["jacD"]), dev(jac["jacO1"]), dev(jac["jacO2"]), dev(jac["rhs"])
-    z = lambda k: torch.zeros(k, dtype=torch.float64, device="cuda")
+    z = zeros
     DLU, DinvC, Am, U, dU, R = z(nv * nv * n), z(nv * nv * n), z(nv * nv * n), z(nv * n), z(nv * n), z(nv * n)
-    torch.cuda.synchronize()
+    mctx().sync()
     ctx.assembleTriBlocks(d, O1, O2, Am)
     ctx.copyAtoBjac(n, D, DLU)
     ctx.lineLU(n, m.nintface, d, DLU, O1, O2, DinvC)

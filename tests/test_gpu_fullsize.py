@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import closed_forms as cf
-from gpu_util import dev, host
+from gpu_util import dev, host, mctx, zeros
 
 pytestmark = pytest.mark.gpu
 
@@ -18,7 +18,6 @@ def relerr(a, b):
 
 
 def test_cfg2_against_oracle(tbs, oracle):
-    import torch
     print("case cfg2 360x384 fp64", flush=True)
     ctx = tbs.Context(0, 9)
     m = tbs.structured_mesh(360, 384)
@@ -27,7 +26,7 @@ def test_cfg2_against_oracle(tbs, oracle):
     ref = oracle.lij_run(m, jac, 8, nthreads=nthreads)
     plan = tbs.Plan(ctx, m)
     plan.factor(dev(jac["jacD"]), dev(jac["jacO1"]), dev(jac["jacO2"]))
-    U = torch.zeros(9 * m.nelem, dtype=torch.float64, device="cuda")
+    U = zeros(9 * m.nelem)
     hist, ns = plan.solve(dev(jac["rhs"]), U, 8, 0.0)
     ok, worst = cf.history_close(hist, ref["hist"])
     print(f"cfg2 history worst {worst:.3e} U relerr {relerr(host(U), ref['U']):.3e}")
@@ -38,23 +37,22 @@ def test_cfg2_against_oracle(tbs, oracle):
 
 
 def test_cfg4_properties(tbs):
-    import torch
     print("case cfg4 1200x1280 shift 12", flush=True)
     ctx = tbs.Context(0, 9)
     m = tbs.structured_mesh(1200, 1280)
     n, nif = m.nelem, m.nintface
-    z = lambda k: torch.empty(k, dtype=torch.float64, device="cuda")
+    z = zeros
     D, O1, O2, rhs = z(81 * n), z(81 * nif), z(81 * nif), z(9 * n)
     ctx.generate_device(4, n, nif, 12.0, D, O1, O2, rhs)
     ctx.sync()
     # device generator == host generator on a slice (counter-based streams)
     hs = tbs.generate_host(4, 9, 1000, 1000, 12.0)
-    assert np.array_equal(host(D[:81000]), hs["jacD"]) and np.array_equal(host(rhs[:9000]), hs["rhs"])
+    assert np.array_equal(D.numpy(count=81000), hs["jacD"]) and np.array_equal(rhs.numpy(count=9000), hs["rhs"])
     res = {}
     for prec in (0, 1):
         plan = tbs.Plan(ctx, m, prec)
         plan.factor(D, O1, O2)
-        U = torch.zeros(9 * n, dtype=torch.float64, device="cuda")
+        U = zeros(9 * n)
         hist, ns = plan.solve(rhs, U, 50, 1e-10)
         res[prec] = (hist, ns, host(U))
         print(f"prec {prec}: sweeps {ns} final {hist[-1] / hist[0]:.3e}", flush=True)
@@ -69,15 +67,15 @@ def test_cfg4_properties(tbs):
     # repeatability (FP64)
     plan = tbs.Plan(ctx, m, 0)
     plan.factor(D, O1, O2)
-    U = torch.zeros(9 * n, dtype=torch.float64, device="cuda")
+    U = zeros(9 * n)
     hist, ns = plan.solve(rhs, U, 50, 1e-10)
     assert np.array_equal(hist, h64) and np.array_equal(host(U), U64)
     plan.close()
     del U
     # drop-in kernel loop (the reference's launch sequence) agrees with the fused path
-    d = m.to_device()
+    d = m.to_device(mctx())
     DLU, DinvC, Am = z(81 * n), z(81 * n), z(81 * n)
-    Ud, dU, R = torch.zeros(9 * n, dtype=torch.float64, device="cuda"), z(9 * n), z(9 * n)
+    Ud, dU, R = zeros(9 * n), z(9 * n), z(9 * n)
     ctx.assembleTriBlocks(d, O1, O2, Am)
     ctx.copyAtoBjac(n, D, DLU)
     ctx.lineLU(n, nif, d, DLU, O1, O2, DinvC)

@@ -12,7 +12,7 @@ import pytest
 import closed_forms as cf
 import synth_np
 from conftest import system
-from gpu_util import cross_lines, dev, host, ragged_lines, reversed_lines
+from gpu_util import cross_lines, dev, host, mctx, ragged_lines, reversed_lines, zeros
 
 pytestmark = pytest.mark.gpu
 
@@ -35,11 +35,10 @@ def ctxs(tbs):
 
 
 def solve_plan(tbs, ctx, m, jac, iters, rtol=0.0, prec=0, want_R=False):
-    import torch
     plan = tbs.Plan(ctx, m, prec)
     plan.factor(dev(jac["jacD"]), dev(jac["jacO1"]), dev(jac["jacO2"]))
-    U = torch.zeros(m.nvar * m.nelem, dtype=torch.float64, device="cuda")
-    R = torch.zeros_like(U) if want_R else None
+    U = zeros(m.nvar * m.nelem)
+    R = zeros(U.size) if want_R else None
     hist, ns = plan.solve(dev(jac["rhs"]), U, iters, rtol, R_out=R)
     out = dict(hist=hist, ns=ns, U=host(U), R=host(R) if want_R else None)
     plan.close()
@@ -100,7 +99,6 @@ def test_fp32_factors_reach_fp64_accuracy(tbs, ctxs, oracle):
 def test_graph_replay_equals_eager(tbs, ctxs):
     """Several replays of a captured K-sweep graph, with eager work between them,
     give exactly the eager launches' history and solution."""
-    import torch
     m, jac = system(30, 40, 9, seed=2)
     ctx = ctxs(9)
     res = {}
@@ -112,8 +110,8 @@ def test_graph_replay_equals_eager(tbs, ctxs):
         for _ in range(3):
             plan.launch_sweeps(2, use_graph=(mode == "graph"))
             ctx.sync()
-            torch.ones(1000, device="cuda").sum().item()   # unrelated eager work
-        U = torch.zeros(9 * m.nelem, dtype=torch.float64, device="cuda")
+            zeros(1000).numpy().sum()   # unrelated work and a device-wide sync
+        U = zeros(9 * m.nelem)
         plan.end(U)
         ctx.sync()
         res[mode] = (plan.history(), host(U))
@@ -132,7 +130,6 @@ def test_deterministic(tbs, ctxs):
 def test_batch_equals_alone_and_oracle(tbs, ctxs, oracle):
     """Systems of uneven lines advanced by one launch per kernel: each result is
     bitwise the one it gets alone, and matches the oracle."""
-    import torch
     ctx = ctxs(9)
     specs = [system(20, 30, 9, seed=5)[0], reversed_lines(system(6, 17, 9)[0]),
              ragged_lines([5, 96, 7, 130, 64], 9), system(3, 200, 9)[0]]
@@ -143,7 +140,7 @@ def test_batch_equals_alone_and_oracle(tbs, ctxs, oracle):
         p.factor(dev(j["jacD"]), dev(j["jacO1"]), dev(j["jacO2"]))
         plans.append(p)
     batch = tbs.Batch(ctx, plans)
-    Us = [torch.zeros(9 * m.nelem, dtype=torch.float64, device="cuda") for m in specs]
+    Us = [zeros(9 * m.nelem) for m in specs]
     hists, ns = batch.solve([dev(j["rhs"]) for j in jacs], Us, 40, rtol=1e-10, sweeps_per_check=3)
     batch.close()
     for i, (m, j) in enumerate(zip(specs, jacs)):
@@ -164,9 +161,8 @@ def test_bad_inputs_raise(tbs, ctxs):
     with pytest.raises(tbs.TbsError):
         tbs.Plan(ctxs(9), bad)
     p = tbs.Plan(ctxs(9), m)
-    import torch
     with pytest.raises(tbs.TbsError) as ei:
-        p.solve(dev(jac["rhs"]), torch.zeros(9 * m.nelem, dtype=torch.float64, device="cuda"), 3)
+        p.solve(dev(jac["rhs"]), zeros(9 * m.nelem), 3)
     assert ei.value.code == -2
     p.close()
     with pytest.raises(tbs.TbsError):
@@ -175,13 +171,12 @@ def test_bad_inputs_raise(tbs, ctxs):
 
 def test_divergence_is_reported(tbs, ctxs):
     """Off-line blocks scaled so the iteration diverges: the solve stops with TBS_ERR_NUMERIC."""
-    import torch
     m, jac = system(10, 12, 9, seed=6, shift=0.5)
     for k in ("jacO1", "jacO2"):
         jac[k] = jac[k] * 40.0
     plan = tbs.Plan(ctxs(9), m)
     plan.factor(dev(jac["jacD"]), dev(jac["jacO1"]), dev(jac["jacO2"]))
     with pytest.raises(tbs.TbsError) as ei:
-        plan.solve(dev(jac["rhs"]), torch.zeros(9 * m.nelem, dtype=torch.float64, device="cuda"), 200, 1e-10)
+        plan.solve(dev(jac["rhs"]), zeros(9 * m.nelem), 200, 1e-10)
     assert ei.value.code == -6
     plan.close()

@@ -3,14 +3,15 @@
 The product is the HIP library libtriblock_hip.so (C ABI: include/triblock_hip.h)
 and the C++ host API / triblock.exe built on it (include/{Platform,Mesh,Jacobian}.hxx).
 This Python package is a thin ctypes layer over that C ABI for tests and the
-benchmark: device memory comes from torch, every computation runs in the HIP
-library, and a missing library raises instead of falling back to anything.
+benchmark: device memory and events come from the library itself, every
+computation runs in the HIP library, and a missing library raises instead of
+falling back to anything.
 """
 from ._capi import (TBS_FP32, TBS_FP64, TbsError, load, LIB_PATH)  # noqa: F401
-from .solver import (Context, StructuredMesh, HostMesh, Plan, Batch,  # noqa: F401
+from .solver import (Context, DeviceBuffer, StructuredMesh, HostMesh, Plan, Batch,  # noqa: F401
                      generate_host, structured_mesh, read_mesh, write_mesh,
                      read_jacobian, write_jacobian, model_bytes, reference_model_bytes)
 
-__all__ = ["Context", "StructuredMesh", "HostMesh", "Plan", "Batch", "TBS_FP32", "TBS_FP64",
+__all__ = ["Context", "DeviceBuffer", "StructuredMesh", "HostMesh", "Plan", "Batch", "TBS_FP32", "TBS_FP64",
            "TbsError", "load", "generate_host", "structured_mesh", "read_mesh", "write_mesh",
            "read_jacobian", "write_jacobian", "model_bytes", "reference_model_bytes"]

@@ -60,11 +60,16 @@ SIGNATURES = {
     "tbs_ctx_stream": [P],
     "tbs_ctx_synchronize": [P],
     "tbs_ctx_nvar": [P],
+    "tbs_device_synchronize": [P],
     "tbs_malloc": [P, C.c_size_t, PP],
     "tbs_free": [P, P],
     "tbs_memcpy_h2d": [P, P, P, C.c_size_t],
     "tbs_memcpy_d2h": [P, P, P, C.c_size_t],
     "tbs_memset": [P, P, I, C.c_size_t],
+    "tbs_event_create": [P, PP],
+    "tbs_event_destroy": [P, P],
+    "tbs_event_record": [P, P],
+    "tbs_event_elapsed_ms": [P, P, C.POINTER(C.c_float)],
     "tbs_copyAtoBjac": [P, I, P, P],
     "tbs_copyAtoB": [P, I, P, P],
     "tbs_addAtoB": [P, I, P, P],

@@ -98,6 +98,13 @@ int tbs_ctx_synchronize(tbs_ctx *ctx) {
     return TBS_OK;
 }
 
+int tbs_device_synchronize(tbs_ctx *ctx) {
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    TBS_HIP(hipDeviceSynchronize());
+    return TBS_OK;
+}
+
 int tbs_ctx_nvar(tbs_ctx *ctx) { return ctx ? ctx->nvar : TBS_ERR_ARG; }
 
 int tbs_malloc(tbs_ctx *ctx, size_t bytes, void **dptr) {
@@ -146,6 +153,38 @@ int tbs_memset(tbs_ctx *ctx, void *dst, int value, size_t bytes) {
     if (rc) return rc;
     if (bytes == 0) return TBS_OK;
     TBS_HIP(hipMemsetAsync(dst, value, bytes, ctx->stream));
+    return TBS_OK;
+}
+
+int tbs_event_create(tbs_ctx *ctx, void **ev) {
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    TBS_REQUIRE(ev != nullptr, TBS_ERR_ARG, "null output pointer");
+    hipEvent_t e;
+    TBS_HIP(hipEventCreate(&e));
+    *ev = (void *)e;
+    return TBS_OK;
+}
+
+int tbs_event_destroy(tbs_ctx *ctx, void *ev) {
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    if (ev) TBS_HIP(hipEventDestroy((hipEvent_t)ev));
+    return TBS_OK;
+}
+
+int tbs_event_record(tbs_ctx *ctx, void *ev) {
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    TBS_REQUIRE(ev != nullptr, TBS_ERR_ARG, "null event");
+    TBS_HIP(hipEventRecord((hipEvent_t)ev, ctx->stream));
+    return TBS_OK;
+}
+
+int tbs_event_elapsed_ms(void *ev_start, void *ev_stop, float *ms) {
+    TBS_REQUIRE(ev_start && ev_stop && ms, TBS_ERR_ARG, "null argument");
+    TBS_HIP(hipEventSynchronize((hipEvent_t)ev_stop));
+    TBS_HIP(hipEventElapsedTime(ms, (hipEvent_t)ev_start, (hipEvent_t)ev_stop));
     return TBS_OK;
 }
 

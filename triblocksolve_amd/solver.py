@@ -3,8 +3,8 @@
 Naming follows the reference host API (src/Mesh.cxx, src/Jacobian.cxx,
 src/main.cxx): meshes carry epoint/ef/fc/linesize/linepoint/lines/lineface
 (0-based), Jacobians jacD/jacO1/jacO2/rhs in the reference block layout.
-Device arrays are torch tensors on a HIP device; all arithmetic happens in
-libtriblock_hip.so.
+Device arrays are DeviceBuffer objects (HBM allocated through the library's
+own C ABI); all arithmetic happens in libtriblock_hip.so.
 """
 import ctypes as C
 from dataclasses import dataclass
@@ -15,9 +15,11 @@ from . import _capi as A
 
 
 def _ptr(x):
-    """c_void_p of a numpy array or a torch tensor (None -> NULL)."""
+    """c_void_p of a numpy array, a DeviceBuffer or an int address (None -> NULL)."""
     if x is None:
         return C.c_void_p(None)
+    if isinstance(x, DeviceBuffer):
+        return C.c_void_p(x.ptr)
     if isinstance(x, np.ndarray):
         if not x.flags["C_CONTIGUOUS"]:
             raise ValueError("array must be C-contiguous")
@@ -29,6 +31,55 @@ def _ptr(x):
     if isinstance(x, int):
         return C.c_void_p(x)
     raise TypeError(f"cannot take a pointer of {type(x)}")
+
+
+class DeviceBuffer:
+    """HBM allocated through the library (tbs_malloc), freed on close/GC.
+
+    Typed view for host transfers: `dtype` and `size` (elements)."""
+
+    def __init__(self, ctx, size, dtype=np.float64, data=None, zero=False):
+        self.ctx = ctx
+        self.dtype = np.dtype(dtype)
+        self.size = int(size)
+        self.nbytes = self.size * self.dtype.itemsize
+        p = C.c_void_p()
+        A.check(ctx._lib.tbs_malloc(ctx.handle, self.nbytes, C.byref(p)))
+        self.ptr = p.value or 0
+        if data is not None:
+            self.upload(data)
+        elif zero:
+            self.zero()
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.size != self.size:
+            raise ValueError(f"size {a.size} != buffer size {self.size}")
+        A.check(self.ctx._lib.tbs_memcpy_h2d(self.ctx.handle, C.c_void_p(self.ptr), C.c_void_p(a.ctypes.data),
+                                             self.nbytes))
+
+    def numpy(self, count=None, offset=0):
+        count = self.size - offset if count is None else count
+        out = np.empty(count, self.dtype)
+        A.check(self.ctx._lib.tbs_device_synchronize(self.ctx.handle))
+        A.check(self.ctx._lib.tbs_memcpy_d2h(self.ctx.handle, C.c_void_p(out.ctypes.data),
+                                             C.c_void_p(self.ptr + offset * self.dtype.itemsize),
+                                             count * self.dtype.itemsize))
+        return out
+
+    def zero(self):
+        A.check(self.ctx._lib.tbs_memset(self.ctx.handle, C.c_void_p(self.ptr), 0, self.nbytes))
+
+    def close(self):
+        if self.ptr:
+            self.ctx._lib.tbs_free(self.ctx.handle, C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 MESH_FIELDS = ("epoint", "ef", "fc", "linesize", "linepoint", "lines", "lineface")
@@ -83,9 +134,9 @@ class HostMesh:
         s._keep = self  # keep arrays alive
         return s
 
-    def to_device(self, device="cuda"):
-        import torch
-        return {f: torch.from_numpy(np.ascontiguousarray(getattr(self, f))).to(device) for f in MESH_FIELDS}
+    def to_device(self, ctx):
+        """Device copies of the connectivity arrays (dict of DeviceBuffer, int32)."""
+        return {f: DeviceBuffer(ctx, getattr(self, f).size, np.int32, data=getattr(self, f)) for f in MESH_FIELDS}
 
 
 def structured_mesh(NL, NK, nvar=9):
@@ -199,6 +250,29 @@ class Context:
     def stream(self):
         return self._lib.tbs_ctx_stream(self.handle)
 
+    def array(self, data=None, size=None, dtype=np.float64, zero=True):
+        """DeviceBuffer holding `data` (host array) or `size` zeroed elements."""
+        if data is not None:
+            data = np.ascontiguousarray(data)
+            return DeviceBuffer(self, data.size, data.dtype, data=data)
+        return DeviceBuffer(self, size, dtype, zero=zero)
+
+    def event(self):
+        e = C.c_void_p()
+        A.check(self._lib.tbs_event_create(self.handle, C.byref(e)))
+        return e
+
+    def record(self, ev):
+        A.check(self._lib.tbs_event_record(self.handle, ev))
+
+    def elapsed_ms(self, a, b):
+        ms = C.c_float()
+        A.check(self._lib.tbs_event_elapsed_ms(a, b, C.byref(ms)))
+        return ms.value
+
+    def destroy_event(self, ev):
+        self._lib.tbs_event_destroy(self.handle, ev)
+
     # --- drop-in kernels (device tensors) ---
     def copyAtoBjac(self, nelem, a, b):
         A.check(self._lib.tbs_copyAtoBjac(self.handle, nelem, _ptr(a), _ptr(b)))
@@ -210,23 +284,23 @@ class Context:
         A.check(self._lib.tbs_addAtoB(self.handle, nelem, _ptr(a), _ptr(b)))
 
     def assembleTriBlocks(self, m, jacO1, jacO2, Aout):
-        A.check(self._lib.tbs_assembleTriBlocks(self.handle, m["linesize"].numel(), _ptr(m["linesize"]),
+        A.check(self._lib.tbs_assembleTriBlocks(self.handle, m["linesize"].size, _ptr(m["linesize"]),
                                                 _ptr(m["linepoint"]), _ptr(m["lines"]), _ptr(m["lineface"]),
                                                 _ptr(m["fc"]), _ptr(jacO1), _ptr(jacO2), _ptr(Aout)))
 
     def lineLU(self, nelem, nintface, m, Dia, Of1, Of2, DinvC):
-        A.check(self._lib.tbs_lineLU(self.handle, nelem, nintface, m["linesize"].numel(), _ptr(m["fc"]),
+        A.check(self._lib.tbs_lineLU(self.handle, nelem, nintface, m["linesize"].size, _ptr(m["fc"]),
                                      _ptr(m["linesize"]), _ptr(m["linepoint"]), _ptr(m["lines"]),
                                      _ptr(m["lineface"]), _ptr(Dia), _ptr(Of1), _ptr(Of2), _ptr(DinvC)))
 
     def solveDU(self, nelem, nintface, m, Dia, DinvC, Amat, dU, R):
-        A.check(self._lib.tbs_triblock_solveDU(self.handle, nelem, nintface, m["ef"].numel(), m["linesize"].numel(),
-                                               m["lines"].numel(), _ptr(m["linesize"]), _ptr(m["linepoint"]),
+        A.check(self._lib.tbs_triblock_solveDU(self.handle, nelem, nintface, m["ef"].size, m["linesize"].size,
+                                               m["lines"].size, _ptr(m["linesize"]), _ptr(m["linepoint"]),
                                                _ptr(m["lines"]), _ptr(Dia), _ptr(DinvC), _ptr(Amat), _ptr(dU), _ptr(R)))
 
     def lineRes(self, nelem, nintface, m, Dia, Of1, Of2, U, R):
-        A.check(self._lib.tbs_triblock_lineRes(self.handle, nelem, nintface, m["ef"].numel(), m["linesize"].numel(),
-                                               m["lines"].numel(), _ptr(m["epoint"]), _ptr(m["ef"]), _ptr(m["fc"]),
+        A.check(self._lib.tbs_triblock_lineRes(self.handle, nelem, nintface, m["ef"].size, m["linesize"].size,
+                                               m["lines"].size, _ptr(m["epoint"]), _ptr(m["ef"]), _ptr(m["fc"]),
                                                _ptr(m["linesize"]), _ptr(m["linepoint"]), _ptr(m["lines"]),
                                                _ptr(Dia), _ptr(Of1), _ptr(Of2), _ptr(U), _ptr(R)))
 
