@@ -39,6 +39,18 @@ def parse():
     return ap.parse_args()
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t[workload][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(tbs, args, nelem_full):
     """The oracle (OpenMP over lines, all host cores) on a bounded slice of the
     workload: the first lines of the same grid, same generator, 3 timed sweeps."""
@@ -85,6 +97,7 @@ def main():
         ctx.sync()
         t_factor = time.perf_counter() - t0
         info = plan.info()
+        tuned = plan.autotune(reps=3)
         plan.begin(rhs, None)
         for _ in range(warmup):
             plan.launch_thomas()
@@ -115,7 +128,8 @@ def main():
         ctx.sync()
         t_graph = time.perf_counter() - t1
         plan.close()
-        return dict(wall=wall, thomas=th, residual=rs, info=info, t_factor=t_factor, t_graph=t_graph)
+        return dict(wall=wall, thomas=th, residual=rs, info=info, t_factor=t_factor, t_graph=t_graph,
+                    tuned=tuned)
 
     def time_to_tol(prec, rtol=1e-10):
         plan = tbs.Plan(ctx, m, prec)
@@ -147,7 +161,8 @@ def main():
     }
     dom = "residual" if t_rs >= t_th else "thomas"
     roof = {"bound": "hbm", "kernel": dom, "achieved": kern[dom]["GBps"], "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": kern[dom]["GBps"] / HBM_PEAK_GBS, "traffic": None}
+            "unit": "GB/s", "frac": kern[dom]["GBps"] / HBM_PEAK_GBS,
+            "traffic": measured_traffic(f"lij_{args.nl}x{args.nk}", dom)}
     out = {
         "metric": "LIJ sweeps/s at 1.5M elem 9x9 (FP64)",
         "value": sweeps_per_s,
@@ -173,6 +188,7 @@ def main():
         "roofline": roof,
         "graph_sweeps_per_s": args.steps / r64["t_graph"],
         "factor_s": r64["t_factor"],
+        "autotune": {"chosen_variant": r64["tuned"][0], "thomas_us_by_variant": r64["tuned"][1]},
     }
     if not args.no_extras:
         r32 = timed_sweeps(tbs.TBS_FP32, args.steps, args.warmup)

@@ -171,6 +171,15 @@ int tbs_plan_solve(tbs_plan *plan, const double *rhs, double *U,
                    int max_sweeps, double rtol, double *hist, int *nsweeps,
                    double *R_final);
 
+/* Autotuner: time every built Thomas prefetch-depth variant on this plan
+ * (reps launches each, HIP events), keep the fastest.  Resets the iterate:
+ * call before tbs_plan_begin / tbs_plan_solve.  us[v] = mean microseconds.  */
+int tbs_plan_autotune(tbs_plan *plan, int reps, int *chosen, double *us, int cap);
+int tbs_plan_set_variant(tbs_plan *plan, int variant);
+/* Number of Thomas variants; pf[v] = 100 * (windows prefetched) + (ring
+ * cycles unrolled per loop iteration) of variant v. */
+int tbs_plan_variants(tbs_plan *plan, int *pf, int cap);
+
 /* Low-level stepping, for timing and graph capture (all asynchronous). */
 int tbs_plan_begin(tbs_plan *plan, const double *rhs, const double *U0);
 int tbs_plan_launch_thomas(tbs_plan *plan);
@@ -178,6 +187,10 @@ int tbs_plan_launch_residual(tbs_plan *plan);   /* residual + norm of sweep */
 int tbs_plan_launch_sweeps(tbs_plan *plan, int nsweeps, int use_graph);
 int tbs_plan_end(tbs_plan *plan, double *U, double *R_final);
 int tbs_plan_history(tbs_plan *plan, double *hist, int cap, int *count);
+/* Diagnostics: device address and byte size of an internal array
+ * (0 g, 1 d', 2 U, 3 rhs, 4 R, 5 W, 6 E windows, 7 Gamma windows,
+ * 8 residual blocks); vectors are in line-major internal order. */
+int tbs_plan_internal(tbs_plan *plan, int which, void **dptr, size_t *bytes);
 
 /* ---------------------------------------------------------------------- */
 /* Batched solves: many independent plans (uneven line lengths) advanced   */
@@ -186,6 +199,7 @@ int tbs_plan_history(tbs_plan *plan, double *hist, int cap, int *count);
 int tbs_batch_create(tbs_ctx *ctx, tbs_plan *const *plans, int nplans,
                      tbs_batch **out);
 int tbs_batch_destroy(tbs_batch *batch);
+int tbs_batch_autotune(tbs_batch *batch, int reps, int *chosen, double *us, int cap);
 /* rhs[i], U[i]: device pointers of plan i; hist: host [nplans][max_sweeps+1];
  * nsweeps: host [nplans].                                                  */
 int tbs_batch_solve(tbs_batch *batch, const double *const *rhs,

@@ -84,14 +84,19 @@ SIGNATURES = {
     "tbs_plan_info_get": [P, C.POINTER(PlanInfo)],
     "tbs_plan_factor": [P, P, P, P],
     "tbs_plan_solve": [P, P, P, I, D, P, C.POINTER(C.c_int), P],
+    "tbs_plan_autotune": [P, I, C.POINTER(C.c_int), P, I],
+    "tbs_plan_set_variant": [P, I],
+    "tbs_plan_variants": [P, P, I],
     "tbs_plan_begin": [P, P, P],
     "tbs_plan_launch_thomas": [P],
     "tbs_plan_launch_residual": [P],
     "tbs_plan_launch_sweeps": [P, I, I],
     "tbs_plan_end": [P, P, P],
     "tbs_plan_history": [P, P, I, C.POINTER(C.c_int)],
+    "tbs_plan_internal": [P, I, PP, C.POINTER(C.c_size_t)],
     "tbs_batch_create": [P, P, I, PP],
     "tbs_batch_destroy": [P],
+    "tbs_batch_autotune": [P, I, C.POINTER(C.c_int), P, I],
     "tbs_batch_solve": [P, P, P, I, D, I, P, P],
     "tbs_structured_sizes": [I, I, P, P, P, P, P],
     "tbs_structured_mesh": [I, I, P, P, P, P, P, P, P],

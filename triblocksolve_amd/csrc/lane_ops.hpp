@@ -24,3 +24,89 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 }  // namespace tbs
+
+// ---------------------------------------------------------------------------
+// Buffer-resource (V#) loads and stores.  The descriptor holds a wave-uniform
+// base and a byte range; a lane whose offset falls outside the range reads 0
+// and its store is dropped, which is how the sweep kernels handle ragged line
+// ends, padded lanes and padded groups without branches.  Ranges are < 4 GiB.
+namespace tbs {
+
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+
+// Word 3 of the descriptor for gfx9-family raw buffers: DATA_FORMAT = 32.
+constexpr int kRsrcWord3 = 0x00020000;
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void *base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, kRsrcWord3);
+}
+
+// Offset used by lanes that must neither load nor store.
+constexpr uint32_t kOOB = 0xFFFFFF00u;
+
+template <typename T>
+__device__ __forceinline__ T bload(rsrc_t r, uint32_t voff, uint32_t soff);
+
+template <>
+__device__ __forceinline__ double bload<double>(rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+template <>
+__device__ __forceinline__ float bload<float>(rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+template <>
+__device__ __forceinline__ int bload<int>(rsrc_t r, uint32_t voff, uint32_t soff) {
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0);
+}
+
+// 16 bytes = 2 doubles or 4 floats.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ u32x4 bload16(rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0);
+}
+template <typename T>
+__device__ __forceinline__ T lane_elem(const u32x4 &v, int k);
+template <>
+__device__ __forceinline__ double lane_elem<double>(const u32x4 &v, int k) {
+    const unsigned long long b = (unsigned long long)v[2 * k] | ((unsigned long long)v[2 * k + 1] << 32);
+    return __builtin_bit_cast(double, b);
+}
+template <>
+__device__ __forceinline__ float lane_elem<float>(const u32x4 &v, int k) {
+    // copy out first: bit-casting the element lvalue v[k] reads the vector's
+    // first element whatever k is
+    const unsigned int u = v[k];
+    return __builtin_bit_cast(float, u);
+}
+
+template <typename T>
+__device__ __forceinline__ void bstore(rsrc_t r, T v, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ void bstore<double>(rsrc_t r, double v, uint32_t voff, uint32_t soff) {
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)voff, (int)soff, 0);
+}
+template <>
+__device__ __forceinline__ void bstore<float>(rsrc_t r, float v, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)voff, (int)soff, 0);
+}
+
+// Lane-interleaved block layout ("SoA-64"): a group of 64 elements stores each
+// BS-entry block as NC chunks of VW = 16/sizeof(T) consecutive entries per
+// lane ([chunk][lane][VW]), then the BS - NC*VW remaining entries
+// ([entry][lane]); every chunk load of a wave is 1 KiB contiguous.
+template <int BS, typename T>
+struct Soa64 {
+    static constexpr int VW = 16 / (int)sizeof(T);
+    static constexpr int NC = BS / VW;
+    static constexpr int REM = BS - NC * VW;
+    static constexpr int ELEMS = BS * 64;            // T values per block slot
+    // index (in T units, within one slot) of entry q of lane `lane`
+    __host__ __device__ static constexpr long index(int q, int lane) {
+        return q < NC * VW ? (long)(q / VW) * 64 * VW + (long)lane * VW + q % VW
+                           : (long)NC * VW * 64 + (long)(q - NC * VW) * 64 + lane;
+    }
+};
+
+}  // namespace tbs

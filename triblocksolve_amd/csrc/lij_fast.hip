@@ -35,6 +35,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <climits>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -46,19 +47,25 @@ using tbs::rdlane;
 
 constexpr int kGroup = 64;      // elements per residual group (one wave)
 constexpr int kHistCap = 4096;  // sweeps of history kept per system
+constexpr int kPad = 64;                  // zero elements before and after U, rhs, R
+constexpr int kSlotNone = INT_MIN;        // no lane of the group has this face
+constexpr int kSlotGather = INT_MIN + 1;  // irregular neighbours: per-lane gather
 
 template <typename TF>
 struct SysDev {
     const double *blk;   // [ngroups][F+1][BS][64] residual blocks (slot 0 = D)
     const int *nbr;      // [ngroups][F][64] neighbour internal index or -1
+    const int *slotc;    // [ngroups][F] neighbour map of the slot: kSlotNone,
+                         // kSlotGather, or the shift c with nbr = m + c for
+                         // every lane that has a face in this slot
     const TF *W;         // [ngroups][BS][64] D'^-1
     const TF *EF;        // [windows][NV][SPW*NV] E_k rows, zero padded
     const TF *GB;        // [windows][NV][SPW*NV] Gamma_k rows, zero padded
-    const double *rhs;   // [n][NV] internal order
-    double *U;           // [n][NV]
+    const double *rhs;   // [n][NV] internal order; kPad zero elements either side
+    double *U;           // [n][NV]; kPad zero elements either side
     TF *g;               // [n][NV] Thomas right-hand side
     TF *dUp;             // [n][NV] forward-sweep intermediate
-    double *R;           // [n][NV] residual (written on request)
+    double *R;           // [n][NV] residual (written on request); padded like U
     double *gpart;       // [ngroups] sum of R^2 per group
     double *hist;        // [kHistCap]
     int *state;          // [0] = sweeps recorded, [1] = active, [2] = status
@@ -66,85 +73,163 @@ struct SysDev {
 };
 
 // ----------------------------------------------------------- residual --
+// acc[i] += B(i,j) x[j] over one SoA-64 block slot, in entry order q = i + NV*j
+// (ascending j for every row i, the reference's summation order).
+template <int NV, typename T>
+__device__ __forceinline__ void slot_matvec(tbs::rsrc_t r, uint32_t slot_soff, int lane,
+                                            const double (&x)[NV], double (&acc)[NV]) {
+    using L = tbs::Soa64<NV * NV, T>;
+#pragma unroll
+    for (int c = 0; c < L::NC; ++c) {
+        const tbs::u32x4 v = tbs::bload16(r, (uint32_t)lane * 16u, slot_soff + (uint32_t)c * 1024u);
+#pragma unroll
+        for (int k = 0; k < L::VW; ++k) {
+            const int q = c * L::VW + k;
+            acc[q % NV] += (double)tbs::lane_elem<T>(v, k) * x[q / NV];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < L::REM; ++k) {
+        const int q = L::NC * L::VW + k;
+        const T b = tbs::bload<T>(r, (uint32_t)lane * sizeof(T),
+                                  slot_soff + (uint32_t)(L::NC * L::VW * 64 + k * 64) * sizeof(T));
+        acc[q % NV] += (double)b * x[q / NV];
+    }
+}
+
+template <int NV>
+__device__ __forceinline__ void load_vec(tbs::rsrc_t r, uint32_t byte_off, double (&v)[NV]) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) v[j] = tbs::bload<double>(r, byte_off + 8u * j, 0);
+}
+
+// Copy the vector tile of elements [m_base, m_base + 64) (AoS, NV doubles per
+// element, contiguous) into this wave's LDS tile with 16-byte loads; elements
+// outside the vector read as 0.  Each lane then reads its own element.
+template <int NV>
+__device__ __forceinline__ void stage_tile(tbs::rsrc_t r, int m_base, int lane, double *tile) {
+    constexpr int TB = kGroup * NV * 8;          // tile bytes
+    constexpr int NC = TB / 1024;                // full 1 KiB wave loads
+    const uint32_t base = (uint32_t)(m_base * NV * 8);
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        const tbs::u32x4 v = tbs::bload16(r, base + 16u * lane + 1024u * k, 0);
+        tile[2 * lane + 128 * k] = tbs::lane_elem<double>(v, 0);
+        tile[2 * lane + 128 * k + 1] = tbs::lane_elem<double>(v, 1);
+    }
+    static_assert(TB - NC * 1024 == 0 || TB - NC * 1024 == 512, "tile = 512*NV bytes");
+    if (TB - NC * 1024 == 512)   // odd NV: one more 8-byte load per lane
+        tile[NC * 128 + lane] = tbs::bload<double>(r, base + (uint32_t)(NC * 1024) + 8u * lane, 0);
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int NV>
+__device__ __forceinline__ void read_tile(const double *tile, int lane, double (&v)[NV]) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) v[j] = tile[lane * NV + j];
+    __builtin_amdgcn_wave_barrier();
+}
+
 template <int NV, typename TF>
 __global__ __launch_bounds__(256) void fres_kernel(const SysDev<TF> *__restrict__ sys,
                                                    const int2 *__restrict__ gtab, int ng,
                                                    int want_R, int force) {
     constexpr int BS = NV * NV;
-    const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+    __shared__ double tiles[4][kGroup * NV];
+    // XCD-aware order: hardware block b runs on XCD b % 8; give each XCD a
+    // contiguous run of groups so neighbouring lines share an L2.
+    const int nblk = (ng + 3) / 4;
+    const int per = (nblk + 7) / 8;
+    const int blk = (int)(blockIdx.x % 8u) * per + (int)(blockIdx.x / 8u);
+    if (blk >= nblk) return;
+    const int wave = threadIdx.x >> 6;
+    const int wid = __builtin_amdgcn_readfirstlane(blk * 4 + wave);
     if (wid >= ng) return;
+    double *tile = tiles[wave];
     const int lane = threadIdx.x & 63;
     const int2 sg = gtab[wid];
     const SysDev<TF> &S = sys[sg.x];
     if (!force && !S.state[1]) return;
     const int g = sg.y;
-    const int m = g * kGroup + lane;
-    const bool valid = m < S.n;
     const int F = S.F;
-    const double *B = S.blk + (size_t)g * (F + 1) * BS * kGroup + lane;
+    const int m = g * kGroup + lane;
+    const int m0 = g * kGroup;
+    constexpr uint32_t kSlot = (uint32_t)BS * kGroup * sizeof(double);
+    const tbs::rsrc_t rB = tbs::make_rsrc(S.blk + (size_t)g * (F + 1) * BS * kGroup, (F + 1) * kSlot);
+    const tbs::rsrc_t rN = tbs::make_rsrc(S.nbr + (size_t)g * F * kGroup, (uint32_t)F * kGroup * 4u);
+    const tbs::rsrc_t rW = tbs::make_rsrc(S.W + (size_t)g * BS * kGroup, (uint32_t)BS * kGroup * sizeof(TF));
+    // U and rhs descriptors start kPad elements before element 0, so tiles
+    // shifted by a neighbour offset never start at a negative offset while
+    // they still touch real elements.
+    const uint32_t pbytes = (uint32_t)(S.n + 2 * kPad) * NV * 8u;
+    const tbs::rsrc_t rU = tbs::make_rsrc(S.U - kPad * NV, pbytes);
+    const tbs::rsrc_t rRhs = tbs::make_rsrc(S.rhs - kPad * NV, pbytes);
+    const uint32_t moff = (uint32_t)m * NV * 8u;
 
     double u[NV], acc[NV];
-#pragma unroll
-    for (int j = 0; j < NV; ++j) u[j] = valid ? S.U[(size_t)NV * m + j] : 0.0;
+    stage_tile<NV>(rU, m0 + kPad, lane, tile);
+    read_tile<NV>(tile, lane, u);
 #pragma unroll
     for (int i = 0; i < NV; ++i) acc[i] = 0.0;
-#pragma unroll
-    for (int j = 0; j < NV; ++j) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) acc[i] += B[(size_t)(i + NV * j) * kGroup] * u[j];
-    }
+    slot_matvec<NV, double>(rB, 0u, lane, u, acc);                     // D U
     for (int f = 0; f < F; ++f) {
-        const int nb = S.nbr[((size_t)g * F + f) * kGroup + lane];
-        if (nb >= 0) {
-            const double *O = B + (size_t)(1 + f) * BS * kGroup;
-            double un[NV], t[NV];
-#pragma unroll
-            for (int j = 0; j < NV; ++j) un[j] = S.U[(size_t)NV * nb + j];
+        const int c = S.slotc[g * F + f];
+        if (c == kSlotNone) continue;
+        const int nb = tbs::bload<int>(rN, (uint32_t)(f * kGroup + lane) * 4u, 0);
+        double un[NV];
+        if (c != kSlotGather) {                                           // wave-uniform
+            stage_tile<NV>(rU, m0 + c + kPad, lane, tile);
+            read_tile<NV>(tile, lane, un);
+        }
+        if (nb >= 0) {                                                    // interior face
+            if (c == kSlotGather) load_vec<NV>(rU, (uint32_t)(nb + kPad) * NV * 8u, un);
+            double t[NV];
 #pragma unroll
             for (int i = 0; i < NV; ++i) t[i] = 0.0;
-#pragma unroll
-            for (int j = 0; j < NV; ++j) {
-#pragma unroll
-                for (int i = 0; i < NV; ++i) t[i] += O[(size_t)(i + NV * j) * kGroup] * un[j];
-            }
+            slot_matvec<NV, double>(rB, (uint32_t)(1 + f) * kSlot, lane, un, t);
 #pragma unroll
             for (int i = 0; i < NV; ++i) acc[i] += t[i];
         }
     }
-    double r[NV];
+    double rh[NV], r[NV], nr[NV];
+    stage_tile<NV>(rRhs, m0 + kPad, lane, tile);
+    read_tile<NV>(tile, lane, rh);
     double rr = 0.0;
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
-        r[i] = valid ? S.rhs[(size_t)NV * m + i] + acc[i] : 0.0;
+        r[i] = rh[i] + acc[i];                                            // 0 for padded lanes
         rr += r[i] * r[i];
+        nr[i] = -r[i];
     }
-    const TF *W = S.W + (size_t)g * BS * kGroup + lane;
     double gv[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) gv[i] = 0.0;
+    slot_matvec<NV, TF>(rW, 0u, lane, nr, gv);                          // g = D'^-1 (-R)
+    const tbs::rsrc_t rG = tbs::make_rsrc(S.g, (uint32_t)S.n * NV * sizeof(TF));
+    const uint32_t goff = (uint32_t)m * NV * sizeof(TF);
 #pragma unroll
-    for (int j = 0; j < NV; ++j) {
+    for (int i = 0; i < NV; ++i) tbs::bstore<TF>(rG, (TF)gv[i], goff + (uint32_t)i * sizeof(TF), 0);
+    if (want_R) {
+        const tbs::rsrc_t rR = tbs::make_rsrc(S.R, (uint32_t)S.n * NV * 8u);
 #pragma unroll
-        for (int i = 0; i < NV; ++i) gv[i] += (double)W[(size_t)(i + NV * j) * kGroup] * (-r[j]);
-    }
-    if (valid) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) S.g[(size_t)NV * m + i] = (TF)gv[i];
-        if (want_R) {
-#pragma unroll
-            for (int i = 0; i < NV; ++i) S.R[(size_t)NV * m + i] = r[i];
-        }
+        for (int i = 0; i < NV; ++i) tbs::bstore<double>(rR, r[i], moff + 8u * i, 0);
     }
     rr = tbs::wave_sum(rr);
     if (lane == 0) S.gpart[g] = rr;
 }
 
 // -------------------------------------------------------------- Thomas --
-template <int NV, typename TF>
+// One wave per line; windows of SPW steps, PF windows prefetched ahead.
+// The window loop is unrolled K ring cycles (K*NB windows) per iteration:
+// the compiler drains all in-flight loads at a loop header, so a longer body
+// means fewer drains per line.
+template <int NV, typename TF, int PF, int K>
 __global__ __launch_bounds__(64) void fthomas_kernel(const SysDev<TF> *__restrict__ sys,
                                                      const int4 *__restrict__ ltab, int nl) {
     constexpr int SPW = 64 / NV;      // steps per window
     constexpr int LW = SPW * NV;      // lanes carrying data
+    constexpr int NB = PF + 1;        // register ring depth
+    constexpr int G = K * NB;         // windows per loop iteration
     const int l = blockIdx.x;
     if (l >= nl) return;
     const int4 lt = ltab[l];          // {system, m0, n, window offset}
@@ -152,103 +237,143 @@ __global__ __launch_bounds__(64) void fthomas_kernel(const SysDev<TF> *__restric
     if (!S.state[1]) return;
     const int lane = threadIdx.x;
     const int st_of_lane = lane / NV;
-    const int row = lane - st_of_lane * NV;
     const bool lane_ok = lane < LW;
     const int m0 = lt.y, n = lt.z;
     const int nw = (n + SPW - 1) / SPW;
-    const TF *EF = S.EF + (size_t)lt.w * NV * LW;
-    const TF *GB = S.GB + (size_t)lt.w * NV * LW;
+    const uint32_t wbytes = (uint32_t)NV * LW * sizeof(TF);           // one window of factors
+    const tbs::rsrc_t rEF = tbs::make_rsrc(S.EF + (size_t)lt.w * NV * LW, (uint32_t)nw * wbytes);
+    const tbs::rsrc_t rGB = tbs::make_rsrc(S.GB + (size_t)lt.w * NV * LW, (uint32_t)nw * wbytes);
+    const uint32_t vb = (uint32_t)n * NV;
+    const tbs::rsrc_t rg = tbs::make_rsrc(S.g + (size_t)NV * m0, vb * sizeof(TF));
+    const tbs::rsrc_t rd = tbs::make_rsrc(S.dUp + (size_t)NV * m0, vb * sizeof(TF));
+    const tbs::rsrc_t rU = tbs::make_rsrc(S.U + (size_t)NV * m0, vb * 8u);
+    // element index of this lane's (step, row) inside window w: w*LW + lane
+    // Offsets for windows outside [0, nw) or lane >= LW are out of range by
+    // construction (negative windows wrap to > 2^31 bytes, far past any range).
+    auto vidx = [&](int w) -> uint32_t {
+        return (lane_ok && w >= 0) ? (uint32_t)(w * LW + lane) : (tbs::kOOB >> 3);
+    };
+    auto foff = [&](int w, int j) -> uint32_t {
+        return w >= 0 ? ((uint32_t)(w * NV + j) * LW + lane) * sizeof(TF) : tbs::kOOB;
+    };
 
-    // ---- forward: d_k = g_k - E_k d_{k-1}
     TF d[NV];
+    TF eb[NB][NV];
+    TF gb[NB];
+    // ---- forward: d_k = g_k - E_k d_{k-1}
 #pragma unroll
     for (int j = 0; j < NV; ++j) d[j] = TF(0);
-    TF ec[NV], en[NV];
-    TF gc, gn;
-    auto load_f = [&](int w, TF (&e)[NV], TF &gg) {
-        const int k = w * SPW + st_of_lane;
-        const bool ok = lane_ok && k < n;
 #pragma unroll
-        for (int j = 0; j < NV; ++j) e[j] = lane_ok ? EF[((size_t)w * NV + j) * LW + lane] : TF(0);
-        gg = ok ? S.g[(size_t)NV * (m0 + k) + row] : TF(0);
-    };
-    load_f(0, ec, gc);
-    for (int w = 0; w < nw; ++w) {
-        if (w + 1 < nw) load_f(w + 1, en, gn);
-        TF out = TF(0);
+    for (int p = 0; p < PF; ++p) {
 #pragma unroll
-        for (int st = 0; st < SPW; ++st) {
-            TF x = TF(0);
+        for (int j = 0; j < NV; ++j) eb[p][j] = tbs::bload<TF>(rEF, foff(p, j), 0);
+        gb[p] = tbs::bload<TF>(rg, vidx(p) * sizeof(TF), 0);
+    }
+    // Windows past the line end need no branch: their loads read 0 and their
+    // stores fall outside the descriptors, so each group of NB windows is one
+    // straight-line block and the prefetches stay in flight across it.
+    for (int w0 = 0; w0 < nw; w0 += G) {
 #pragma unroll
-            for (int j = 0; j < NV; ++j) x += ec[j] * d[j];
-            const TF v = gc - x;
+        for (int bb = 0; bb < G; ++bb) {
+            const int w = w0 + bb;
+            const int b = bb % NB;
+            const int bn = (b + PF) % NB;
 #pragma unroll
-            for (int j = 0; j < NV; ++j) d[j] = rdlane(v, st * NV + j);
-            out = (st_of_lane == st) ? v : out;
-        }
-        const int k = w * SPW + st_of_lane;
-        if (lane_ok && k < n) S.dUp[(size_t)NV * (m0 + k) + row] = out;
-        if (w + 1 < nw) {
+            for (int j = 0; j < NV; ++j) eb[bn][j] = tbs::bload<TF>(rEF, foff(w + PF, j), 0);
+            gb[bn] = tbs::bload<TF>(rg, vidx(w + PF) * sizeof(TF), 0);
+            TF out = TF(0);
 #pragma unroll
-            for (int j = 0; j < NV; ++j) ec[j] = en[j];
-            gc = gn;
+            for (int st = 0; st < SPW; ++st) {
+                TF x0 = TF(0), x1 = TF(0), x2 = TF(0);
+#pragma unroll
+                for (int j = 0; j < NV; j += 3) {
+                    x0 += eb[b][j] * d[j];
+                    if (j + 1 < NV) x1 += eb[b][j + 1] * d[j + 1];
+                    if (j + 2 < NV) x2 += eb[b][j + 2] * d[j + 2];
+                }
+                const TF v = gb[b] - ((x0 + x1) + x2);
+#pragma unroll
+                for (int j = 0; j < NV; ++j) d[j] = rdlane(v, st * NV + j);
+                out = (st_of_lane == st) ? v : out;
+            }
+            tbs::bstore<TF>(rd, out, vidx(w) * sizeof(TF), 0);
         }
     }
     __threadfence_block();
 
     // ---- backward: d_k = d'_k - Gamma_k d_{k+1};  U_k += d_k
+    TF pb[NB];
+    double ub[NB];
 #pragma unroll
     for (int j = 0; j < NV; ++j) d[j] = TF(0);
-    TF pc, pn;
-    double uc, un;
-    auto load_b = [&](int w, TF (&e)[NV], TF &pp, double &uu) {
-        const int k = w * SPW + st_of_lane;
-        const bool ok = lane_ok && k < n;
+    // Window indices below 0 give out-of-range offsets: loads read 0, stores
+    // are dropped (same straight-line structure as the forward pass).
 #pragma unroll
-        for (int j = 0; j < NV; ++j) e[j] = lane_ok ? GB[((size_t)w * NV + j) * LW + lane] : TF(0);
-        pp = ok ? S.dUp[(size_t)NV * (m0 + k) + row] : TF(0);
-        uu = ok ? S.U[(size_t)NV * (m0 + k) + row] : 0.0;
-    };
-    load_b(nw - 1, ec, pc, uc);
-    for (int w = nw - 1; w >= 0; --w) {
-        if (w > 0) load_b(w - 1, en, pn, un);
-        TF out = TF(0);
+    for (int p = 0; p < PF; ++p) {
+        const int w = nw - 1 - p;
 #pragma unroll
-        for (int st = SPW - 1; st >= 0; --st) {
-            TF x = TF(0);
+        for (int j = 0; j < NV; ++j) eb[p][j] = tbs::bload<TF>(rGB, foff(w, j), 0);
+        pb[p] = tbs::bload<TF>(rd, vidx(w) * sizeof(TF), 0);
+        ub[p] = tbs::bload<double>(rU, vidx(w) * 8u, 0);
+    }
+    for (int w0 = 0; w0 < nw; w0 += G) {
 #pragma unroll
-            for (int j = 0; j < NV; ++j) x += ec[j] * d[j];
-            const TF v = pc - x;
+        for (int bb = 0; bb < G; ++bb) {
+            const int w = nw - 1 - (w0 + bb);
+            const int b = bb % NB;
+            const int bn = (b + PF) % NB;
+            const int wn = w - PF;
 #pragma unroll
-            for (int j = 0; j < NV; ++j) d[j] = rdlane(v, st * NV + j);
-            out = (st_of_lane == st) ? v : out;
-        }
-        const int k = w * SPW + st_of_lane;
-        if (lane_ok && k < n) S.U[(size_t)NV * (m0 + k) + row] = uc + (double)out;
-        if (w > 0) {
+            for (int j = 0; j < NV; ++j) eb[bn][j] = tbs::bload<TF>(rGB, foff(wn, j), 0);
+            pb[bn] = tbs::bload<TF>(rd, vidx(wn) * sizeof(TF), 0);
+            ub[bn] = tbs::bload<double>(rU, vidx(wn) * 8u, 0);
+            TF out = TF(0);
 #pragma unroll
-            for (int j = 0; j < NV; ++j) ec[j] = en[j];
-            pc = pn;
-            uc = un;
+            for (int st = SPW - 1; st >= 0; --st) {
+                TF x0 = TF(0), x1 = TF(0), x2 = TF(0);
+#pragma unroll
+                for (int j = 0; j < NV; j += 3) {
+                    x0 += eb[b][j] * d[j];
+                    if (j + 1 < NV) x1 += eb[b][j + 1] * d[j + 1];
+                    if (j + 2 < NV) x2 += eb[b][j + 2] * d[j + 2];
+                }
+                const TF v = pb[b] - ((x0 + x1) + x2);
+#pragma unroll
+                for (int j = 0; j < NV; ++j) d[j] = rdlane(v, st * NV + j);
+                out = (st_of_lane == st) ? v : out;
+            }
+            tbs::bstore<double>(rU, ub[b] + (double)out, vidx(w) * 8u, 0);
         }
     }
 }
 
 // ------------------------------------------------ norm / convergence --
-// One block per system: fixed-order sum of the group partials.
+// One block per system: fixed-order sum of the group partials (thread t
+// adds partials t, t + 1024, ... in order, then a fixed LDS tree).
+constexpr int kFinalThreads = 1024;
 template <typename TF>
-__global__ __launch_bounds__(256) void ffinal_kernel(const SysDev<TF> *__restrict__ sys, int nsys,
-                                                     int max_sweeps, double rtol, int force) {
-    __shared__ double s[256];
+__global__ __launch_bounds__(kFinalThreads) void ffinal_kernel(const SysDev<TF> *__restrict__ sys, int nsys,
+                                                               int max_sweeps, double rtol, int force) {
+    __shared__ double s[kFinalThreads];
     const int sidx = blockIdx.x;
     if (sidx >= nsys) return;
     const SysDev<TF> &S = sys[sidx];
     if (!force && !S.state[1]) return;
+    const int ng = S.ngroups;
     double acc = 0.0;
-    for (int i = threadIdx.x; i < S.ngroups; i += 256) acc += S.gpart[i];
+    int i = threadIdx.x;
+    for (; i + 3 * kFinalThreads < ng; i += 4 * kFinalThreads) {
+        const double a0 = S.gpart[i], a1 = S.gpart[i + kFinalThreads];
+        const double a2 = S.gpart[i + 2 * kFinalThreads], a3 = S.gpart[i + 3 * kFinalThreads];
+        acc += a0;
+        acc += a1;
+        acc += a2;
+        acc += a3;
+    }
+    for (; i < ng; i += kFinalThreads) acc += S.gpart[i];
     s[threadIdx.x] = acc;
     __syncthreads();
-    for (int w = 128; w >= 1; w >>= 1) {
+    for (int w = kFinalThreads / 2; w >= 1; w >>= 1) {
         if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
         __syncthreads();
     }
@@ -278,22 +403,31 @@ __global__ void reset_state_kernel(const SysDev<TF> *__restrict__ sys, int nsys)
 }
 
 // ------------------------------------------------------------ packing --
-// Residual blocks into SoA-64: src code per (internal element, slot):
-//   x = array (0 none, 1 jacD, 2 jacO1, 3 jacO2), y = block index.
+// Residual blocks into the SoA-64 layout (tbs::Soa64<BS, double>): src code
+// per (internal element, slot): x = array (0 none, 1 jacD, 2 jacO1, 3 jacO2),
+// y = block index.  The destination is enumerated in order (coalesced writes).
 template <int NV>
 __global__ void pack_blocks_kernel(long total, int S, const int2 *__restrict__ src,
                                    const double *__restrict__ D, const double *__restrict__ O1,
                                    const double *__restrict__ O2, double *__restrict__ blk) {
     constexpr int BS = NV * NV;
+    using L = tbs::Soa64<BS, double>;
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
          t += (long)gridDim.x * blockDim.x) {
-        // t enumerates the destination in order -> coalesced writes
-        const int lane = (int)(t % kGroup);
-        const long r1 = t / kGroup;
-        const int q = (int)(r1 % BS);
-        const long r2 = r1 / BS;
-        const int slot = (int)(r2 % S);
-        const long g = r2 / S;
+        const int idx = (int)(t % L::ELEMS);
+        const long gs = t / L::ELEMS;
+        const int slot = (int)(gs % S);
+        const long g = gs / S;
+        int q, lane;
+        if (idx < L::NC * L::VW * 64) {
+            const int c = idx / (64 * L::VW), rem = idx % (64 * L::VW);
+            lane = rem / L::VW;
+            q = c * L::VW + rem % L::VW;
+        } else {
+            const int r = idx - L::NC * L::VW * 64;
+            q = L::NC * L::VW + r / 64;
+            lane = r % 64;
+        }
         const long m = g * kGroup + lane;
         const int2 c = src[m * S + slot];
         double v = 0.0;
@@ -352,7 +486,7 @@ __global__ void invform_kernel(int nle, const int *__restrict__ lines,
     solve(b, x);
     const int g = m / kGroup, lane = m % kGroup;
 #pragma unroll
-    for (int i = 0; i < NV; ++i) W[((size_t)g * BS + i + NV * c) * kGroup + lane] = (TF)x[i];
+    for (int i = 0; i < NV; ++i) W[(size_t)g * BS * kGroup + tbs::Soa64<BS, TF>::index(i + NV * c, lane)] = (TF)x[i];
     const int w = k / SPW, st = k % SPW;
     const size_t wbase = ((size_t)(woff[l] + w) * NV + c) * LW + (size_t)st * NV;
     if (k >= 1) {
@@ -417,8 +551,10 @@ struct tbs_plan {
     // device: data
     double *d_blk = nullptr;
     int *d_nbr = nullptr;
+    int *d_slotc = nullptr;
     void *d_W = nullptr, *d_EF = nullptr, *d_GB = nullptr;
     double *d_rhs = nullptr, *d_U = nullptr, *d_R = nullptr, *d_gpart = nullptr, *d_hist = nullptr;
+    double *a_rhs = nullptr, *a_U = nullptr, *a_R = nullptr;   // allocations (kPad before element 0)
     void *d_g = nullptr, *d_dUp = nullptr;
     int *d_state = nullptr;
     // the plan as a one-system batch
@@ -428,6 +564,7 @@ struct tbs_plan {
     std::map<int, hipGraphExec_t> graphs;
     int max_sweeps = kHistCap - 1;
     double rtol = 0.0;
+    int variant = 0;          // Thomas prefetch variant (autotuned)
     size_t bytes = 0;
     int *h_state = nullptr;  // pinned
 };
@@ -442,6 +579,7 @@ struct tbs_batch {
     int nl = 0, ng = 0;
     std::map<std::pair<int, std::pair<int, double>>, hipGraphExec_t> graphs;
     int *h_state = nullptr;
+    int variant = 0;
 };
 
 namespace {
@@ -451,6 +589,7 @@ SysDev<TF> make_sysdev(const tbs_plan *p) {
     SysDev<TF> s{};
     s.blk = p->d_blk;
     s.nbr = p->d_nbr;
+    s.slotc = p->d_slotc;
     s.W = (const TF *)p->d_W;
     s.EF = (const TF *)p->d_EF;
     s.GB = (const TF *)p->d_GB;
@@ -596,8 +735,8 @@ void free_plan(tbs_plan *p) {
     hipSetDevice(p->ctx->device);
     for (auto &kv : p->graphs) hipGraphExecDestroy(kv.second);
     void *ptrs[] = {p->d_lines, p->d_mline, p->d_linepoint, p->d_linesize, p->d_woff, p->d_fc,
-                    p->d_lineface, p->d_src, p->d_blk, p->d_nbr, p->d_W, p->d_EF, p->d_GB,
-                    p->d_rhs, p->d_U, p->d_R, p->d_gpart, p->d_hist, p->d_g, p->d_dUp,
+                    p->d_lineface, p->d_src, p->d_blk, p->d_nbr, p->d_slotc, p->d_W, p->d_EF, p->d_GB,
+                    p->a_rhs, p->a_U, p->a_R, p->d_gpart, p->d_hist, p->d_g, p->d_dUp,
                     p->d_state, p->d_sys, p->d_ltab, p->d_gtab};
     for (void *q : ptrs)
         if (q) hipFree(q);
@@ -608,24 +747,67 @@ void free_plan(tbs_plan *p) {
 template <int NV, typename TF>
 int launch_residual(hipStream_t st, const void *sys, const int2 *gtab, int ng, int want_R, int force) {
     if (ng == 0) return TBS_OK;
-    hipLaunchKernelGGL((fres_kernel<NV, TF>), dim3((ng + 3) / 4), dim3(256), 0, st,
+    const int nblk = (ng + 3) / 4;
+    hipLaunchKernelGGL((fres_kernel<NV, TF>), dim3(8 * ((nblk + 7) / 8)), dim3(256), 0, st,
                        (const SysDev<TF> *)sys, gtab, ng, want_R, force);
     TBS_LAUNCHED("fres_kernel");
     return TBS_OK;
 }
 
-template <int NV, typename TF>
-int launch_thomas(hipStream_t st, const void *sys, const int4 *ltab, int nl) {
-    if (nl == 0) return TBS_OK;
-    hipLaunchKernelGGL((fthomas_kernel<NV, TF>), dim3(nl), dim3(64), 0, st,
+// Thomas variants the autotuner picks from: {prefetch windows PF, ring
+// cycles K per loop iteration}; index 0 is the default.
+struct TV { int pf, k; };
+template <int NV, typename TF> struct ThomasV { static constexpr int n = 1; static constexpr TV v[1] = {{3, 2}}; };
+template <> struct ThomasV<9, double> {
+    static constexpr int n = 4;
+    static constexpr TV v[4] = {{3, 2}, {2, 3}, {3, 4}, {5, 2}};
+};
+template <> struct ThomasV<9, float> {
+    static constexpr int n = 4;
+    static constexpr TV v[4] = {{5, 2}, {3, 3}, {7, 2}, {3, 5}};
+};
+
+template <int NV, typename TF, int PF, int K>
+int launch_thomas_pf(hipStream_t st, const void *sys, const int4 *ltab, int nl) {
+    hipLaunchKernelGGL((fthomas_kernel<NV, TF, PF, K>), dim3(nl), dim3(64), 0, st,
                        (const SysDev<TF> *)sys, ltab, nl);
     TBS_LAUNCHED("fthomas_kernel");
     return TBS_OK;
 }
 
+template <int NV, typename TF>
+int launch_thomas(hipStream_t st, const void *sys, const int4 *ltab, int nl, int variant) {
+    if (nl == 0) return TBS_OK;
+    using P = ThomasV<NV, TF>;
+    if (variant < 0 || variant >= P::n) variant = 0;
+    constexpr TV a = P::v[0];
+    constexpr TV b = P::v[P::n > 1 ? 1 : 0];
+    constexpr TV c = P::v[P::n > 2 ? 2 : 0];
+    constexpr TV d = P::v[P::n > 3 ? 3 : 0];
+    switch (variant) {
+        case 1: return launch_thomas_pf<NV, TF, b.pf, b.k>(st, sys, ltab, nl);
+        case 2: return launch_thomas_pf<NV, TF, c.pf, c.k>(st, sys, ltab, nl);
+        case 3: return launch_thomas_pf<NV, TF, d.pf, d.k>(st, sys, ltab, nl);
+        default: return launch_thomas_pf<NV, TF, a.pf, a.k>(st, sys, ltab, nl);
+    }
+}
+
+int thomas_variants(int nv, int prec) {
+    if (nv == 9) return prec == TBS_FP32 ? ThomasV<9, float>::n : ThomasV<9, double>::n;
+    return 1;
+}
+
+// encoded as 100 * PF + K
+int thomas_pf_of(int nv, int prec, int variant) {
+    TV t = {3, 2};
+    if (nv == 9) t = prec == TBS_FP32 ? ThomasV<9, float>::v[variant % ThomasV<9, float>::n]
+                                      : ThomasV<9, double>::v[variant % ThomasV<9, double>::n];
+    return 100 * t.pf + t.k;
+}
+
 template <typename TF>
 int launch_final(hipStream_t st, const void *sys, int nsys, int max_sweeps, double rtol, int force) {
-    hipLaunchKernelGGL((ffinal_kernel<TF>), dim3(nsys), dim3(256), 0, st,
+    hipLaunchKernelGGL((ffinal_kernel<TF>), dim3(nsys), dim3(kFinalThreads), 0, st,
                        (const SysDev<TF> *)sys, nsys, max_sweeps, rtol, force);
     TBS_LAUNCHED("ffinal_kernel");
     return TBS_OK;
@@ -645,10 +827,10 @@ int k_residual(int nv, int prec, hipStream_t st, const void *sys, const int2 *gt
     tbs::set_error("nvar %d not built for the fused path", nv);
     return TBS_ERR_UNSUPPORTED;
 }
-int k_thomas(int nv, int prec, hipStream_t st, const void *sys, const int4 *ltab, int nl) {
+int k_thomas(int nv, int prec, hipStream_t st, const void *sys, const int4 *ltab, int nl, int variant) {
     switch (nv) {
-#define X(v) case v: return prec == TBS_FP32 ? launch_thomas<v, float>(st, sys, ltab, nl) \
-                                         : launch_thomas<v, double>(st, sys, ltab, nl);
+#define X(v) case v: return prec == TBS_FP32 ? launch_thomas<v, float>(st, sys, ltab, nl, variant) \
+                                         : launch_thomas<v, double>(st, sys, ltab, nl, variant);
         TBS_PLAN_NVARS(X)
 #undef X
     }
@@ -681,8 +863,8 @@ bool plan_nvar_ok(int nv) {
 
 // One sweep: Thomas (+U update), residual, norm/convergence.
 int enqueue_sweep(int nv, int prec, hipStream_t st, const void *sys, int nsys, const int4 *ltab,
-                  int nl, const int2 *gtab, int ng, int max_sweeps, double rtol) {
-    TBS_TRY(k_thomas(nv, prec, st, sys, ltab, nl));
+                  int nl, const int2 *gtab, int ng, int max_sweeps, double rtol, int variant) {
+    TBS_TRY(k_thomas(nv, prec, st, sys, ltab, nl, variant));
     TBS_TRY(k_residual(nv, prec, st, sys, gtab, ng, 0, 0));
     TBS_TRY(k_final(prec, st, sys, nsys, max_sweeps, rtol, 0));
     return TBS_OK;
@@ -773,6 +955,20 @@ int tbs_plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pla
             nbr[((size_t)g * L.F + s) * kGroup + lane] = inv[e == e1 ? e2 : e1];
         }
     }
+    // slot maps: affine shift when every lane with a face has nbr = m + c
+    std::vector<int> slotc((size_t)L.ngroups * std::max(L.F, 1), kSlotNone);
+    for (int g = 0; g < L.ngroups; ++g)
+        for (int f = 0; f < L.F; ++f) {
+            int c = kSlotNone;
+            for (int lane = 0; lane < kGroup; ++lane) {
+                const int mm = g * kGroup + lane;
+                const int nb = nbr[((size_t)g * L.F + f) * kGroup + lane];
+                if (mm >= p->nle || nb < 0) continue;
+                if (c == kSlotNone) c = nb - mm;
+                else if (c != nb - mm) { c = kSlotGather; break; }
+            }
+            slotc[(size_t)g * L.F + f] = c;
+        }
     std::vector<int4> ltab(p->nline);
     for (int l = 0; l < p->nline; ++l) ltab[l] = make_int4(0, m->linepoint[l], m->linesize[l], L.woff[l]);
     std::vector<int2> gtab(L.ngroups);
@@ -789,12 +985,18 @@ int tbs_plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pla
     TBS_TRY(dalloc(P, &P->d_src, src.size()));
     TBS_TRY(dalloc(P, &P->d_blk, G * L.S * BS));
     TBS_TRY(dalloc(P, &P->d_nbr, nbr.size()));
+    TBS_TRY(dalloc(P, &P->d_slotc, slotc.size()));
     TBS_TRY(dalloc(P, (char **)&P->d_W, G * BS * tf));
     TBS_TRY(dalloc(P, (char **)&P->d_EF, (size_t)L.nwin * nv * lw * tf));
     TBS_TRY(dalloc(P, (char **)&P->d_GB, (size_t)L.nwin * nv * lw * tf));
-    TBS_TRY(dalloc(P, &P->d_rhs, (size_t)P->nle * nv));
-    TBS_TRY(dalloc(P, &P->d_U, (size_t)P->nle * nv));
-    TBS_TRY(dalloc(P, &P->d_R, (size_t)P->nle * nv));
+    const size_t padded = (size_t)(P->nle + 2 * kPad) * nv;
+    TBS_TRY(dalloc(P, &P->a_rhs, padded));
+    TBS_TRY(dalloc(P, &P->a_U, padded));
+    TBS_TRY(dalloc(P, &P->a_R, padded));
+    P->d_rhs = P->a_rhs + (size_t)kPad * nv;
+    P->d_U = P->a_U + (size_t)kPad * nv;
+    P->d_R = P->a_R + (size_t)kPad * nv;
+    for (double *a : {P->a_rhs, P->a_U, P->a_R}) TBS_HIP(hipMemsetAsync(a, 0, padded * sizeof(double), ctx->stream));
     TBS_TRY(dalloc(P, (char **)&P->d_g, (size_t)P->nle * nv * tf));
     TBS_TRY(dalloc(P, (char **)&P->d_dUp, (size_t)P->nle * nv * tf));
     TBS_TRY(dalloc(P, &P->d_gpart, L.ngroups));
@@ -819,6 +1021,7 @@ int tbs_plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pla
     TBS_TRY(up(P->d_lineface, m->lineface, P->nle * 4));
     TBS_TRY(up(P->d_src, src.data(), src.size() * sizeof(int2)));
     TBS_TRY(up(P->d_nbr, nbr.data(), nbr.size() * 4));
+    TBS_TRY(up(P->d_slotc, slotc.data(), slotc.size() * 4));
     TBS_TRY(up(P->d_ltab, ltab.data(), ltab.size() * sizeof(int4)));
     TBS_TRY(up(P->d_gtab, gtab.data(), gtab.size() * sizeof(int2)));
     if (precision == TBS_FP32) {
@@ -964,7 +1167,7 @@ int tbs_plan_begin(tbs_plan *p, const double *rhs, const double *U0) {
 int tbs_plan_launch_thomas(tbs_plan *p) {
     TBS_REQUIRE(p, TBS_ERR_ARG, "null plan");
     TBS_TRY(tbs::use_ctx(p->ctx));
-    return k_thomas(p->nv, p->prec, p->ctx->stream, p->d_sys, p->d_ltab, p->nline);
+    return k_thomas(p->nv, p->prec, p->ctx->stream, p->d_sys, p->d_ltab, p->nline, p->variant);
 }
 
 int tbs_plan_launch_residual(tbs_plan *p) {
@@ -980,7 +1183,7 @@ int tbs_plan_launch_sweeps(tbs_plan *p, int nsweeps, int use_graph) {
     hipStream_t st = p->ctx->stream;
     auto one = [p, st]() {
         return enqueue_sweep(p->nv, p->prec, st, p->d_sys, 1, p->d_ltab, p->nline, p->d_gtab,
-                             p->ngroups, p->max_sweeps, p->rtol);
+                             p->ngroups, p->max_sweeps, p->rtol, p->variant);
     };
     if (!use_graph) {
         for (int i = 0; i < nsweeps; ++i) TBS_TRY(one());
@@ -1029,6 +1232,113 @@ int tbs_plan_history(tbs_plan *p, double *hist, int cap, int *count) {
     }
     return TBS_OK;
 }
+
+int tbs_plan_internal(tbs_plan *p, int which, void **dptr, size_t *bytes) {
+    TBS_REQUIRE(p && dptr && bytes, TBS_ERR_ARG, "null argument");
+    const size_t tf = tf_size(p->prec), nv = p->nv, n = p->nle, G = (size_t)p->ngroups * kGroup;
+    const size_t lw = (64 / nv) * nv;
+    switch (which) {
+        case 0: *dptr = p->d_g; *bytes = n * nv * tf; break;
+        case 1: *dptr = p->d_dUp; *bytes = n * nv * tf; break;
+        case 2: *dptr = p->d_U; *bytes = n * nv * 8; break;
+        case 3: *dptr = p->d_rhs; *bytes = n * nv * 8; break;
+        case 4: *dptr = p->d_R; *bytes = n * nv * 8; break;
+        case 5: *dptr = p->d_W; *bytes = G * nv * nv * tf; break;
+        case 6: *dptr = p->d_EF; *bytes = (size_t)p->nwin * nv * lw * tf; break;
+        case 7: *dptr = p->d_GB; *bytes = (size_t)p->nwin * nv * lw * tf; break;
+        case 8: *dptr = p->d_blk; *bytes = G * p->S * nv * nv * 8; break;
+        default: TBS_REQUIRE(false, TBS_ERR_ARG, "unknown internal array %d", which);
+    }
+    return TBS_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// Time each Thomas variant on zeroed vectors (the kernel's cost does not
+// depend on the values) and return the fastest.  Leaves the systems' state
+// reset; callers start a solve with begin() afterwards.
+int autotune_thomas(tbs_ctx *ctx, int nv, int prec, const void *sys, int nsys, const int4 *ltab,
+                    int nl, const std::vector<std::pair<double *, long>> &zero_vecs, int reps,
+                    int *chosen, double *us, int cap) {
+    hipStream_t st = ctx->stream;
+    for (auto &zv : zero_vecs) {
+        hipLaunchKernelGGL(zero_kernel, dim3(gs_grid(zv.second)), dim3(256), 0, st, zv.second, zv.first);
+        TBS_LAUNCHED("zero_kernel");
+    }
+    TBS_TRY(k_reset(prec, st, sys, nsys));
+    const int nvar = thomas_variants(nv, prec);
+    hipEvent_t e0, e1;
+    TBS_HIP(hipEventCreate(&e0));
+    TBS_HIP(hipEventCreate(&e1));
+    int best = 0;
+    double best_t = 1e300;
+    int rc = TBS_OK;
+    for (int v = 0; v < nvar && rc == TBS_OK; ++v) {
+        rc = k_thomas(nv, prec, st, sys, ltab, nl, v);     // warm-up
+        if (rc) break;
+        if (hipEventRecord(e0, st)) { rc = TBS_ERR_STATE; break; }
+        for (int r = 0; r < reps && rc == TBS_OK; ++r) rc = k_thomas(nv, prec, st, sys, ltab, nl, v);
+        if (rc) break;
+        if (hipEventRecord(e1, st) || hipEventSynchronize(e1)) { rc = TBS_ERR_STATE; break; }
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        const double t = 1e3 * ms / reps;
+        if (us && v < cap) us[v] = t;
+        if (t < best_t) { best_t = t; best = v; }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    TBS_TRY(k_reset(prec, st, sys, nsys));
+    TBS_HIP(hipStreamSynchronize(st));
+    if (chosen) *chosen = best;
+    return TBS_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int tbs_plan_autotune(tbs_plan *p, int reps, int *chosen, double *us, int cap) {
+    TBS_REQUIRE(p && reps >= 1, TBS_ERR_ARG, "bad arguments");
+    TBS_TRY(tbs::use_ctx(p->ctx));
+    TBS_REQUIRE(p->factored, TBS_ERR_STATE, "autotune before factor");
+    const long nvec = (long)p->nle * p->nv;
+    const size_t tf = tf_size(p->prec);
+    std::vector<std::pair<double *, long>> zv = {{p->d_U, nvec}, {p->d_rhs, nvec},
+                                                 {(double *)p->d_g, (long)(nvec * tf / 8)}};
+    int best = 0;
+    TBS_TRY(autotune_thomas(p->ctx, p->nv, p->prec, p->d_sys, 1, p->d_ltab, p->nline, zv, reps, &best, us, cap));
+    if (best != p->variant) {
+        for (auto &kv : p->graphs) hipGraphExecDestroy(kv.second);
+        p->graphs.clear();
+        p->variant = best;
+    }
+    if (chosen) *chosen = best;
+    return TBS_OK;
+}
+
+int tbs_plan_set_variant(tbs_plan *p, int variant) {
+    TBS_REQUIRE(p && variant >= 0 && variant < thomas_variants(p->nv, p->prec), TBS_ERR_ARG,
+                "variant %d not built", variant);
+    if (variant != p->variant) {
+        for (auto &kv : p->graphs) hipGraphExecDestroy(kv.second);
+        p->graphs.clear();
+        p->variant = variant;
+    }
+    return TBS_OK;
+}
+
+int tbs_plan_variants(tbs_plan *p, int *pf, int cap) {
+    TBS_REQUIRE(p, TBS_ERR_ARG, "null plan");
+    const int n = thomas_variants(p->nv, p->prec);
+    for (int v = 0; v < n && v < cap; ++v) if (pf) pf[v] = thomas_pf_of(p->nv, p->prec, v);
+    return n;
+}
+
+}  // extern "C"
+
+extern "C" {
 
 int tbs_plan_solve(tbs_plan *p, const double *rhs, double *U, int max_sweeps, double rtol,
                    double *hist, int *nsweeps, double *R_final) {
@@ -1117,6 +1427,29 @@ int tbs_batch_create(tbs_ctx *ctx, tbs_plan *const *plans, int nplans, tbs_batch
     return TBS_OK;
 }
 
+int tbs_batch_autotune(tbs_batch *b, int reps, int *chosen, double *us, int cap) {
+    TBS_REQUIRE(b && reps >= 1, TBS_ERR_ARG, "bad arguments");
+    TBS_TRY(tbs::use_ctx(b->ctx));
+    std::vector<std::pair<double *, long>> zv;
+    const size_t tf = tf_size(b->prec);
+    for (tbs_plan *p : b->plans) {
+        const long nvec = (long)p->nle * p->nv;
+        zv.push_back({p->d_U, nvec});
+        zv.push_back({p->d_rhs, nvec});
+        zv.push_back({(double *)p->d_g, (long)(nvec * tf / 8)});
+    }
+    int best = 0;
+    TBS_TRY(autotune_thomas(b->ctx, b->nv, b->prec, b->d_sys, (int)b->plans.size(), b->d_ltab, b->nl, zv,
+                            reps, &best, us, cap));
+    if (best != b->variant) {
+        for (auto &kv : b->graphs) hipGraphExecDestroy(kv.second);
+        b->graphs.clear();
+        b->variant = best;
+    }
+    if (chosen) *chosen = best;
+    return TBS_OK;
+}
+
 int tbs_batch_destroy(tbs_batch *b) {
     if (!b) return TBS_OK;
     hipSetDevice(b->ctx->device);
@@ -1157,7 +1490,7 @@ int tbs_batch_solve(tbs_batch *b, const double *const *rhs, double *const *U, in
         hipGraphExec_t ge = nullptr;
         auto one = [b, st, ns, max_sweeps, rtol]() {
             return enqueue_sweep(b->nv, b->prec, st, b->d_sys, ns, b->d_ltab, b->nl, b->d_gtab, b->ng,
-                                 max_sweeps, rtol);
+                                 max_sweeps, rtol, b->variant);
         };
         TBS_TRY(capture_sweeps(st, sweeps_per_check, &ge, one));
         it = b->graphs.emplace(key, ge).first;

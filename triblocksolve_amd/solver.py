@@ -7,6 +7,7 @@ Device arrays are DeviceBuffer objects (HBM allocated through the library's
 own C ABI); all arithmetic happens in libtriblock_hip.so.
 """
 import ctypes as C
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -46,6 +47,7 @@ class DeviceBuffer:
         p = C.c_void_p()
         A.check(ctx._lib.tbs_malloc(ctx.handle, self.nbytes, C.byref(p)))
         self.ptr = p.value or 0
+        ctx._children.add(self)
         if data is not None:
             self.upload(data)
         elif zero:
@@ -71,9 +73,9 @@ class DeviceBuffer:
         A.check(self.ctx._lib.tbs_memset(self.ctx.handle, C.c_void_p(self.ptr), 0, self.nbytes))
 
     def close(self):
-        if self.ptr:
+        if self.ptr and self.ctx.handle:
             self.ctx._lib.tbs_free(self.ctx.handle, C.c_void_p(self.ptr))
-            self.ptr = 0
+        self.ptr = 0
 
     def __del__(self):
         try:
@@ -232,9 +234,13 @@ class Context:
         self.handle = h
         self.nvar = nvar
         self.device = device
+        self._children = weakref.WeakSet()   # plans, batches, buffers: closed before the context
 
     def close(self):
         if self.handle:
+            for ch in sorted(list(self._children), key=lambda o: 0 if isinstance(o, Batch) else
+                             (1 if isinstance(o, Plan) else 2)):
+                ch.close()
             self._lib.tbs_ctx_destroy(self.handle)
             self.handle = None
 
@@ -325,6 +331,7 @@ class Plan:
         A.check(ctx._lib.tbs_plan_create(ctx.handle, C.byref(self._ms), precision, C.byref(h)))
         self.handle = h
         self.precision = precision
+        ctx._children.add(self)
 
     @staticmethod
     def footprint(mesh, precision=A.TBS_FP64):
@@ -334,9 +341,16 @@ class Plan:
         return out.value
 
     def close(self):
-        if self.handle:
+        if self.handle and self.ctx.handle:
             self.ctx._lib.tbs_plan_destroy(self.handle)
-            self.handle = None
+        self.handle = None
+
+    def internal(self, which):
+        """(device address, bytes) of an internal array, for diagnostics:
+        0 g, 1 d', 2 U, 3 rhs, 4 R, 5 W, 6 E windows, 7 Gamma windows, 8 residual blocks."""
+        p, nb = C.c_void_p(), C.c_size_t()
+        A.check(self.ctx._lib.tbs_plan_internal(self.handle, which, C.byref(p), C.byref(nb)))
+        return p.value or 0, nb.value
 
     def __del__(self):
         try:
@@ -348,6 +362,19 @@ class Plan:
         inf = A.PlanInfo()
         A.check(self.ctx._lib.tbs_plan_info_get(self.handle, C.byref(inf)))
         return {k: getattr(inf, k) for k, _ in A.PlanInfo._fields_}
+
+    def autotune(self, reps=3):
+        """Pick the fastest Thomas variant; returns (chosen index, {"pf<P>_k<K>": us})."""
+        n = self.ctx._lib.tbs_plan_variants(self.handle, None, 0)
+        pf = np.zeros(n, np.int32)
+        self.ctx._lib.tbs_plan_variants(self.handle, _ptr(pf), n)
+        us = np.zeros(n)
+        ch = C.c_int()
+        A.check(self.ctx._lib.tbs_plan_autotune(self.handle, reps, C.byref(ch), _ptr(us), n))
+        return ch.value, {f"pf{int(p) // 100}_k{int(p) % 100}": float(t) for p, t in zip(pf, us)}
+
+    def set_variant(self, v):
+        A.check(self.ctx._lib.tbs_plan_set_variant(self.handle, v))
 
     def factor(self, jacD, jacO1, jacO2):
         A.check(self.ctx._lib.tbs_plan_factor(self.handle, _ptr(jacD), _ptr(jacO1), _ptr(jacO2)))
@@ -391,17 +418,24 @@ class Batch:
         h = C.c_void_p()
         A.check(ctx._lib.tbs_batch_create(ctx.handle, arr, len(self.plans), C.byref(h)))
         self.handle = h
+        ctx._children.add(self)
 
     def close(self):
-        if self.handle:
+        if self.handle and self.ctx.handle:
             self.ctx._lib.tbs_batch_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:
             self.close()
         except Exception:
             pass
+
+    def autotune(self, reps=3, nvariants=8):
+        us = np.zeros(nvariants)
+        ch = C.c_int()
+        A.check(self.ctx._lib.tbs_batch_autotune(self.handle, reps, C.byref(ch), _ptr(us), nvariants))
+        return ch.value, us
 
     def solve(self, rhs_list, U_list, max_sweeps, rtol=0.0, sweeps_per_check=4):
         n = len(self.plans)
