@@ -109,4 +109,23 @@ struct Soa64 {
     }
 };
 
+// Factor-window layout of the Thomas kernel: a window holds, for each of LW
+// lanes, one row of N = NV entries; entries are stored in 16-byte chunks per
+// lane ([chunk][lane][VW]) followed by the remaining entries ([entry][lane]),
+// so one wave load moves LW*16 bytes.
+template <int N, int LW, typename T>
+struct WinLayout {
+    static constexpr int VW = 16 / (int)sizeof(T);
+    static constexpr int NC = N / VW;
+    static constexpr int REM = N - NC * VW;
+    static constexpr int ELEMS = N * LW;              // T values of data per window
+    // window stride in T values, padded to a multiple of 16 bytes so every
+    // 16-byte chunk load is aligned
+    static constexpr int STRIDE = (int)(((long)ELEMS * (long)sizeof(T) + 15) / 16 * 16 / (long)sizeof(T));
+    __host__ __device__ static constexpr long index(int j, int lane) {
+        return j < NC * VW ? (long)(j / VW) * LW * VW + (long)lane * VW + j % VW
+                           : (long)NC * VW * LW + (long)(j - NC * VW) * LW + lane;
+    }
+};
+
 }  // namespace tbs

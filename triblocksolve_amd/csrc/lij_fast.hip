@@ -219,17 +219,84 @@ __global__ __launch_bounds__(256) void fres_kernel(const SysDev<TF> *__restrict_
 }
 
 // -------------------------------------------------------------- Thomas --
-// One wave per line; windows of SPW steps, PF windows prefetched ahead.
-// The window loop is unrolled K ring cycles (K*NB windows) per iteration:
-// the compiler drains all in-flight loads at a loop header, so a longer body
-// means fewer drains per line.
+// Vector I/O of a group of windows: V values of T, contiguous in memory,
+// moved with 16-byte wave loads/stores and staged in LDS ([flat] layout equal
+// to the memory layout).  NCH 1 KiB chunks cover V values (+1 spare value so
+// the idle lane of a window may read one past the end).
+template <int V, typename T>
+struct GroupIO {
+    static constexpr int VW = 16 / (int)sizeof(T);
+    static constexpr int NCH = (V + 1 + 64 * VW - 1) / (64 * VW);
+    static constexpr int LDS = NCH * 64 * VW;            // padded LDS length
+};
+
+template <int V, typename T>
+__device__ __forceinline__ void bulk_load(tbs::rsrc_t r, uint32_t base, int lane,
+                                          tbs::u32x4 (&reg)[GroupIO<V, T>::NCH]) {
+#pragma unroll
+    for (int c = 0; c < GroupIO<V, T>::NCH; ++c) reg[c] = tbs::bload16(r, base + (uint32_t)(c * 1024 + lane * 16), 0);
+}
+
+template <int V, typename T>
+__device__ __forceinline__ void bulk_commit(const tbs::u32x4 (&reg)[GroupIO<V, T>::NCH], int lane, T *lds) {
+    using IO = GroupIO<V, T>;
+#pragma unroll
+    for (int c = 0; c < IO::NCH; ++c) {
+#pragma unroll
+        for (int k = 0; k < IO::VW; ++k) lds[c * 64 * IO::VW + lane * IO::VW + k] = tbs::lane_elem<T>(reg[c], k);
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Store exactly V values from LDS (16-byte stores for whole chunks, per-value
+// stores for the tail so nothing past the group is overwritten).
+template <int V, typename T>
+__device__ __forceinline__ void bulk_store(tbs::rsrc_t r, uint32_t base, int lane, const T *lds) {
+    using IO = GroupIO<V, T>;
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int c = 0; c < IO::NCH; ++c) {
+        const int first = c * 64 * IO::VW + lane * IO::VW;
+        if ((c + 1) * 64 * IO::VW <= V) {                      // whole chunk inside the group
+            tbs::u32x4 v;
+            if constexpr (sizeof(T) == 8) {
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const unsigned long long bits = __builtin_bit_cast(unsigned long long, (double)lds[first + k]);
+                    v[2 * k] = (unsigned)bits;
+                    v[2 * k + 1] = (unsigned)(bits >> 32);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) v[k] = __builtin_bit_cast(unsigned, (float)lds[first + k]);
+            }
+            __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)(base + (uint32_t)(c * 1024 + lane * 16)), 0, 0);
+        } else {
+#pragma unroll
+            for (int k = 0; k < IO::VW; ++k) {
+                const int q = first + k;
+                const uint32_t off = q < V ? base + (uint32_t)q * sizeof(T) : tbs::kOOB;
+                tbs::bstore<T>(r, lds[q], off, 0);
+            }
+        }
+    }
+}
+
+// One wave per line; windows of SPW steps; PF factor windows prefetched in a
+// register ring; vectors staged per group of G = K*(PF+1) windows in LDS.
 template <int NV, typename TF, int PF, int K>
 __global__ __launch_bounds__(64) void fthomas_kernel(const SysDev<TF> *__restrict__ sys,
                                                      const int4 *__restrict__ ltab, int nl) {
     constexpr int SPW = 64 / NV;      // steps per window
     constexpr int LW = SPW * NV;      // lanes carrying data
     constexpr int NB = PF + 1;        // register ring depth
-    constexpr int G = K * NB;         // windows per loop iteration
+    constexpr int G = K * NB;         // windows per group (loop iteration)
+    constexpr int VG = G * LW;        // vector values per group
+    using IOT = GroupIO<VG, TF>;
+    using IOD = GroupIO<VG, double>;
+    __shared__ TF s_vin[2][IOT::LDS];       // forward: g; backward: d'
+    __shared__ TF s_out[IOT::LDS];          // forward: d' staging
+    __shared__ double s_u[2][IOD::LDS];     // backward: U (updated in place)
     const int l = blockIdx.x;
     if (l >= nl) return;
     const int4 lt = ltab[l];          // {system, m0, n, window offset}
@@ -237,50 +304,58 @@ __global__ __launch_bounds__(64) void fthomas_kernel(const SysDev<TF> *__restric
     if (!S.state[1]) return;
     const int lane = threadIdx.x;
     const int st_of_lane = lane / NV;
-    const bool lane_ok = lane < LW;
     const int m0 = lt.y, n = lt.z;
     const int nw = (n + SPW - 1) / SPW;
-    const uint32_t wbytes = (uint32_t)NV * LW * sizeof(TF);           // one window of factors
-    const tbs::rsrc_t rEF = tbs::make_rsrc(S.EF + (size_t)lt.w * NV * LW, (uint32_t)nw * wbytes);
-    const tbs::rsrc_t rGB = tbs::make_rsrc(S.GB + (size_t)lt.w * NV * LW, (uint32_t)nw * wbytes);
-    const uint32_t vb = (uint32_t)n * NV;
+    const int ngrp = (nw + G - 1) / G;
+    constexpr int WSTRIDE = tbs::WinLayout<NV, LW, TF>::STRIDE;
+    const uint32_t wbytes = (uint32_t)WSTRIDE * sizeof(TF);           // one window of factors
+    const tbs::rsrc_t rEF = tbs::make_rsrc(S.EF + (size_t)lt.w * WSTRIDE, (uint32_t)nw * wbytes);
+    const tbs::rsrc_t rGB = tbs::make_rsrc(S.GB + (size_t)lt.w * WSTRIDE, (uint32_t)nw * wbytes);
+    const uint32_t vb = (uint32_t)n * NV;                              // vector values of the line
     const tbs::rsrc_t rg = tbs::make_rsrc(S.g + (size_t)NV * m0, vb * sizeof(TF));
     const tbs::rsrc_t rd = tbs::make_rsrc(S.dUp + (size_t)NV * m0, vb * sizeof(TF));
     const tbs::rsrc_t rU = tbs::make_rsrc(S.U + (size_t)NV * m0, vb * 8u);
-    // element index of this lane's (step, row) inside window w: w*LW + lane
-    // Offsets for windows outside [0, nw) or lane >= LW are out of range by
-    // construction (negative windows wrap to > 2^31 bytes, far past any range).
-    auto vidx = [&](int w) -> uint32_t {
-        return (lane_ok && w >= 0) ? (uint32_t)(w * LW + lane) : (tbs::kOOB >> 3);
-    };
-    auto foff = [&](int w, int j) -> uint32_t {
-        return w >= 0 ? ((uint32_t)(w * NV + j) * LW + lane) * sizeof(TF) : tbs::kOOB;
+    // group j's vector block starts at value j*VG (windows j*G ...)
+    auto gbase = [&](int j, int bytes_per) -> uint32_t { return (uint32_t)(j * VG) * (uint32_t)bytes_per; };
+
+    using WL = tbs::WinLayout<NV, LW, TF>;
+    auto wbase = [&](int w) -> uint32_t { return w >= 0 ? (uint32_t)w * wbytes : tbs::kOOB; };
+    auto load_row = [&](tbs::rsrc_t r, int w, TF (&e)[NV]) {
+        const uint32_t b = wbase(w);
+#pragma unroll
+        for (int c = 0; c < WL::NC; ++c) {
+            const tbs::u32x4 v = tbs::bload16(r, b + (uint32_t)(c * LW * 16 + lane * 16), 0);
+#pragma unroll
+            for (int k = 0; k < WL::VW; ++k) e[c * WL::VW + k] = tbs::lane_elem<TF>(v, k);
+        }
+#pragma unroll
+        for (int k = 0; k < WL::REM; ++k)
+            e[WL::NC * WL::VW + k] = tbs::bload<TF>(r, b + (uint32_t)((WL::NC * WL::VW * LW + k * LW + lane) * sizeof(TF)), 0);
     };
 
     TF d[NV];
     TF eb[NB][NV];
-    TF gb[NB];
     // ---- forward: d_k = g_k - E_k d_{k-1}
 #pragma unroll
     for (int j = 0; j < NV; ++j) d[j] = TF(0);
 #pragma unroll
-    for (int p = 0; p < PF; ++p) {
-#pragma unroll
-        for (int j = 0; j < NV; ++j) eb[p][j] = tbs::bload<TF>(rEF, foff(p, j), 0);
-        gb[p] = tbs::bload<TF>(rg, vidx(p) * sizeof(TF), 0);
+    for (int p = 0; p < PF; ++p) load_row(rEF, p, eb[p]);
+    {
+        tbs::u32x4 vr[IOT::NCH];
+        bulk_load<VG, TF>(rg, gbase(0, sizeof(TF)), lane, vr);
+        bulk_commit<VG, TF>(vr, lane, s_vin[0]);
     }
-    // Windows past the line end need no branch: their loads read 0 and their
-    // stores fall outside the descriptors, so each group of NB windows is one
-    // straight-line block and the prefetches stay in flight across it.
-    for (int w0 = 0; w0 < nw; w0 += G) {
+    for (int jg = 0; jg < ngrp; ++jg) {
+        const int cur = jg & 1;
+        tbs::u32x4 vr[IOT::NCH];
+        bulk_load<VG, TF>(rg, gbase(jg + 1, sizeof(TF)), lane, vr);   // next group's g
 #pragma unroll
         for (int bb = 0; bb < G; ++bb) {
-            const int w = w0 + bb;
+            const int w = jg * G + bb;
             const int b = bb % NB;
             const int bn = (b + PF) % NB;
-#pragma unroll
-            for (int j = 0; j < NV; ++j) eb[bn][j] = tbs::bload<TF>(rEF, foff(w + PF, j), 0);
-            gb[bn] = tbs::bload<TF>(rg, vidx(w + PF) * sizeof(TF), 0);
+            load_row(rEF, w + PF, eb[bn]);
+            const TF gv = s_vin[cur][bb * LW + lane];
             TF out = TF(0);
 #pragma unroll
             for (int st = 0; st < SPW; ++st) {
@@ -291,42 +366,51 @@ __global__ __launch_bounds__(64) void fthomas_kernel(const SysDev<TF> *__restric
                     if (j + 1 < NV) x1 += eb[b][j + 1] * d[j + 1];
                     if (j + 2 < NV) x2 += eb[b][j + 2] * d[j + 2];
                 }
-                const TF v = gb[b] - ((x0 + x1) + x2);
+                const TF v = gv - ((x0 + x1) + x2);
 #pragma unroll
                 for (int j = 0; j < NV; ++j) d[j] = rdlane(v, st * NV + j);
                 out = (st_of_lane == st) ? v : out;
             }
-            tbs::bstore<TF>(rd, out, vidx(w) * sizeof(TF), 0);
+            if (lane < LW) s_out[bb * LW + lane] = out;
         }
+        bulk_store<VG, TF>(rd, gbase(jg, sizeof(TF)), lane, s_out);
+        bulk_commit<VG, TF>(vr, lane, s_vin[cur ^ 1]);
     }
     __threadfence_block();
 
     // ---- backward: d_k = d'_k - Gamma_k d_{k+1};  U_k += d_k
-    TF pb[NB];
-    double ub[NB];
+    // Same group partition as the forward pass, visited last to first;
+    // windows past the line end read 0 and keep d = 0.
+    const int wtop = ngrp * G;        // one past the last (padded) window
 #pragma unroll
     for (int j = 0; j < NV; ++j) d[j] = TF(0);
-    // Window indices below 0 give out-of-range offsets: loads read 0, stores
-    // are dropped (same straight-line structure as the forward pass).
 #pragma unroll
-    for (int p = 0; p < PF; ++p) {
-        const int w = nw - 1 - p;
-#pragma unroll
-        for (int j = 0; j < NV; ++j) eb[p][j] = tbs::bload<TF>(rGB, foff(w, j), 0);
-        pb[p] = tbs::bload<TF>(rd, vidx(w) * sizeof(TF), 0);
-        ub[p] = tbs::bload<double>(rU, vidx(w) * 8u, 0);
+    for (int p = 0; p < PF; ++p) load_row(rGB, wtop - 1 - p, eb[p]);
+    {
+        tbs::u32x4 vr[IOT::NCH];
+        tbs::u32x4 ur[IOD::NCH];
+        bulk_load<VG, TF>(rd, gbase(ngrp - 1, sizeof(TF)), lane, vr);
+        bulk_load<VG, double>(rU, gbase(ngrp - 1, 8), lane, ur);
+        bulk_commit<VG, TF>(vr, lane, s_vin[0]);
+        bulk_commit<VG, double>(ur, lane, s_u[0]);
     }
-    for (int w0 = 0; w0 < nw; w0 += G) {
+    for (int t = 0; t < ngrp; ++t) {
+        const int jg = ngrp - 1 - t;
+        const int cur = t & 1;
+        tbs::u32x4 vr[IOT::NCH];
+        tbs::u32x4 ur[IOD::NCH];
+        if (jg > 0) {                                                   // previous group's d', U
+            bulk_load<VG, TF>(rd, gbase(jg - 1, sizeof(TF)), lane, vr);
+            bulk_load<VG, double>(rU, gbase(jg - 1, 8), lane, ur);
+        }
 #pragma unroll
         for (int bb = 0; bb < G; ++bb) {
-            const int w = nw - 1 - (w0 + bb);
+            const int w = wtop - 1 - (t * G + bb);
             const int b = bb % NB;
             const int bn = (b + PF) % NB;
-            const int wn = w - PF;
-#pragma unroll
-            for (int j = 0; j < NV; ++j) eb[bn][j] = tbs::bload<TF>(rGB, foff(wn, j), 0);
-            pb[bn] = tbs::bload<TF>(rd, vidx(wn) * sizeof(TF), 0);
-            ub[bn] = tbs::bload<double>(rU, vidx(wn) * 8u, 0);
+            load_row(rGB, w - PF, eb[bn]);
+            const int slot = (G - 1 - bb) * LW + lane;                  // window w inside group jg
+            const TF pv = s_vin[cur][slot];
             TF out = TF(0);
 #pragma unroll
             for (int st = SPW - 1; st >= 0; --st) {
@@ -337,12 +421,17 @@ __global__ __launch_bounds__(64) void fthomas_kernel(const SysDev<TF> *__restric
                     if (j + 1 < NV) x1 += eb[b][j + 1] * d[j + 1];
                     if (j + 2 < NV) x2 += eb[b][j + 2] * d[j + 2];
                 }
-                const TF v = pb[b] - ((x0 + x1) + x2);
+                const TF v = pv - ((x0 + x1) + x2);
 #pragma unroll
                 for (int j = 0; j < NV; ++j) d[j] = rdlane(v, st * NV + j);
                 out = (st_of_lane == st) ? v : out;
             }
-            tbs::bstore<double>(rU, ub[b] + (double)out, vidx(w) * 8u, 0);
+            if (lane < LW) s_u[cur][slot] += (double)out;
+        }
+        bulk_store<VG, double>(rU, gbase(jg, 8), lane, s_u[cur]);
+        if (jg > 0) {
+            bulk_commit<VG, TF>(vr, lane, s_vin[cur ^ 1]);
+            bulk_commit<VG, double>(ur, lane, s_u[cur ^ 1]);
         }
     }
 }
@@ -487,18 +576,20 @@ __global__ void invform_kernel(int nle, const int *__restrict__ lines,
     const int g = m / kGroup, lane = m % kGroup;
 #pragma unroll
     for (int i = 0; i < NV; ++i) W[(size_t)g * BS * kGroup + tbs::Soa64<BS, TF>::index(i + NV * c, lane)] = (TF)x[i];
+    // row i of step k sits in lane st*NV + i of window w, entry j = column c
+    using WL = tbs::WinLayout<NV, LW, TF>;
     const int w = k / SPW, st = k % SPW;
-    const size_t wbase = ((size_t)(woff[l] + w) * NV + c) * LW + (size_t)st * NV;
+    const size_t wb = (size_t)(woff[l] + w) * WL::STRIDE;
     if (k >= 1) {
 #pragma unroll
         for (int i = 0; i < NV; ++i) b[i] = A[(size_t)BS * e + i + NV * c];
         solve(b, x);
 #pragma unroll
-        for (int i = 0; i < NV; ++i) EF[wbase + i] = (TF)x[i];
+        for (int i = 0; i < NV; ++i) EF[wb + WL::index(c, st * NV + i)] = (TF)x[i];
     }
     if (k <= n - 2) {
 #pragma unroll
-        for (int i = 0; i < NV; ++i) GB[wbase + i] = (TF)DinvC[(size_t)BS * e + i + NV * c];
+        for (int i = 0; i < NV; ++i) GB[wb + WL::index(c, st * NV + i)] = (TF)DinvC[(size_t)BS * e + i + NV * c];
     }
 }
 
@@ -609,6 +700,12 @@ SysDev<TF> make_sysdev(const tbs_plan *p) {
 
 size_t tf_size(int prec) { return prec == TBS_FP32 ? sizeof(float) : sizeof(double); }
 
+// Thomas factor-window stride in values (tbs::WinLayout::STRIDE for this nvar/precision).
+size_t win_stride(int nv, int prec) {
+    const size_t tf = tf_size(prec), lw = (size_t)(64 / nv) * nv;
+    return (nv * lw * tf + 15) / 16 * 16 / tf;
+}
+
 int check_mesh(const tbs_mesh_host *m) {
     TBS_REQUIRE(m != nullptr, TBS_ERR_ARG, "null mesh");
     TBS_REQUIRE(m->nelem >= 1 && m->nline >= 1 && m->nintface >= 0 && m->eftot >= 0, TBS_ERR_ARG,
@@ -676,13 +773,12 @@ Layout layout_of(const tbs_mesh_host *m) {
 
 size_t footprint_of(const tbs_mesh_host *m, const Layout &L, int prec) {
     const size_t nv = m->nvar, bs = nv * nv, tf = tf_size(prec);
-    const size_t spw = 64 / nv, lw = spw * nv;
     const size_t G = (size_t)L.ngroups * kGroup;
     size_t b = 0;
     b += G * L.S * bs * 8;                // residual blocks
     b += G * L.F * 4;                     // neighbour table
     b += G * bs * tf;                     // W
-    b += 2 * (size_t)L.nwin * nv * lw * tf;  // E, Gamma windows
+    b += 2 * (size_t)L.nwin * win_stride((int)nv, prec) * tf;  // E, Gamma windows
     b += (size_t)m->nlineelem * nv * (8 * 3 + 2 * tf);  // rhs U R g dUp
     b += (size_t)L.ngroups * 8 + kHistCap * 8 + 64;
     b += (size_t)m->nlineelem * 4 * 2 + (size_t)(m->nline + 1) * 4 * 3;
@@ -758,13 +854,14 @@ int launch_residual(hipStream_t st, const void *sys, const int2 *gtab, int ng, i
 // cycles K per loop iteration}; index 0 is the default.
 struct TV { int pf, k; };
 template <int NV, typename TF> struct ThomasV { static constexpr int n = 1; static constexpr TV v[1] = {{3, 2}}; };
+// (LDS per line stays <= 31 KiB so five lines fit a CU.)
 template <> struct ThomasV<9, double> {
     static constexpr int n = 4;
-    static constexpr TV v[4] = {{3, 2}, {2, 3}, {3, 4}, {5, 2}};
+    static constexpr TV v[4] = {{3, 2}, {2, 3}, {5, 2}, {2, 2}};
 };
 template <> struct ThomasV<9, float> {
     static constexpr int n = 4;
-    static constexpr TV v[4] = {{5, 2}, {3, 3}, {7, 2}, {3, 5}};
+    static constexpr TV v[4] = {{3, 3}, {5, 2}, {7, 2}, {3, 2}};
 };
 
 template <int NV, typename TF, int PF, int K>
@@ -933,7 +1030,7 @@ int tbs_plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pla
     p->h_woff = L.woff;
     const size_t G = (size_t)L.ngroups * kGroup;
     const size_t tf = tf_size(precision);
-    const int spw = 64 / nv, lw = spw * nv;
+    const int spw = 64 / nv;
 
     // host tables
     std::vector<int> mline(p->nle), inv(p->nelem, -1);
@@ -987,8 +1084,8 @@ int tbs_plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pla
     TBS_TRY(dalloc(P, &P->d_nbr, nbr.size()));
     TBS_TRY(dalloc(P, &P->d_slotc, slotc.size()));
     TBS_TRY(dalloc(P, (char **)&P->d_W, G * BS * tf));
-    TBS_TRY(dalloc(P, (char **)&P->d_EF, (size_t)L.nwin * nv * lw * tf));
-    TBS_TRY(dalloc(P, (char **)&P->d_GB, (size_t)L.nwin * nv * lw * tf));
+    TBS_TRY(dalloc(P, (char **)&P->d_EF, (size_t)L.nwin * win_stride(nv, precision) * tf));
+    TBS_TRY(dalloc(P, (char **)&P->d_GB, (size_t)L.nwin * win_stride(nv, precision) * tf));
     const size_t padded = (size_t)(P->nle + 2 * kPad) * nv;
     TBS_TRY(dalloc(P, &P->a_rhs, padded));
     TBS_TRY(dalloc(P, &P->a_U, padded));
@@ -1080,7 +1177,6 @@ int tbs_plan_factor(tbs_plan *p, const double *jacD, const double *jacO1, const 
     const int nv = p->nv, BS = nv * nv;
     const size_t G = (size_t)p->ngroups * kGroup;
     const size_t tf = tf_size(p->prec);
-    const int lw = (64 / nv) * nv;
     // residual blocks
     {
         const long total = (long)G * p->S * BS;
@@ -1119,8 +1215,8 @@ int tbs_plan_factor(tbs_plan *p, const double *jacD, const double *jacO1, const 
         }
         if ((rc = tbs_lineLU(ctx, p->nelem, p->nintface, p->nline, p->d_fc, p->d_linesize,
                              p->d_linepoint, p->d_lines, p->d_lineface, Dp, jacO1, jacO2, DinvC))) break;
-        if (hipMemsetAsync(p->d_EF, 0, (size_t)p->nwin * nv * lw * tf, st) ||
-            hipMemsetAsync(p->d_GB, 0, (size_t)p->nwin * nv * lw * tf, st) ||
+        if (hipMemsetAsync(p->d_EF, 0, (size_t)p->nwin * win_stride(nv, p->prec) * tf, st) ||
+            hipMemsetAsync(p->d_GB, 0, (size_t)p->nwin * win_stride(nv, p->prec) * tf, st) ||
             hipMemsetAsync(p->d_W, 0, G * BS * tf, st)) { rc = TBS_ERR_STATE; break; }
         const long threads = (long)p->nle * nv;
         switch (nv) {
@@ -1236,7 +1332,6 @@ int tbs_plan_history(tbs_plan *p, double *hist, int cap, int *count) {
 int tbs_plan_internal(tbs_plan *p, int which, void **dptr, size_t *bytes) {
     TBS_REQUIRE(p && dptr && bytes, TBS_ERR_ARG, "null argument");
     const size_t tf = tf_size(p->prec), nv = p->nv, n = p->nle, G = (size_t)p->ngroups * kGroup;
-    const size_t lw = (64 / nv) * nv;
     switch (which) {
         case 0: *dptr = p->d_g; *bytes = n * nv * tf; break;
         case 1: *dptr = p->d_dUp; *bytes = n * nv * tf; break;
@@ -1244,8 +1339,8 @@ int tbs_plan_internal(tbs_plan *p, int which, void **dptr, size_t *bytes) {
         case 3: *dptr = p->d_rhs; *bytes = n * nv * 8; break;
         case 4: *dptr = p->d_R; *bytes = n * nv * 8; break;
         case 5: *dptr = p->d_W; *bytes = G * nv * nv * tf; break;
-        case 6: *dptr = p->d_EF; *bytes = (size_t)p->nwin * nv * lw * tf; break;
-        case 7: *dptr = p->d_GB; *bytes = (size_t)p->nwin * nv * lw * tf; break;
+        case 6: *dptr = p->d_EF; *bytes = (size_t)p->nwin * win_stride((int)nv, p->prec) * tf; break;
+        case 7: *dptr = p->d_GB; *bytes = (size_t)p->nwin * win_stride((int)nv, p->prec) * tf; break;
         case 8: *dptr = p->d_blk; *bytes = G * p->S * nv * nv * 8; break;
         default: TBS_REQUIRE(false, TBS_ERR_ARG, "unknown internal array %d", which);
     }
