@@ -73,6 +73,124 @@ def cpu_baseline(tbs, args, nelem_full):
             "t_sweep_sample_s": out["t_sweeps"] / 3.0, "t_factor_sample_s": out["t_factor"]}
 
 
+def bench_batched(tbs, ctx, args, seed=3):
+    """64 independent systems of uneven lines (96..1280 elements, 8..40 lines
+    per system, 384,000 elements in all) advanced by one launch per kernel per
+    sweep, FP64 and FP32-factor; Thomas variant chosen by the autotuner."""
+    import numpy as np
+    shapes = tbs.batch_shapes(seed, nsys=64, total=384000, lmin=96, lmax=1280, nlmin=8, nlmax=40)
+    meshes = [tbs.ragged_mesh(s) for s in shapes]
+    res = {"workload": f"batch64_384000_seed{seed}", "nsys": len(meshes),
+           "nelem": int(sum(m.nelem for m in meshes)), "nline": int(sum(m.nline for m in meshes))}
+    bufs = []
+    for i, m in enumerate(meshes):
+        z = lambda k: ctx.array(size=k, zero=False)
+        D, O1, O2, rhs = z(81 * m.nelem), z(81 * m.nintface), z(81 * m.nintface), z(9 * m.nelem)
+        ctx.generate_device(1000 + i, m.nelem, m.nintface, args.shift, D, O1, O2, rhs)
+        bufs.append((D, O1, O2, rhs))
+    ctx.sync()
+    for prec, name in ((tbs.TBS_FP64, "fp64"), (tbs.TBS_FP32, "fp32_factor")):
+        plans = [tbs.Plan(ctx, m, prec) for m in meshes]
+        t0 = time.perf_counter()
+        for p, (D, O1, O2, _) in zip(plans, bufs):
+            p.factor(D, O1, O2)
+        ctx.sync()
+        t_factor = time.perf_counter() - t0
+        b = tbs.Batch(ctx, plans)
+        chosen, us = b.autotune(reps=3)
+        model = sum(p.info()["alg_bytes_per_sweep"] for p in plans)
+        steps = args.steps
+        for p, (_, _, _, rhs) in zip(plans, bufs):
+            p.begin(rhs, None)
+        b.launch_sweeps(args.warmup, 4000)
+        ctx.sync()
+        e0, e1 = ctx.event(), ctx.event()
+        ctx.record(e0)
+        b.launch_sweeps(steps, 4000)
+        ctx.record(e1)
+        ctx.sync()
+        t = ctx.elapsed_ms(e0, e1) * 1e-3
+        ctx.destroy_event(e0)
+        ctx.destroy_event(e1)
+        # time to 1e-10 from U = 0 (factorisation timed above, solve here)
+        Us = [ctx.array(size=9 * m.nelem) for m in meshes]
+        ctx.sync()
+        t1 = time.perf_counter()
+        hists, ns = b.solve([x[3] for x in bufs], Us, 100, rtol=1e-10, sweeps_per_check=2)
+        ctx.sync()
+        t_solve = time.perf_counter() - t1
+        res[name] = {"sweeps_per_s": steps / t, "gbps_model": model / (t / steps) / 1e9,
+                     "frac_hbm_peak": model / (t / steps) / 1e9 / HBM_PEAK_GBS, "bytes_per_sweep_model": model,
+                     "autotune_variant": int(chosen), "factor_s": t_factor, "solve_to_1e-10_s": t_solve,
+                     "time_to_1e-10_s": t_factor + t_solve, "max_sweeps_to_1e-10": int(max(ns)),
+                     "worst_final_rel_residual": float(max(h[-1] / h[0] for h in hists))}
+        b.close()
+        for p in plans:
+            p.close()
+        for u in Us:
+            u.close()
+    for t in bufs:
+        for x in t:
+            x.close()
+    return res
+
+
+def bench_serving(tbs, ctx, args, nreq=160, seed=7):
+    """Continuous-batching server, FP32 factors / FP64 residual, tolerance
+    1e-10.  Request sizes n log-uniform in [3840, 384000], line length L
+    uniform in [64, 960], grid max(1, round(n/L)) x L.  Saturated throughput
+    from a burst of nreq requests; then Poisson arrivals at 0.5, 0.8 and 0.95
+    of it, latency = scheduled arrival -> result on the host."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+
+    def draw(k):
+        n = np.exp(rng.uniform(np.log(3840), np.log(384000), k))
+        L = rng.integers(64, 961, k)
+        NL = np.maximum(1, np.rint(n / L)).astype(int)
+        return [(int(a), int(b)) for a, b in zip(NL, L)]
+
+    def run(sizes, arrivals):
+        srv = tbs.Server(ctx, precision=tbs.TBS_FP32, rtol=1e-10, max_sweeps=100, sweeps_per_step=2)
+        t0 = srv.info()["now"]
+        pending = list(range(len(sizes)))
+        rid_of, lat, nsw, status = {}, [], [], []
+        k = 0
+        while len(lat) < len(sizes):
+            now = srv.info()["now"] - t0
+            while k < len(sizes) and arrivals[k] <= now:
+                NL, NK = sizes[k]
+                rid_of[srv.submit(NL, NK, 5000 + k, args.shift)] = k
+                k += 1
+            srv.step()
+            for rid in srv.poll():
+                r = srv.result(rid, want_U=False)
+                lat.append(r["t_done"] - t0 - arrivals[rid_of[rid]])
+                nsw.append(r["nsweeps"])
+                status.append(r["status"])
+                srv.release(rid)
+        wall = srv.info()["now"] - t0
+        srv.close()
+        return np.array(lat), wall, nsw, status
+
+    sizes = draw(nreq)
+    elems = sum(a * b for a, b in sizes)
+    lat, wall, nsw, st = run(sizes, np.zeros(nreq))
+    sat = nreq / wall
+    out = {"workload": f"serve_{nreq}req_seed{seed}", "mean_elements": elems / nreq,
+           "saturated_req_per_s": sat, "saturated_elements_per_s": elems / wall,
+           "all_converged": bool(all(s == 1 for s in st)), "max_sweeps": int(max(nsw)), "loads": {}}
+    for frac in (0.5, 0.8, 0.95):
+        sizes = draw(nreq)
+        arr = np.cumsum(rng.exponential(1.0 / (frac * sat), nreq))
+        lat, wall, nsw, st = run(sizes, arr)
+        out["loads"][f"{frac:g}"] = {"offered_req_per_s": frac * sat, "achieved_req_per_s": nreq / wall,
+                                     "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                                     "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                                     "all_converged": bool(all(s == 1 for s in st))}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -207,6 +325,8 @@ def main():
         for d in (a, b):
             d.pop("U")
         out["time_to_1e-10"] = {"fp64": a, "fp32_factor": b, "fp32_vs_fp64_U_relerr": err}
+        out["batched"] = bench_batched(tbs, ctx, args)
+        out["serving"] = bench_serving(tbs, ctx, args)
     if not args.no_cpu:
         cb = cpu_baseline(tbs, args, n)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}

@@ -200,11 +200,53 @@ int tbs_batch_create(tbs_ctx *ctx, tbs_plan *const *plans, int nplans,
                      tbs_batch **out);
 int tbs_batch_destroy(tbs_batch *batch);
 int tbs_batch_autotune(tbs_batch *batch, int reps, int *chosen, double *us, int cap);
+/* Enqueue nsweeps sweeps of every still-active system (asynchronous); the
+ * plans must have been started with tbs_plan_begin.                        */
+int tbs_batch_launch_sweeps(tbs_batch *batch, int nsweeps, int max_sweeps, double rtol);
+/* Wait for the batch's stream, then copy every system's state:
+ * states[4*s + 0] = residual norms recorded (sweeps + 1), [1] = active,
+ * [2] = status (1 converged, 2 non-finite, 3 sweep cap, 4 diverging).    */
+int tbs_batch_states(tbs_batch *batch, int *states);
 /* rhs[i], U[i]: device pointers of plan i; hist: host [nplans][max_sweeps+1];
  * nsweeps: host [nplans].                                                  */
 int tbs_batch_solve(tbs_batch *batch, const double *const *rhs,
                     double *const *U, int max_sweeps, double rtol,
                     int sweeps_per_check, double *hist, int *nsweeps);
+
+/* ---------------------------------------------------------------------- */
+/* Serving: continuous batching of solve requests on one GPU.  Requests    */
+/* wait in a FIFO; each tbs_server_step runs K sweeps of the active batch, */
+/* admits queued requests whose HBM footprint fits the budget (generation, */
+/* factorisation and first residual on a second stream, overlapping the   */
+/* batch), retires converged ones (U copied to host) and lets admitted ones */
+/* join.  A request's result does not depend on what it is batched with.  */
+/* ---------------------------------------------------------------------- */
+typedef struct tbs_server tbs_server;
+typedef struct tbs_server_request {
+    int NL, NK;          /* seeded structured system: NL lines of NK elements */
+    uint64_t seed;
+    double shift;        /* diagonal shift of the generator                  */
+} tbs_server_request;
+/* hbm_budget 0 = 80% of the device's free memory at creation. */
+int tbs_server_create(tbs_ctx *ctx, size_t hbm_budget, int precision, double rtol,
+                      int max_sweeps, int sweeps_per_step, tbs_server **out);
+int tbs_server_destroy(tbs_server *srv);
+/* HBM bytes a request holds from admission to retirement (plan, generated
+ * Jacobian, factor workspace, rhs and U); admission keeps the sum of the
+ * admitted requests' footprints within the budget. */
+int tbs_server_footprint(tbs_server *srv, const tbs_server_request *req, size_t *bytes);
+/* TBS_ERR_NOMEM if the request alone exceeds the budget. */
+int tbs_server_submit(tbs_server *srv, const tbs_server_request *req, long *id);
+int tbs_server_step(tbs_server *srv, int *nactive, int *nqueued, int *nadmitting);
+/* ids of requests finished since the last poll */
+int tbs_server_poll(tbs_server *srv, long *ids, int cap, int *n);
+/* U (host, element order), residual history, sweeps, status (1 converged,
+ * 2 non-finite, 3 sweep cap, 4 diverging, -1 admission failed) and
+ * times[3] = submit, admit, done (seconds on the server's clock).          */
+int tbs_server_result(tbs_server *srv, long id, double *U, size_t ucap, double *hist,
+                      int hcap, int *nsweeps, int *status, double *times);
+int tbs_server_release(tbs_server *srv, long id);
+int tbs_server_info(tbs_server *srv, size_t *budget, size_t *used, double *now);
 
 /* ---------------------------------------------------------------------- */
 /* Synthetic inputs (the reference reads files only; these generate the    */
@@ -218,6 +260,20 @@ int tbs_structured_sizes(int NL, int NK, int *nelem, int *nintface,
 int tbs_structured_mesh(int NL, int NK, int *epoint, int *ef, int *fc,
                         int *linesize, int *linepoint, int *lines,
                         int *lineface);
+/* Lines of uneven length (lens[nline]) stacked like the structured grid:
+ * element (l,k) couples in-line to (l,k+-1) and across to (l+-1,k) where
+ * that position exists.  Equal lengths reproduce tbs_structured_mesh.    */
+int tbs_ragged_sizes(int nline, const int *lens, int *nelem, int *nintface,
+                     int *eftot);
+int tbs_ragged_mesh(int nline, const int *lens, int *epoint, int *ef, int *fc,
+                    int *linesize, int *linepoint, int *lines, int *lineface);
+/* Seeded shapes of a batch of nsys systems: nlines[s] ~ U{nlmin..nlmax},
+ * lengths ~ U{lmin..lmax}, scaled to sum to `total` (if > 0), clamped to
+ * [lmin, lmax], remainder settled from the last line backwards.  lens gets
+ * all lengths system by system (capacity cap); *nlens their count.       */
+int tbs_batch_shapes(uint64_t seed, int nsys, long total, int lmin, int lmax,
+                     int nlmin, int nlmax, int *nlines, int *lens, int cap,
+                     int *nlens);
 /* Fill a Jacobian system: D = U[-1,1] + shift*I, Of1/Of2 = U[-0.1,0.1],
  * rhs = U[-1,1].  Host or device variant (device: on the ctx stream).    */
 int tbs_generate_host(uint64_t seed, int nvar, int nelem, int nintface,

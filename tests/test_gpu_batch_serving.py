@@ -1,0 +1,115 @@
+"""GPU: batched uneven lines (many systems per launch) and the continuous-
+batching server.  Bar: every system's result is bitwise the one it gets when
+solved alone, converges to 1e-10, and its history matches the oracle
+(normalised history within 1e-12 * max(h, 1e-3) for FP64)."""
+import numpy as np
+import pytest
+
+import closed_forms as cf
+from gpu_util import dev, host, zeros
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx(tbs):
+    c = tbs.Context(0, 9)
+    yield c
+    c.close()
+
+
+def _solve_alone(tbs, ctx, m, j, prec, iters, rtol):
+    p = tbs.Plan(ctx, m, prec)
+    p.factor(dev(j["jacD"]), dev(j["jacO1"]), dev(j["jacO2"]))
+    U = zeros(9 * m.nelem)
+    hist, ns = p.solve(dev(j["rhs"]), U, iters, rtol)
+    p.close()
+    return hist, ns, host(U)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_uneven_batch_autotuned_graph(tbs, ctx, oracle, prec):
+    print(f"case uneven batch prec {prec}", flush=True)
+    shapes = tbs.batch_shapes(5, nsys=8, total=30000, lmin=96, lmax=1280, nlmin=3, nlmax=9)
+    meshes = [tbs.ragged_mesh(s) for s in shapes]
+    jacs = [tbs.generate_host(100 + i, 9, m.nelem, m.nintface, 20.0) for i, m in enumerate(meshes)]
+    plans = []
+    for m, j in zip(meshes, jacs):
+        p = tbs.Plan(ctx, m, prec)
+        p.factor(dev(j["jacD"]), dev(j["jacO1"]), dev(j["jacO2"]))
+        plans.append(p)
+    b = tbs.Batch(ctx, plans)
+    chosen, us = b.autotune(reps=2)
+    Us = [zeros(9 * m.nelem) for m in meshes]
+    hists, ns = b.solve([dev(j["rhs"]) for j in jacs], Us, 40, rtol=1e-10, sweeps_per_check=4)
+    b.close()
+    for p in plans:
+        p.close()
+    for i, (m, j) in enumerate(zip(meshes, jacs)):
+        h, n, U = _solve_alone(tbs, ctx, m, j, prec, 40, 1e-10)
+        assert ns[i] == n and np.array_equal(hists[i], h) and np.array_equal(host(Us[i]), U), i
+        assert h[-1] <= 1e-10 * h[0]
+        if prec == 0:
+            ref = oracle.lij_run(m, j, n)
+            assert cf.history_close(h, ref["hist"])[0]
+            assert np.linalg.norm(U - ref["U"]) <= 1e-12 * np.linalg.norm(ref["U"])
+
+
+def test_server_results_equal_solo_solves(tbs, ctx):
+    print("case server mix", flush=True)
+    specs = [(3, 64, 11), (40, 96, 12), (7, 960, 13), (120, 320, 14), (1, 64, 15), (60, 64, 16), (25, 500, 17)]
+    srv = tbs.Server(ctx, precision=tbs.TBS_FP32, rtol=1e-10, max_sweeps=60, sweeps_per_step=2)
+    ids = {}
+    for k, (NL, NK, seed) in enumerate(specs):
+        ids[srv.submit(NL, NK, seed, 12.0)] = (NL, NK, seed)
+        if k % 2:
+            srv.step()
+    done = {}
+    for _ in range(2000):
+        a, q, m = srv.step()
+        for rid in srv.poll():
+            done[rid] = srv.result(rid)
+            srv.release(rid)
+        if len(done) == len(specs):
+            break
+    assert len(done) == len(specs)
+    for rid, (NL, NK, seed) in ids.items():
+        r = done[rid]
+        assert r["status"] == 1 and r["hist"][-1] <= 1e-10 * r["hist"][0]
+        assert r["t_submit"] <= r["t_admit"] <= r["t_done"]
+        mesh = tbs.structured_mesh(NL, NK)
+        jac = tbs.generate_host(seed, 9, mesh.nelem, mesh.nintface, 12.0)
+        h, n, U = _solve_alone(tbs, ctx, mesh, jac, tbs.TBS_FP32, 60, 1e-10)
+        assert r["nsweeps"] == n and np.array_equal(r["hist"], h), (NL, NK)
+        assert np.array_equal(r["U"], U), (NL, NK)
+    srv.close()
+
+
+def test_server_admits_by_footprint(tbs, ctx):
+    """A budget that holds about two requests still completes a queue of six,
+    in FIFO order of admission, and refuses a request larger than the budget."""
+    print("case server budget", flush=True)
+    probe = tbs.Server(ctx, precision=tbs.TBS_FP32)
+    fp = probe.footprint(40, 200)
+    probe.close()
+    assert fp > tbs.Plan.footprint(tbs.structured_mesh(40, 200), tbs.TBS_FP32)
+    budget = int(2.5 * fp)
+    srv = tbs.Server(ctx, hbm_budget=budget, precision=tbs.TBS_FP32, rtol=1e-10, max_sweeps=60, sweeps_per_step=2)
+    rids = [srv.submit(40, 200, 30 + i) for i in range(6)]
+    with pytest.raises(tbs.TbsError) as ei:
+        srv.submit(2000, 500, 1)
+    assert ei.value.code == -4
+    done, peak = {}, 0
+    for _ in range(4000):
+        a, q, m = srv.step()
+        peak = max(peak, a + m)
+        assert srv.info()["used"] <= budget
+        for rid in srv.poll():
+            done[rid] = srv.result(rid, want_U=False)
+        if len(done) == 6:
+            break
+    assert len(done) == 6 and all(d["status"] == 1 for d in done.values())
+    assert peak == 2
+    admits = [done[r]["t_admit"] for r in rids]
+    assert admits == sorted(admits)
+    srv.close()

@@ -32,6 +32,10 @@ class MeshHost(C.Structure):
     ]
 
 
+class ServerRequest(C.Structure):
+    _fields_ = [("NL", C.c_int), ("NK", C.c_int), ("seed", C.c_uint64), ("shift", C.c_double)]
+
+
 class PlanInfo(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("nvar", "nelem", "nline", "max_faces", "precision")] + [
         ("device_bytes", C.c_size_t),
@@ -98,8 +102,22 @@ SIGNATURES = {
     "tbs_batch_destroy": [P],
     "tbs_batch_autotune": [P, I, C.POINTER(C.c_int), P, I],
     "tbs_batch_solve": [P, P, P, I, D, I, P, P],
+    "tbs_batch_launch_sweeps": [P, I, I, D],
+    "tbs_batch_states": [P, P],
+    "tbs_server_create": [P, C.c_size_t, I, D, I, I, PP],
+    "tbs_server_destroy": [P],
+    "tbs_server_footprint": [P, C.POINTER(ServerRequest), C.POINTER(C.c_size_t)],
+    "tbs_server_submit": [P, C.POINTER(ServerRequest), C.POINTER(C.c_long)],
+    "tbs_server_step": [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "tbs_server_poll": [P, P, I, C.POINTER(C.c_int)],
+    "tbs_server_result": [P, C.c_long, P, C.c_size_t, P, I, C.POINTER(C.c_int), C.POINTER(C.c_int), P],
+    "tbs_server_release": [P, C.c_long],
+    "tbs_server_info": [P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), C.POINTER(C.c_double)],
     "tbs_structured_sizes": [I, I, P, P, P, P, P],
     "tbs_structured_mesh": [I, I, P, P, P, P, P, P, P],
+    "tbs_ragged_sizes": [I, P, P, P, P],
+    "tbs_ragged_mesh": [I, P, P, P, P, P, P, P, P],
+    "tbs_batch_shapes": [C.c_uint64, I, L, I, I, I, I, P, P, I, P],
     "tbs_generate_host": [C.c_uint64, I, I, I, D, P, P, P, P],
     "tbs_generate_device": [P, C.c_uint64, I, I, I, D, P, P, P, P],
     "tbs_read_mesh_header": [C.c_char_p, P],

@@ -670,6 +670,7 @@ struct tbs_batch {
     int nl = 0, ng = 0;
     std::map<std::pair<int, std::pair<int, double>>, hipGraphExec_t> graphs;
     int *h_state = nullptr;
+    int *h_states = nullptr;    // pinned [nplans][4]
     int variant = 0;
 };
 
@@ -824,7 +825,6 @@ int dalloc(tbs_plan *p, T **ptr, size_t count) {
     return TBS_OK;
 }
 
-#define TBS_TRY(x) do { int _rc = (x); if (_rc) return _rc; } while (0)
 
 void free_plan(tbs_plan *p) {
     if (!p) return;
@@ -1030,7 +1030,6 @@ int tbs_plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pla
     p->h_woff = L.woff;
     const size_t G = (size_t)L.ngroups * kGroup;
     const size_t tf = tf_size(precision);
-    const int spw = 64 / nv;
 
     // host tables
     std::vector<int> mline(p->nle), inv(p->nelem, -1);
@@ -1478,7 +1477,8 @@ int tbs_batch_create(tbs_ctx *ctx, tbs_plan *const *plans, int nplans, tbs_batch
     *out = nullptr;
     const int nv = plans[0]->nv, prec = plans[0]->prec;
     for (int i = 0; i < nplans; ++i) {
-        TBS_REQUIRE(plans[i] && plans[i]->ctx == ctx, TBS_ERR_ARG, "plan %d not of this context", i);
+        TBS_REQUIRE(plans[i] && plans[i]->ctx->device == ctx->device, TBS_ERR_ARG,
+                    "plan %d is on another device", i);
         TBS_REQUIRE(plans[i]->nv == nv && plans[i]->prec == prec, TBS_ERR_ARG,
                     "plans in a batch must share nvar and precision");
         TBS_REQUIRE(plans[i]->factored, TBS_ERR_STATE, "plan %d not factored", i);
@@ -1515,6 +1515,7 @@ int tbs_batch_create(tbs_ctx *ctx, tbs_plan *const *plans, int nplans, tbs_batch
     if ((e = hipMalloc((void **)&b->d_ltab, ltab.size() * sizeof(int4)))) return fail(e);
     if ((e = hipMalloc((void **)&b->d_gtab, gtab.size() * sizeof(int2)))) return fail(e);
     if ((e = hipHostMalloc((void **)&b->h_state, 4 * sizeof(int), hipHostMallocDefault))) return fail(e);
+    if ((e = hipHostMalloc((void **)&b->h_states, 4 * sizeof(int) * nplans, hipHostMallocDefault))) return fail(e);
     TBS_HIP(hipMemcpy(b->d_sys, sysh.data(), sysh.size(), hipMemcpyHostToDevice));
     TBS_HIP(hipMemcpy(b->d_ltab, ltab.data(), ltab.size() * sizeof(int4), hipMemcpyHostToDevice));
     TBS_HIP(hipMemcpy(b->d_gtab, gtab.data(), gtab.size() * sizeof(int2), hipMemcpyHostToDevice));
@@ -1545,6 +1546,29 @@ int tbs_batch_autotune(tbs_batch *b, int reps, int *chosen, double *us, int cap)
     return TBS_OK;
 }
 
+int tbs_batch_launch_sweeps(tbs_batch *b, int nsweeps, int max_sweeps, double rtol) {
+    TBS_REQUIRE(b && nsweeps >= 0 && max_sweeps >= 0 && max_sweeps < kHistCap, TBS_ERR_ARG, "bad arguments");
+    TBS_TRY(tbs::use_ctx(b->ctx));
+    hipStream_t st = b->ctx->stream;
+    for (int i = 0; i < nsweeps; ++i)
+        TBS_TRY(enqueue_sweep(b->nv, b->prec, st, b->d_sys, (int)b->plans.size(), b->d_ltab, b->nl, b->d_gtab,
+                              b->ng, max_sweeps, rtol, b->variant));
+    return TBS_OK;
+}
+
+int tbs_batch_states(tbs_batch *b, int *states) {
+    TBS_REQUIRE(b && states, TBS_ERR_ARG, "null argument");
+    TBS_TRY(tbs::use_ctx(b->ctx));
+    hipStream_t st = b->ctx->stream;
+    const int ns = (int)b->plans.size();
+    for (int s2 = 0; s2 < ns; ++s2)
+        TBS_HIP(hipMemcpyAsync(&b->h_states[4 * s2], b->plans[s2]->d_state, 4 * sizeof(int),
+                               hipMemcpyDeviceToHost, st));
+    TBS_HIP(hipStreamSynchronize(st));
+    std::memcpy(states, b->h_states, 4 * sizeof(int) * ns);
+    return TBS_OK;
+}
+
 int tbs_batch_destroy(tbs_batch *b) {
     if (!b) return TBS_OK;
     hipSetDevice(b->ctx->device);
@@ -1554,6 +1578,7 @@ int tbs_batch_destroy(tbs_batch *b) {
     if (b->d_ltab) hipFree(b->d_ltab);
     if (b->d_gtab) hipFree(b->d_gtab);
     if (b->h_state) hipHostFree(b->h_state);
+    if (b->h_states) hipHostFree(b->h_states);
     delete b;
     return TBS_OK;
 }

@@ -25,6 +25,8 @@
 // -1, -2, ...; this reproduces data/ex05/gpuline.mesh.data.bin exactly.
 #include "tbs_common.hpp"
 
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <sys/stat.h>
@@ -144,6 +146,120 @@ int tbs_structured_mesh(int NL, int NK, int *epoint, int *ef, int *fc, int *line
     epoint[n] = 4 * n;
     for (int l = 0; l < NL; ++l) { linesize[l] = NK; linepoint[l] = l * NK; }
     linepoint[NL] = n;
+    return TBS_OK;
+}
+
+int tbs_ragged_sizes(int nline, const int *lens, int *nelem, int *nintface, int *eftot) {
+    TBS_REQUIRE(nline >= 1 && lens, TBS_ERR_ARG, "bad ragged mesh arguments");
+    long n = 0, nif = 0;
+    for (int l = 0; l < nline; ++l) {
+        TBS_REQUIRE(lens[l] >= 1, TBS_ERR_ARG, "line %d has length %d", l, lens[l]);
+        n += lens[l];
+        nif += lens[l] - 1;
+        if (l + 1 < nline) nif += std::min(lens[l], lens[l + 1]);
+    }
+    TBS_REQUIRE(4 * n < (1L << 31), TBS_ERR_ARG, "ragged mesh overflows int32");
+    if (nelem) *nelem = (int)n;
+    if (nintface) *nintface = (int)nif;
+    if (eftot) *eftot = (int)(4 * n);
+    return TBS_OK;
+}
+
+// Lines of uneven length stacked like the structured grid: element (l, k) =
+// start[l] + k, in-line faces (e+1, e), cross faces (start[l+1]+k, e) when
+// line l+1 has a position k; ef = [S, E, N, W]; boundary faces numbered by
+// side (S, E, N, W), each in element order.  Equal lengths give exactly the
+// structured mesh.
+int tbs_ragged_mesh(int nline, const int *lens, int *epoint, int *ef, int *fc, int *linesize,
+                    int *linepoint, int *lines, int *lineface) {
+    int n, nif, eft;
+    int rc = tbs_ragged_sizes(nline, lens, &n, &nif, &eft);
+    if (rc) return rc;
+    TBS_REQUIRE(epoint && ef && fc && linesize && linepoint && lines && lineface, TBS_ERR_ARG,
+                "null mesh array");
+    std::vector<int> start(nline + 1, 0);
+    for (int l = 0; l < nline; ++l) start[l + 1] = start[l] + lens[l];
+    std::vector<int> f_in(n, -1), f_cross(n, -1);
+    int f = 0;
+    for (int l = 0; l < nline; ++l)
+        for (int k = 0; k < lens[l]; ++k) {
+            const int e = start[l] + k;
+            if (k < lens[l] - 1) { f_in[e] = f; fc[2 * f] = e + 1; fc[2 * f + 1] = e; ++f; }
+            if (l + 1 < nline && k < lens[l + 1]) {
+                f_cross[e] = f; fc[2 * f] = start[l + 1] + k; fc[2 * f + 1] = e; ++f;
+            }
+        }
+    // boundary ids by side, element order within a side (0-based: -(b+2))
+    int b = 0;
+    std::vector<int> bS(n, 0), bE(n, 0), bN(n, 0), bW(n, 0);
+    for (int pass = 0; pass < 4; ++pass)
+        for (int l = 0; l < nline; ++l)
+            for (int k = 0; k < lens[l]; ++k) {
+                const int e = start[l] + k;
+                if (pass == 0 && !(l > 0 && k < lens[l - 1])) bS[e] = -(b++) - 2;
+                if (pass == 1 && k == lens[l] - 1) bE[e] = -(b++) - 2;
+                if (pass == 2 && f_cross[e] < 0) bN[e] = -(b++) - 2;
+                if (pass == 3 && k == 0) bW[e] = -(b++) - 2;
+            }
+    for (int l = 0; l < nline; ++l) {
+        linesize[l] = lens[l];
+        linepoint[l] = start[l];
+        for (int k = 0; k < lens[l]; ++k) {
+            const int e = start[l] + k;
+            epoint[e] = 4 * e;
+            ef[4 * e + 0] = (l > 0 && k < lens[l - 1]) ? f_cross[start[l - 1] + k] : bS[e];
+            ef[4 * e + 1] = (k < lens[l] - 1) ? f_in[e] : bE[e];
+            ef[4 * e + 2] = (f_cross[e] >= 0) ? f_cross[e] : bN[e];
+            ef[4 * e + 3] = (k > 0) ? f_in[e - 1] : bW[e];
+            lines[e] = e;
+            lineface[e] = ef[4 * e + 3];
+        }
+    }
+    epoint[n] = 4 * n;
+    linepoint[nline] = n;
+    return TBS_OK;
+}
+
+// Seeded batch shapes: system s gets nl_s ~ U{nlmin..nlmax} lines, line
+// lengths ~ U{lmin..lmax}; when total > 0 every length is scaled by
+// total / sum, rounded, clamped to [lmin, lmax], and the remainder is
+// settled one element at a time from the last line backwards.
+int tbs_batch_shapes(uint64_t seed, int nsys, long total, int lmin, int lmax, int nlmin, int nlmax,
+                     int *nlines, int *lens, int cap, int *nlens) {
+    TBS_REQUIRE(nsys >= 1 && lmin >= 1 && lmax >= lmin && nlmin >= 1 && nlmax >= nlmin && nlines &&
+                    nlens, TBS_ERR_ARG, "bad batch shape arguments");
+    uint64_t st = seed * kG + 0x5851F42D4C957F2Dull;
+    auto next = [&]() { st += kG; return mix64(st); };
+    std::vector<long> len;
+    for (int s = 0; s < nsys; ++s) {
+        nlines[s] = nlmin + (int)(next() % (uint64_t)(nlmax - nlmin + 1));
+        for (int l = 0; l < nlines[s]; ++l) len.push_back(lmin + (long)(next() % (uint64_t)(lmax - lmin + 1)));
+    }
+    const long nl = (long)len.size();
+    TBS_REQUIRE(nl <= cap || !lens, TBS_ERR_ARG, "lens capacity %d < %ld lines", cap, nl);
+    if (total > 0) {
+        TBS_REQUIRE(total >= nl * (long)lmin && total <= nl * (long)lmax, TBS_ERR_ARG,
+                    "total %ld elements not reachable with %ld lines of [%d, %d]", total, nl, lmin, lmax);
+        double sum = 0;
+        for (long v : len) sum += (double)v;
+        const double f = (double)total / sum;
+        long acc = 0;
+        for (long &v : len) {
+            v = std::min<long>(lmax, std::max<long>(lmin, std::lround((double)v * f)));
+            acc += v;
+        }
+        long diff = total - acc;
+        while (diff != 0) {
+            bool moved = false;
+            for (long i = nl - 1; i >= 0 && diff != 0; --i) {
+                if (diff > 0 && len[i] < lmax) { ++len[i]; --diff; moved = true; }
+                else if (diff < 0 && len[i] > lmin) { --len[i]; ++diff; moved = true; }
+            }
+            TBS_REQUIRE(moved, TBS_ERR_ARG, "cannot settle batch total");
+        }
+    }
+    *nlens = (int)nl;
+    if (lens) for (long i = 0; i < nl; ++i) lens[i] = (int)len[i];
     return TBS_OK;
 }
 

@@ -47,6 +47,9 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
         if (!(cond)) { tbs::set_error(__VA_ARGS__); return (code); }         \
     } while (0)
 
+// Propagate a non-zero status.
+#define TBS_TRY(x) do { int _rc = (x); if (_rc) return _rc; } while (0)
+
 // Make the context's device current (every entry point does this first).
 int use_ctx(tbs_ctx *ctx);
 

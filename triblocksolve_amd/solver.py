@@ -158,6 +158,38 @@ def structured_mesh(NL, NK, nvar=9):
 StructuredMesh = structured_mesh
 
 
+def ragged_mesh(lens, nvar=9):
+    """Lines of the given lengths stacked like the structured grid."""
+    lens = np.ascontiguousarray(lens, dtype=np.int32)
+    lib = A.load()
+    s = (C.c_int * 3)()
+    A.check(lib.tbs_ragged_sizes(lens.size, _ptr(lens), C.byref(s, 0), C.byref(s, 4), C.byref(s, 8)))
+    nelem, nif, eftot = list(s)
+    nline = lens.size
+    arrs = dict(epoint=np.empty(nelem + 1, np.int32), ef=np.empty(eftot, np.int32),
+                fc=np.empty(2 * nif, np.int32), linesize=np.empty(nline, np.int32),
+                linepoint=np.empty(nline + 1, np.int32), lines=np.empty(nelem, np.int32),
+                lineface=np.empty(nelem, np.int32))
+    A.check(lib.tbs_ragged_mesh(nline, _ptr(lens), *[_ptr(arrs[f]) for f in MESH_FIELDS]))
+    return HostMesh(nvar=nvar, **arrs)
+
+
+def batch_shapes(seed, nsys=64, total=384000, lmin=96, lmax=1280, nlmin=8, nlmax=40):
+    """Seeded line lengths per system (list of int arrays)."""
+    lib = A.load()
+    nl = np.zeros(nsys, np.int32)
+    cap = nsys * nlmax
+    lens = np.zeros(cap, np.int32)
+    cnt = C.c_int()
+    A.check(lib.tbs_batch_shapes(seed, nsys, total, lmin, lmax, nlmin, nlmax, _ptr(nl), _ptr(lens), cap,
+                                 C.byref(cnt)))
+    out, o = [], 0
+    for s in range(nsys):
+        out.append(lens[o:o + nl[s]].copy())
+        o += nl[s]
+    return out
+
+
 def generate_host(seed, nvar, nelem, nintface, shift):
     """Seeded synthetic Jacobian on the host (product generator)."""
     bs = nvar * nvar
@@ -238,7 +270,7 @@ class Context:
 
     def close(self):
         if self.handle:
-            for ch in sorted(list(self._children), key=lambda o: 0 if isinstance(o, Batch) else
+            for ch in sorted(list(self._children), key=lambda o: 0 if isinstance(o, (Batch, Server)) else
                              (1 if isinstance(o, Plan) else 2)):
                 ch.close()
             self._lib.tbs_ctx_destroy(self.handle)
@@ -437,6 +469,16 @@ class Batch:
         A.check(self.ctx._lib.tbs_batch_autotune(self.handle, reps, C.byref(ch), _ptr(us), nvariants))
         return ch.value, us
 
+    def launch_sweeps(self, n, max_sweeps, rtol=0.0):
+        """Enqueue n sweeps of every system (each plan begun with Plan.begin)."""
+        A.check(self.ctx._lib.tbs_batch_launch_sweeps(self.handle, n, max_sweeps, float(rtol)))
+
+    def states(self):
+        """Per-system [sweeps recorded, active, status, 0] after a device sync."""
+        st = np.zeros((len(self.plans), 4), np.int32)
+        A.check(self.ctx._lib.tbs_batch_states(self.handle, _ptr(st)))
+        return st
+
     def solve(self, rhs_list, U_list, max_sweeps, rtol=0.0, sweeps_per_check=4):
         n = len(self.plans)
         rp = (C.c_void_p * n)(*[_ptr(r).value for r in rhs_list])
@@ -446,3 +488,73 @@ class Batch:
         A.check(self.ctx._lib.tbs_batch_solve(self.handle, rp, up, max_sweeps, float(rtol), sweeps_per_check,
                                               _ptr(hist), _ptr(ns)))
         return [hist[i, : ns[i] + 1] for i in range(n)], ns
+
+
+class Server:
+    """Continuous-batching solve service (the C engine in csrc/server.hip)."""
+
+    def __init__(self, ctx, hbm_budget=0, precision=A.TBS_FP32, rtol=1e-10, max_sweeps=100, sweeps_per_step=4):
+        self.ctx = ctx
+        h = C.c_void_p()
+        A.check(ctx._lib.tbs_server_create(ctx.handle, hbm_budget, precision, float(rtol), max_sweeps,
+                                           sweeps_per_step, C.byref(h)))
+        self.handle = h
+        self.max_sweeps = max_sweeps
+        self.nvar = ctx.nvar
+        self.sizes = {}
+        ctx._children.add(self)
+
+    def close(self):
+        if self.handle and self.ctx.handle:
+            self.ctx._lib.tbs_server_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def footprint(self, NL, NK):
+        req = A.ServerRequest(NL, NK, 0, 0.0)
+        out = C.c_size_t()
+        A.check(self.ctx._lib.tbs_server_footprint(self.handle, C.byref(req), C.byref(out)))
+        return out.value
+
+    def submit(self, NL, NK, seed, shift=12.0):
+        req = A.ServerRequest(NL, NK, seed, float(shift))
+        rid = C.c_long()
+        A.check(self.ctx._lib.tbs_server_submit(self.handle, C.byref(req), C.byref(rid)))
+        self.sizes[rid.value] = NL * NK
+        return rid.value
+
+    def step(self):
+        a, q, m = C.c_int(), C.c_int(), C.c_int()
+        A.check(self.ctx._lib.tbs_server_step(self.handle, C.byref(a), C.byref(q), C.byref(m)))
+        return a.value, q.value, m.value
+
+    def poll(self, cap=4096):
+        ids = np.zeros(cap, np.int64)
+        n = C.c_int()
+        A.check(self.ctx._lib.tbs_server_poll(self.handle, _ptr(ids), cap, C.byref(n)))
+        return [int(x) for x in ids[: n.value]]
+
+    def result(self, rid, want_U=True):
+        n = self.sizes[rid]
+        U = np.zeros(self.nvar * n) if want_U else None
+        hist = np.zeros(self.max_sweeps + 1)
+        ns, st = C.c_int(), C.c_int()
+        times = np.zeros(3)
+        A.check(self.ctx._lib.tbs_server_result(self.handle, rid, _ptr(U), 0 if U is None else U.size, _ptr(hist),
+                                                hist.size, C.byref(ns), C.byref(st), _ptr(times)))
+        return dict(U=U, hist=hist[: ns.value + 1], nsweeps=ns.value, status=st.value, t_submit=times[0],
+                    t_admit=times[1], t_done=times[2])
+
+    def release(self, rid):
+        A.check(self.ctx._lib.tbs_server_release(self.handle, rid))
+        self.sizes.pop(rid, None)
+
+    def info(self):
+        b, u, t = C.c_size_t(), C.c_size_t(), C.c_double()
+        A.check(self.ctx._lib.tbs_server_info(self.handle, C.byref(b), C.byref(u), C.byref(t)))
+        return dict(budget=b.value, used=u.value, now=t.value)
