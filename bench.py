@@ -170,7 +170,9 @@ def bench_serving(tbs, ctx, args, nreq=160, seed=7):
                 status.append(r["status"])
                 srv.release(rid)
         wall = srv.info()["now"] - t0
+        stats = srv.stats()
         srv.close()
+        run.stats = stats
         return np.array(lat), wall, nsw, status
 
     sizes = draw(nreq)
@@ -179,7 +181,8 @@ def bench_serving(tbs, ctx, args, nreq=160, seed=7):
     sat = nreq / wall
     out = {"workload": f"serve_{nreq}req_seed{seed}", "mean_elements": elems / nreq,
            "saturated_req_per_s": sat, "saturated_elements_per_s": elems / wall,
-           "all_converged": bool(all(s == 1 for s in st)), "max_sweeps": int(max(nsw)), "loads": {}}
+           "all_converged": bool(all(s == 1 for s in st)), "max_sweeps": int(max(nsw)),
+           "saturated_host_phase_s": run.stats, "loads": {}}
     for frac in (0.5, 0.8, 0.95):
         sizes = draw(nreq)
         arr = np.cumsum(rng.exponential(1.0 / (frac * sat), nreq))
@@ -187,7 +190,8 @@ def bench_serving(tbs, ctx, args, nreq=160, seed=7):
         out["loads"][f"{frac:g}"] = {"offered_req_per_s": frac * sat, "achieved_req_per_s": nreq / wall,
                                      "p50_ms": float(np.percentile(lat, 50) * 1e3),
                                      "p99_ms": float(np.percentile(lat, 99) * 1e3),
-                                     "all_converged": bool(all(s == 1 for s in st))}
+                                     "all_converged": bool(all(s == 1 for s in st)),
+                                     "host_phase_s": run.stats}
     return out
 
 

@@ -160,7 +160,11 @@ int tbs_plan_create(tbs_ctx *ctx, const tbs_mesh_host *mesh, int precision,
                     tbs_plan **out);
 int tbs_plan_destroy(tbs_plan *plan);
 int tbs_plan_info_get(tbs_plan *plan, tbs_plan_info *info);
-/* Factorise: block-Thomas LU (lineLU semantics) + the sweep-form factors. */
+/* Factorise: the block-Thomas recurrence of lineLU (src/okl/lineLU_v2.okl:
+ * 58-182) carried out in inverse form by one fused kernel that writes the
+ * sweep-form factors W = D'^-1, E = D'^-1 A and Gamma = D'^-1 C directly.
+ * Asynchronous on the context's stream: the Jacobian arrays must stay valid
+ * until the stream has run it. */
 int tbs_plan_factor(tbs_plan *plan, const double *jacD, const double *jacO1,
                     const double *jacO2);
 /* Solve J U = -rhs by LIJ sweeps from the initial U (device, reference
@@ -199,6 +203,10 @@ int tbs_plan_internal(tbs_plan *plan, int which, void **dptr, size_t *bytes);
 int tbs_batch_create(tbs_ctx *ctx, tbs_plan *const *plans, int nplans,
                      tbs_batch **out);
 int tbs_batch_destroy(tbs_batch *batch);
+/* Replace the batch's systems (0 allowed: sweeps become no-ops).  The
+ * descriptor tables are rewritten in place on the batch's stream, ordered
+ * after the sweeps already enqueued; device tables grow when needed. */
+int tbs_batch_set_plans(tbs_batch *batch, tbs_plan *const *plans, int nplans);
 int tbs_batch_autotune(tbs_batch *batch, int reps, int *chosen, double *us, int cap);
 /* Enqueue nsweeps sweeps of every still-active system (asynchronous); the
  * plans must have been started with tbs_plan_begin.                        */
@@ -247,6 +255,11 @@ int tbs_server_result(tbs_server *srv, long id, double *U, size_t ucap, double *
                       int hcap, int *nsweeps, int *status, double *times);
 int tbs_server_release(tbs_server *srv, long id);
 int tbs_server_info(tbs_server *srv, size_t *budget, size_t *used, double *now);
+/* Host time per engine phase, cumulative seconds: [0] waiting for the batch's
+ * sweeps, [1] membership update, [2] sweep launch, [3] retirement, [4]
+ * admission; counts: [0] steps, [1] admitted, [2] retired, [3] batch
+ * membership updates. */
+int tbs_server_stats(tbs_server *srv, double *seconds, int nseconds, long *counts, int ncounts);
 
 /* ---------------------------------------------------------------------- */
 /* Synthetic inputs (the reference reads files only; these generate the    */

@@ -109,7 +109,57 @@ def test_server_admits_by_footprint(tbs, ctx):
         if len(done) == 6:
             break
     assert len(done) == 6 and all(d["status"] == 1 for d in done.values())
-    assert peak == 2
+    assert peak >= 2
+    st = srv.stats()
+    assert st["admitted"] == 6 and st["retired"] == 6
     admits = [done[r]["t_admit"] for r in rids]
     assert admits == sorted(admits)
     srv.close()
+
+
+def test_plan_footprint_is_exact(tbs, ctx):
+    for NL, NK, prec in ((7, 300, 0), (40, 64, 1), (3, 1, 0)):
+        m = tbs.structured_mesh(NL, NK)
+        p = tbs.Plan(ctx, m, prec)
+        assert p.info()["device_bytes"] == tbs.Plan.footprint(m, prec)
+        p.close()
+
+
+def test_batch_membership_changes_in_place(tbs, ctx):
+    """Systems leave and join a running batch between sweeps; every system's
+    history and U stay bitwise those of its solo solve."""
+    print("case batch set_plans", flush=True)
+    specs = [(5, 200, 41), (12, 96, 42), (2, 700, 43), (30, 64, 44)]
+    systems = []
+    for NL, NK, seed in specs:
+        m = tbs.structured_mesh(NL, NK)
+        j = tbs.generate_host(seed, 9, m.nelem, m.nintface, 12.0)
+        p = tbs.Plan(ctx, m, tbs.TBS_FP32)
+        p.factor(dev(j["jacD"]), dev(j["jacO1"]), dev(j["jacO2"]))
+        rhs = dev(j["rhs"])
+        systems.append((m, j, p, rhs))
+    for m, j, p, rhs in systems[:2]:
+        p.begin(rhs, None)
+    b = tbs.Batch(ctx, [systems[0][2], systems[1][2]])
+    b.launch_sweeps(3, 60, 1e-10)
+    for m, j, p, rhs in systems[2:]:
+        p.begin(rhs, None)
+    b.set_plans([systems[1][2], systems[2][2], systems[3][2], systems[0][2]])
+    b.launch_sweeps(2, 60, 1e-10)
+    b.set_plans([])
+    b.launch_sweeps(2, 60, 1e-10)      # no systems: nothing runs
+    b.set_plans([s[2] for s in systems])
+    for _ in range(40):
+        b.launch_sweeps(2, 60, 1e-10)
+        if not b.states()[:, 1].any():
+            break
+    assert not b.states()[:, 1].any()
+    for m, j, p, rhs in systems:
+        U = zeros(9 * m.nelem)
+        p.end(U)
+        h = p.history()
+        h0, n0, U0 = _solve_alone(tbs, ctx, m, j, tbs.TBS_FP32, 60, 1e-10)
+        assert np.array_equal(h, h0) and np.array_equal(host(U), U0)
+    b.close()
+    for s_ in systems:
+        s_[2].close()

@@ -100,6 +100,7 @@ SIGNATURES = {
     "tbs_plan_internal": [P, I, PP, C.POINTER(C.c_size_t)],
     "tbs_batch_create": [P, P, I, PP],
     "tbs_batch_destroy": [P],
+    "tbs_batch_set_plans": [P, P, I],
     "tbs_batch_autotune": [P, I, C.POINTER(C.c_int), P, I],
     "tbs_batch_solve": [P, P, P, I, D, I, P, P],
     "tbs_batch_launch_sweeps": [P, I, I, D],
@@ -113,6 +114,7 @@ SIGNATURES = {
     "tbs_server_result": [P, C.c_long, P, C.c_size_t, P, I, C.POINTER(C.c_int), C.POINTER(C.c_int), P],
     "tbs_server_release": [P, C.c_long],
     "tbs_server_info": [P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), C.POINTER(C.c_double)],
+    "tbs_server_stats": [P, P, I, P, I],
     "tbs_structured_sizes": [I, I, P, P, P, P, P],
     "tbs_structured_mesh": [I, I, P, P, P, P, P, P, P],
     "tbs_ragged_sizes": [I, P, P, P, P],

@@ -30,6 +30,30 @@ int use_ctx(tbs_ctx *ctx) {
     return TBS_OK;
 }
 
+int dev_alloc(tbs_ctx *ctx, size_t bytes, void **ptr, bool *from_region) {
+    *ptr = nullptr;
+    if (bytes == 0) bytes = 1;
+    if (ctx->region) {
+        const size_t off = (ctx->region_used + kRegionAlign - 1) / kRegionAlign * kRegionAlign;
+        TBS_REQUIRE(off + bytes <= ctx->region_size, TBS_ERR_NOMEM,
+                    "region exhausted: %zu + %zu bytes > %zu", off, bytes, ctx->region_size);
+        *ptr = ctx->region + off;
+        ctx->region_used = off + bytes;
+        if (from_region) *from_region = true;
+        return TBS_OK;
+    }
+    if (from_region) *from_region = false;
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e == hipErrorOutOfMemory) {
+        hipGetLastError();
+        *ptr = nullptr;
+        set_error("device allocation of %zu bytes refused", bytes);
+        return TBS_ERR_NOMEM;
+    }
+    TBS_HIP(e);
+    return TBS_OK;
+}
+
 }  // namespace tbs
 
 extern "C" {

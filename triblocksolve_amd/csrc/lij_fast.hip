@@ -34,10 +34,12 @@
 #include "lane_ops.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <climits>
 #include <cstring>
 #include <functional>
+#include <type_traits>
 #include <map>
 #include <memory>
 
@@ -491,6 +493,12 @@ __global__ void reset_state_kernel(const SysDev<TF> *__restrict__ sys, int nsys)
     }
 }
 
+template <typename TF>
+__global__ void gather_states_kernel(const SysDev<TF> *__restrict__ sys, int nsys, int *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 4 * nsys) out[i] = sys[i >> 2].state[i & 3];
+}
+
 // ------------------------------------------------------------ packing --
 // Residual blocks into the SoA-64 layout (tbs::Soa64<BS, double>): src code
 // per (internal element, slot): x = array (0 none, 1 jacD, 2 jacO1, 3 jacO2),
@@ -527,69 +535,201 @@ __global__ void pack_blocks_kernel(long total, int S, const int2 *__restrict__ s
     }
 }
 
-// Sweep-form factors from the LU factors D' (reference layout) and DinvC:
-// thread per (internal element m, column c):  W(:,c) = solveLU(D', e_c),
-// E(:,c) = solveLU(D', A(:,c)) for k >= 1, Gamma(:,c) = DinvC(:,c) for
-// k <= n-2.  solveLU is src/okl/linesmoothLU_TriBlock_v3.okl:23-45.
+// ------------------------------------------------------------- factor --
+// 1/x from the hardware reciprocal estimate refined by two Newton steps
+// (each doubles the correct bits), for the Gauss-Jordan pivots.
+__device__ __forceinline__ double recip(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// Block-Thomas factorisation of every line straight into the sweep-form
+// factors.  src/okl/lineLU_v2.okl:58-182 runs the same recurrence on LU
+// factors; with A_k, C_k the blocks at face f_k = lineface[m0+k] between
+// elements k-1 and k (A = Of2 and C = Of1 when element k is the face's
+// first element fc[2f], else the reverse, lineLU_v2.okl:116-117):
+//   S_0 = D_0,   S_k = D_k - A_k Gamma_{k-1}               (:147-162)
+//   [Gamma_k | E_k | W_k] = S_k^{-1} [C_{k+1} | A_k | I]
+// by Gauss-Jordan elimination without pivoting (the reference's LU does not
+// pivot either).  Gamma_k is the reference's DinvC, W_k = D'_k^{-1} and
+// E_k = D'_k^{-1} A_k are what the sweep kernels read.
+//
+// One wave per line, one lane per column: lanes P..P+NV-1 hold S_k (the
+// pivot columns), X..X+NV-1 hold C_{k+1} and end with Gamma_k, 2NV.. hold A_k
+// then E_k, 3NV.. hold I then W_k.  P and X swap every step, so the lanes
+// that end a step holding Gamma_k form S_{k+1} in place; each pivot step
+// broadcasts one column with readlane.  The D, A, C blocks of 4 steps at a
+// time are prefetched into double-buffered LDS one chunk ahead, from the
+// per-element source table fidx = {e, A source, C source of the face before}
+// (source = 2 * face + 1 for Of2, 2 * face for Of1, -1 none).
 template <int NV, typename TF>
-__global__ void invform_kernel(int nle, const int *__restrict__ lines,
-                               const int *__restrict__ mline, const int *__restrict__ linepoint,
-                               const int *__restrict__ linesize, const int *__restrict__ woff,
-                               const double *__restrict__ Dp, const double *__restrict__ A,
-                               const double *__restrict__ DinvC, TF *__restrict__ W,
-                               TF *__restrict__ EF, TF *__restrict__ GB) {
+__global__ __launch_bounds__(64) void ffactor_kernel(int nline, const int *__restrict__ linepoint,
+                                                     const int *__restrict__ linesize,
+                                                     const int *__restrict__ woff,
+                                                     const int4 *__restrict__ fidx,
+                                                     const double *__restrict__ jacD,
+                                                     const double *__restrict__ O1,
+                                                     const double *__restrict__ O2, TF *__restrict__ W,
+                                                     TF *__restrict__ EF, TF *__restrict__ GB) {
     constexpr int BS = NV * NV;
-    constexpr int SPW = 64 / NV;
-    constexpr int LW = SPW * NV;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)nle * NV) return;
-    const int m = (int)(t / NV), c = (int)(t % NV);
-    const int e = lines[m];
-    const int l = mline[m];
-    const int k = m - linepoint[l];
-    const int n = linesize[l];
-    const double *LU = Dp + (size_t)BS * e;
-    double Lr[BS];
+    constexpr int CH = 4;                       // steps per chunk
+    constexpr int SB = 3 * BS;                  // doubles per step: D_k, A_k, C_{k+1}
+    constexpr int NLD = (CH * SB + 63) / 64;    // loads per lane per chunk
+    constexpr int SPW = 64 / NV, LW = SPW * NV;
+    using WL = tbs::WinLayout<NV, LW, TF>;
+    using WS = tbs::Soa64<BS, TF>;
+    static_assert(4 * NV <= 64, "lane roles need 4*NV lanes");
+    __shared__ double sblk[2][CH * SB];
+    __shared__ int4 sidx[2][CH + 1];
+
+    const int l = blockIdx.x;
+    if (l >= nline) return;
+    const int lane = threadIdx.x;
+    const int m0 = linepoint[l], n = linesize[l];
+    const long wbase = (long)woff[l] * WL::STRIDE;
+    const int nch = (n + CH - 1) / CH;
+
+    auto load_idx = [&](int c) -> int4 {
+        const int k = c * CH + lane;
+        int4 x = make_int4(0, -1, -1, 0);
+        if (lane <= CH && k < n) x = fidx[m0 + k];
+        return x;
+    };
+    auto load_blocks = [&](int c, const int4 *ix, double (&R)[NLD]) {
 #pragma unroll
-    for (int q = 0; q < BS; ++q) Lr[q] = LU[q];
-    auto solve = [&](const double (&b)[NV], double (&x)[NV]) {
-        double y[NV];
-#pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            double tot = 0.0;
-#pragma unroll
-            for (int j = 0; j < i; ++j) tot += Lr[i + NV * j] * y[j];
-            y[i] = b[i] - tot;
-        }
-#pragma unroll
-        for (int i = NV - 1; i >= 0; --i) {
-            double tot = 0.0;
-#pragma unroll
-            for (int j = i + 1; j < NV; ++j) tot += Lr[i + NV * j] * x[j];
-            x[i] = (y[i] - tot) / Lr[i + NV * i];
+        for (int r = 0; r < NLD; ++r) {
+            const int t = lane + 64 * r;
+            R[r] = 0.0;
+            if (t < CH * SB) {
+                const int st = t / SB, rem = t - st * SB;
+                const int which = rem / BS, q = rem - which * BS;
+                const int k = c * CH + st;
+                if (k < n) {
+                    const double *src = nullptr;
+                    if (which == 0) {
+                        src = jacD + (size_t)BS * ix[st].x;
+                    } else {
+                        const int code = which == 1 ? ix[st].y : (k + 1 < n ? ix[st + 1].z : -1);
+                        if (code >= 0) src = ((code & 1) ? O2 : O1) + (size_t)BS * (code >> 1);
+                    }
+                    if (src) R[r] = src[q];
+                }
+            }
         }
     };
-    double b[NV], x[NV];
+    auto store_blocks = [&](int b, const double (&R)[NLD]) {
 #pragma unroll
-    for (int i = 0; i < NV; ++i) b[i] = (i == c) ? 1.0 : 0.0;
-    solve(b, x);
-    const int g = m / kGroup, lane = m % kGroup;
-#pragma unroll
-    for (int i = 0; i < NV; ++i) W[(size_t)g * BS * kGroup + tbs::Soa64<BS, TF>::index(i + NV * c, lane)] = (TF)x[i];
-    // row i of step k sits in lane st*NV + i of window w, entry j = column c
-    using WL = tbs::WinLayout<NV, LW, TF>;
-    const int w = k / SPW, st = k % SPW;
-    const size_t wb = (size_t)(woff[l] + w) * WL::STRIDE;
-    if (k >= 1) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) b[i] = A[(size_t)BS * e + i + NV * c];
-        solve(b, x);
-#pragma unroll
-        for (int i = 0; i < NV; ++i) EF[wb + WL::index(c, st * NV + i)] = (TF)x[i];
+        for (int r = 0; r < NLD; ++r) {
+            const int t = lane + 64 * r;
+            if (t < CH * SB) sblk[b][t] = R[r];
+        }
+    };
+
+    double R[NLD];
+    // prologue: chunk 0 in LDS, chunk 1 in flight, indices of chunk 2 in flight
+    {
+        int4 x = load_idx(0);
+        if (lane <= CH) sidx[0][lane] = x;
+        __syncthreads();
+        load_blocks(0, sidx[0], R);
+        store_blocks(0, R);
+        x = load_idx(1);
+        if (lane <= CH) sidx[1][lane] = x;
+        __syncthreads();
+        load_blocks(1, sidx[1], R);
     }
-    if (k <= n - 2) {
+    int4 xnext = load_idx(2);
+
+    double col[NV];
 #pragma unroll
-        for (int i = 0; i < NV; ++i) GB[wb + WL::index(c, st * NV + i)] = (TF)DinvC[(size_t)BS * e + i + NV * c];
+    for (int i = 0; i < NV; ++i) col[i] = 0.0;
+
+    auto step = [&](auto par, int b, int s, int k) {
+        constexpr int P = decltype(par)::value ? NV : 0;
+        constexpr int X = decltype(par)::value ? 0 : NV;
+        const double *D = &sblk[b][s * SB];
+        const double *A = D + BS;
+        const double *C = D + 2 * BS;
+        if (lane >= P && lane < P + NV) {
+            const int j = lane - P;
+            if (k == 0) {
+#pragma unroll
+                for (int i = 0; i < NV; ++i) col[i] = D[i + NV * j];
+            } else {
+                double nc[NV];
+#pragma unroll
+                for (int i = 0; i < NV; ++i) {
+                    double tot = 0.0;
+#pragma unroll
+                    for (int m = 0; m < NV; ++m) tot += A[i + NV * m] * col[m];
+                    nc[i] = D[i + NV * j] - tot;
+                }
+#pragma unroll
+                for (int i = 0; i < NV; ++i) col[i] = nc[i];
+            }
+        } else if (lane >= X && lane < X + NV) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) col[i] = C[i + NV * (lane - X)];
+        } else if (lane >= 2 * NV && lane < 3 * NV) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) col[i] = A[i + NV * (lane - 2 * NV)];
+        } else {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) col[i] = (lane - 3 * NV == i) ? 1.0 : 0.0;
+        }
+        // Gauss-Jordan on the pivot columns
+#pragma unroll
+        for (int p = 0; p < NV; ++p) {
+            double cp[NV];
+#pragma unroll
+            for (int i = 0; i < NV; ++i) cp[i] = tbs::rdlane(col[i], P + p);
+            const double rp = recip(cp[p]);
+            const double x = col[p] * rp;
+#pragma unroll
+            for (int i = 0; i < NV; ++i)
+                if (i != p) col[i] -= cp[i] * x;
+            col[p] = x;
+        }
+        // outputs
+        const int w = k / SPW, st = k - w * SPW;
+        const long wb = wbase + (long)w * WL::STRIDE;
+        if (lane >= X && lane < X + NV) {
+            if (k <= n - 2) {
+#pragma unroll
+                for (int i = 0; i < NV; ++i) GB[wb + WL::index(lane - X, st * NV + i)] = (TF)col[i];
+            }
+        } else if (lane >= 2 * NV && lane < 3 * NV) {
+            if (k >= 1) {
+#pragma unroll
+                for (int i = 0; i < NV; ++i) EF[wb + WL::index(lane - 2 * NV, st * NV + i)] = (TF)col[i];
+            }
+        } else if (lane >= 3 * NV && lane < 4 * NV) {
+            const int mm = m0 + k, j = lane - 3 * NV;
+            TF *Wg = W + (size_t)(mm / kGroup) * BS * kGroup;
+#pragma unroll
+            for (int i = 0; i < NV; ++i) Wg[WS::index(i + NV * j, mm % kGroup)] = (TF)col[i];
+        }
+    };
+
+    for (int c = 0; c < nch; ++c) {
+        const int b = c & 1;
+        for (int s = 0; s < CH; s += 2) {
+            const int k = c * CH + s;
+            if (k >= n) break;
+            step(std::integral_constant<int, 0>(), b, s, k);
+            if (k + 1 >= n) break;
+            step(std::integral_constant<int, 1>(), b, s + 1, k + 1);
+        }
+        if (c + 1 >= nch) break;
+        __syncthreads();                       // LDS of chunk c read
+        store_blocks(b ^ 1, R);                // chunk c+1
+        if (lane <= CH) sidx[b][lane] = xnext; // indices of chunk c+2
+        __syncthreads();
+        load_blocks(c + 2, sidx[b], R);
+        xnext = load_idx(c + 3);
     }
 }
 
@@ -639,6 +779,7 @@ struct tbs_plan {
     int *d_woff = nullptr;
     int *d_fc = nullptr, *d_lineface = nullptr;  // for lineLU / assemble
     int2 *d_src = nullptr;                       // packing source codes
+    int4 *d_fidx = nullptr;                      // factor block sources per element
     // device: data
     double *d_blk = nullptr;
     int *d_nbr = nullptr;
@@ -657,7 +798,10 @@ struct tbs_plan {
     double rtol = 0.0;
     int variant = 0;          // Thomas prefetch variant (autotuned)
     size_t bytes = 0;
-    int *h_state = nullptr;  // pinned
+    int *h_state = nullptr;      // pinned unless region-backed
+    bool in_region = false;      // device arrays carved from a context region
+    std::vector<void *> allocs;  // hipMalloc'd arrays (not region-backed)
+    std::vector<char> h_tables;  // host image of the packed tables (source of an async upload)
 };
 
 struct tbs_batch {
@@ -669,9 +813,13 @@ struct tbs_batch {
     int2 *d_gtab = nullptr;
     int nl = 0, ng = 0;
     std::map<std::pair<int, std::pair<int, double>>, hipGraphExec_t> graphs;
-    int *h_state = nullptr;
-    int *h_states = nullptr;    // pinned [nplans][4]
+    int *h_states = nullptr;    // pinned [cap_sys][4]
+    int *d_states = nullptr;    // [cap_sys][4] gathered system states
     int variant = 0;
+    // capacities of the device tables (entries) and of the pinned staging
+    size_t cap_sys = 0, cap_l = 0, cap_g = 0, cap_stage = 0;
+    char *h_stage = nullptr;    // pinned image of the tables being uploaded
+    hipEvent_t staged = nullptr; // the last upload from h_stage has completed
 };
 
 namespace {
@@ -772,20 +920,67 @@ Layout layout_of(const tbs_mesh_host *m) {
     return L;
 }
 
-size_t footprint_of(const tbs_mesh_host *m, const Layout &L, int prec) {
+// Packed integer tables of a plan (one device array): byte offsets, each
+// 16-byte aligned, and the total.
+struct TableLayout {
+    size_t lines, mline, linepoint, linesize, woff, fc, lineface, src, fidx, nbr, slotc, ltab, gtab, sys, total;
+};
+
+TableLayout table_layout(const tbs_mesh_host *m, const Layout &L, int prec) {
+    TableLayout t{};
+    size_t o = 0;
+    auto take = [&o](size_t bytes) { const size_t at = o; o += (bytes + 15) / 16 * 16; return at; };
+    const size_t G = (size_t)L.ngroups * kGroup, Fm = (size_t)std::max(L.F, 1);
+    const size_t nle = m->nlineelem, nl = m->nline;
+    t.lines = take(4 * nle);
+    t.mline = take(4 * nle);
+    t.linepoint = take(4 * (nl + 1));
+    t.linesize = take(4 * nl);
+    t.woff = take(4 * (nl + 1));
+    t.fc = take(4 * std::max<size_t>(2 * (size_t)m->nintface, 1));
+    t.lineface = take(4 * nle);
+    t.src = take(sizeof(int2) * G * L.S);
+    t.fidx = take(sizeof(int4) * nle);
+    t.nbr = take(4 * G * Fm);
+    t.slotc = take(4 * (size_t)L.ngroups * Fm);
+    t.ltab = take(sizeof(int4) * nl);
+    t.gtab = take(sizeof(int2) * L.ngroups);
+    t.sys = take(prec == TBS_FP32 ? sizeof(SysDev<float>) : sizeof(SysDev<double>));
+    t.total = o;
+    return t;
+}
+
+// Device arrays of a plan, in allocation order.
+enum { A_BLK, A_W, A_EF, A_GB, A_RHS, A_U, A_R, A_G, A_DUP, A_GPART, A_HIST, A_STATE, A_TABLES, A_COUNT };
+
+std::array<size_t, A_COUNT> array_sizes(const tbs_mesh_host *m, const Layout &L, int prec) {
     const size_t nv = m->nvar, bs = nv * nv, tf = tf_size(prec);
-    const size_t G = (size_t)L.ngroups * kGroup;
+    const size_t G = (size_t)L.ngroups * kGroup, nle = m->nlineelem;
+    std::array<size_t, A_COUNT> a{};
+    a[A_BLK] = G * L.S * bs * 8;
+    a[A_W] = G * bs * tf;
+    a[A_EF] = a[A_GB] = (size_t)L.nwin * win_stride((int)nv, prec) * tf;
+    a[A_RHS] = a[A_U] = a[A_R] = (nle + 2 * (size_t)kPad) * nv * 8;
+    a[A_G] = a[A_DUP] = nle * nv * tf;
+    a[A_GPART] = (size_t)L.ngroups * 8;
+    a[A_HIST] = (size_t)kHistCap * 8;
+    a[A_STATE] = 4 * sizeof(int);
+    a[A_TABLES] = table_layout(m, L, prec).total;
+    return a;
+}
+
+size_t aligned_bytes(size_t b) { return (std::max<size_t>(b, 1) + tbs::kRegionAlign - 1) / tbs::kRegionAlign * tbs::kRegionAlign; }
+
+// Device bytes a plan holds (each array rounded to the region alignment).
+size_t footprint_of(const tbs_mesh_host *m, const Layout &L, int prec) {
     size_t b = 0;
-    b += G * L.S * bs * 8;                // residual blocks
-    b += G * L.F * 4;                     // neighbour table
-    b += G * bs * tf;                     // W
-    b += 2 * (size_t)L.nwin * win_stride((int)nv, prec) * tf;  // E, Gamma windows
-    b += (size_t)m->nlineelem * nv * (8 * 3 + 2 * tf);  // rhs U R g dUp
-    b += (size_t)L.ngroups * 8 + kHistCap * 8 + 64;
-    b += (size_t)m->nlineelem * 4 * 2 + (size_t)(m->nline + 1) * 4 * 3;
-    b += (size_t)m->nline * 16 + (size_t)L.ngroups * 8;  // tables
+    for (size_t x : array_sizes(m, L, prec)) b += aligned_bytes(x);
     return b;
 }
+
+// Bytes of scratch the factorisation needs beyond the plan (none: the fused
+// factor kernel writes the sweep-form factors directly).
+size_t factor_workspace_bytes(int, int) { return 0; }
 
 // Algorithmic bytes of one sweep, BASELINE model: factor-side D', A, DinvC
 // (tf bytes each), D + Of1 + Of2 at FP64, 6 vector passes at FP64.
@@ -810,33 +1005,24 @@ double reference_model_bytes(const tbs_mesh_host *m) {
     return du + cp + lr;
 }
 
-template <typename T>
-int dalloc(tbs_plan *p, T **ptr, size_t count) {
-    *ptr = nullptr;
-    if (count == 0) count = 1;
-    hipError_t e = hipMalloc((void **)ptr, count * sizeof(T));
-    if (e == hipErrorOutOfMemory) {
-        hipGetLastError();
-        tbs::set_error("plan allocation of %zu bytes refused", count * sizeof(T));
-        return TBS_ERR_NOMEM;
-    }
-    TBS_HIP(e);
-    p->bytes += count * sizeof(T);
+int plan_alloc(tbs_plan *p, size_t bytes, void **ptr) {
+    bool reg = false;
+    TBS_TRY(tbs::dev_alloc(p->ctx, bytes, ptr, &reg));
+    if (reg) p->in_region = true;
+    else p->allocs.push_back(*ptr);
+    p->bytes += aligned_bytes(bytes);
     return TBS_OK;
 }
-
 
 void free_plan(tbs_plan *p) {
     if (!p) return;
     hipSetDevice(p->ctx->device);
     for (auto &kv : p->graphs) hipGraphExecDestroy(kv.second);
-    void *ptrs[] = {p->d_lines, p->d_mline, p->d_linepoint, p->d_linesize, p->d_woff, p->d_fc,
-                    p->d_lineface, p->d_src, p->d_blk, p->d_nbr, p->d_slotc, p->d_W, p->d_EF, p->d_GB,
-                    p->a_rhs, p->a_U, p->a_R, p->d_gpart, p->d_hist, p->d_g, p->d_dUp,
-                    p->d_state, p->d_sys, p->d_ltab, p->d_gtab};
-    for (void *q : ptrs)
-        if (q) hipFree(q);
-    if (p->h_state) hipHostFree(p->h_state);
+    for (void *q : p->allocs) hipFree(q);
+    if (p->h_state) {
+        if (p->in_region) delete[] p->h_state;
+        else hipHostFree(p->h_state);
+    }
     delete p;
 }
 
@@ -984,6 +1170,22 @@ int capture_sweeps(hipStream_t st, int nsweeps, hipGraphExec_t *out,
 
 }  // namespace
 
+namespace tbs {
+
+int plan_scatter_U(tbs_plan *p, hipStream_t st, double *U) {
+    const long total = (long)p->nle * p->nv;
+    hipLaunchKernelGGL(scatter_vec_kernel, dim3(gs_grid(total)), dim3(256), 0, st, total, p->nv, p->d_lines,
+                       p->d_U, U);
+    TBS_LAUNCHED("scatter U");
+    return TBS_OK;
+}
+
+const double *plan_hist(const tbs_plan *p) { return p->d_hist; }
+
+size_t plan_factor_workspace(int nvar, int nelem) { return factor_workspace_bytes(nvar, nelem); }
+
+}  // namespace tbs
+
 extern "C" {
 
 int tbs_plan_supported_nvar(int *out, int cap) {
@@ -1011,7 +1213,7 @@ int tbs_plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pla
     TBS_REQUIRE(m->nvar == ctx->nvar, TBS_ERR_ARG, "mesh nvar %d != context nvar %d", m->nvar, ctx->nvar);
     TBS_REQUIRE(plan_nvar_ok(m->nvar), TBS_ERR_UNSUPPORTED, "nvar %d not built for the fused path", m->nvar);
     TBS_REQUIRE(precision == TBS_FP64 || precision == TBS_FP32, TBS_ERR_ARG, "bad precision");
-    const int nv = m->nvar, BS = nv * nv;
+    const int nv = m->nvar;
     Layout L = layout_of(m);
     std::unique_ptr<tbs_plan, void (*)(tbs_plan *)> p(new tbs_plan(), free_plan);
     p->ctx = ctx;
@@ -1029,7 +1231,6 @@ int tbs_plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pla
     p->h_linepoint.assign(m->linepoint, m->linepoint + m->nline + 1);
     p->h_woff = L.woff;
     const size_t G = (size_t)L.ngroups * kGroup;
-    const size_t tf = tf_size(precision);
 
     // host tables
     std::vector<int> mline(p->nle), inv(p->nelem, -1);
@@ -1067,67 +1268,89 @@ int tbs_plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pla
         }
     std::vector<int4> ltab(p->nline);
     for (int l = 0; l < p->nline; ++l) ltab[l] = make_int4(0, m->linepoint[l], m->linesize[l], L.woff[l]);
+    // factor sources: A_k = Of2, C_k = Of1 at face f_k when element k is the face's
+    // first element, else the reverse (src/okl/lineLU_v2.okl:116-117)
+    std::vector<int4> fidx(p->nle);
+    for (int l = 0; l < p->nline; ++l)
+        for (int k = 0; k < m->linesize[l]; ++k) {
+            const int mm = m->linepoint[l] + k, e = m->lines[mm];
+            int a = -1, c = -1;
+            if (k >= 1) {
+                const int f = m->lineface[mm];
+                const bool first = e == m->fc[2 * f];
+                a = 2 * f + (first ? 1 : 0);
+                c = 2 * f + (first ? 0 : 1);
+            }
+            fidx[mm] = make_int4(e, a, c, 0);
+        }
     std::vector<int2> gtab(L.ngroups);
     for (int g = 0; g < L.ngroups; ++g) gtab[g] = make_int2(0, g);
 
     tbs_plan *P = p.get();
-    TBS_TRY(dalloc(P, &P->d_lines, P->nle));
-    TBS_TRY(dalloc(P, &P->d_mline, P->nle));
-    TBS_TRY(dalloc(P, &P->d_linepoint, P->nline + 1));
-    TBS_TRY(dalloc(P, &P->d_linesize, P->nline));
-    TBS_TRY(dalloc(P, &P->d_woff, P->nline + 1));
-    TBS_TRY(dalloc(P, &P->d_fc, std::max(2 * m->nintface, 1)));
-    TBS_TRY(dalloc(P, &P->d_lineface, P->nle));
-    TBS_TRY(dalloc(P, &P->d_src, src.size()));
-    TBS_TRY(dalloc(P, &P->d_blk, G * L.S * BS));
-    TBS_TRY(dalloc(P, &P->d_nbr, nbr.size()));
-    TBS_TRY(dalloc(P, &P->d_slotc, slotc.size()));
-    TBS_TRY(dalloc(P, (char **)&P->d_W, G * BS * tf));
-    TBS_TRY(dalloc(P, (char **)&P->d_EF, (size_t)L.nwin * win_stride(nv, precision) * tf));
-    TBS_TRY(dalloc(P, (char **)&P->d_GB, (size_t)L.nwin * win_stride(nv, precision) * tf));
-    const size_t padded = (size_t)(P->nle + 2 * kPad) * nv;
-    TBS_TRY(dalloc(P, &P->a_rhs, padded));
-    TBS_TRY(dalloc(P, &P->a_U, padded));
-    TBS_TRY(dalloc(P, &P->a_R, padded));
+    const std::array<size_t, A_COUNT> sz = array_sizes(m, L, precision);
+    void *arr[A_COUNT];
+    for (int i = 0; i < A_COUNT; ++i) TBS_TRY(plan_alloc(P, sz[i], &arr[i]));
+    P->d_blk = (double *)arr[A_BLK];
+    P->d_W = arr[A_W];
+    P->d_EF = arr[A_EF];
+    P->d_GB = arr[A_GB];
+    P->a_rhs = (double *)arr[A_RHS];
+    P->a_U = (double *)arr[A_U];
+    P->a_R = (double *)arr[A_R];
     P->d_rhs = P->a_rhs + (size_t)kPad * nv;
     P->d_U = P->a_U + (size_t)kPad * nv;
     P->d_R = P->a_R + (size_t)kPad * nv;
-    for (double *a : {P->a_rhs, P->a_U, P->a_R}) TBS_HIP(hipMemsetAsync(a, 0, padded * sizeof(double), ctx->stream));
-    TBS_TRY(dalloc(P, (char **)&P->d_g, (size_t)P->nle * nv * tf));
-    TBS_TRY(dalloc(P, (char **)&P->d_dUp, (size_t)P->nle * nv * tf));
-    TBS_TRY(dalloc(P, &P->d_gpart, L.ngroups));
-    TBS_TRY(dalloc(P, &P->d_hist, kHistCap));
-    TBS_TRY(dalloc(P, &P->d_state, 4));
-    TBS_TRY(dalloc(P, &P->d_ltab, ltab.size()));
-    TBS_TRY(dalloc(P, &P->d_gtab, gtab.size()));
-    TBS_TRY(dalloc(P, (char **)&P->d_sys, precision == TBS_FP32 ? sizeof(SysDev<float>) : sizeof(SysDev<double>)));
-    TBS_HIP(hipHostMalloc((void **)&P->h_state, 4 * sizeof(int), hipHostMallocDefault));
+    P->d_g = arr[A_G];
+    P->d_dUp = arr[A_DUP];
+    P->d_gpart = (double *)arr[A_GPART];
+    P->d_hist = (double *)arr[A_HIST];
+    P->d_state = (int *)arr[A_STATE];
+    char *tb = (char *)arr[A_TABLES];
+    const TableLayout T = table_layout(m, L, precision);
+    P->d_lines = (int *)(tb + T.lines);
+    P->d_mline = (int *)(tb + T.mline);
+    P->d_linepoint = (int *)(tb + T.linepoint);
+    P->d_linesize = (int *)(tb + T.linesize);
+    P->d_woff = (int *)(tb + T.woff);
+    P->d_fc = (int *)(tb + T.fc);
+    P->d_lineface = (int *)(tb + T.lineface);
+    P->d_src = (int2 *)(tb + T.src);
+    P->d_fidx = (int4 *)(tb + T.fidx);
+    P->d_nbr = (int *)(tb + T.nbr);
+    P->d_slotc = (int *)(tb + T.slotc);
+    P->d_ltab = (int4 *)(tb + T.ltab);
+    P->d_gtab = (int2 *)(tb + T.gtab);
+    P->d_sys = tb + T.sys;
+    if (P->in_region) P->h_state = new int[4]();
+    else TBS_HIP(hipHostMalloc((void **)&P->h_state, 4 * sizeof(int), hipHostMallocDefault));
 
-    hipStream_t st = ctx->stream;
-    auto up = [&](void *dst, const void *s, size_t bytes) -> int {
-        TBS_HIP(hipMemcpyAsync(dst, s, bytes, hipMemcpyHostToDevice, st));
-        return TBS_OK;
-    };
-    TBS_TRY(up(P->d_lines, m->lines, P->nle * 4));
-    TBS_TRY(up(P->d_mline, mline.data(), P->nle * 4));
-    TBS_TRY(up(P->d_linepoint, m->linepoint, (P->nline + 1) * 4));
-    TBS_TRY(up(P->d_linesize, m->linesize, P->nline * 4));
-    TBS_TRY(up(P->d_woff, L.woff.data(), (P->nline + 1) * 4));
-    if (m->nintface) TBS_TRY(up(P->d_fc, m->fc, (size_t)2 * m->nintface * 4));
-    TBS_TRY(up(P->d_lineface, m->lineface, P->nle * 4));
-    TBS_TRY(up(P->d_src, src.data(), src.size() * sizeof(int2)));
-    TBS_TRY(up(P->d_nbr, nbr.data(), nbr.size() * 4));
-    TBS_TRY(up(P->d_slotc, slotc.data(), slotc.size() * 4));
-    TBS_TRY(up(P->d_ltab, ltab.data(), ltab.size() * sizeof(int4)));
-    TBS_TRY(up(P->d_gtab, gtab.data(), gtab.size() * sizeof(int2)));
+    // host image of the tables, kept by the plan so the upload stays asynchronous
+    P->h_tables.assign(T.total, 0);
+    char *h = P->h_tables.data();
+    auto put = [h](size_t off, const void *src_, size_t bytes) { if (bytes) std::memcpy(h + off, src_, bytes); };
+    put(T.lines, m->lines, (size_t)P->nle * 4);
+    put(T.mline, mline.data(), (size_t)P->nle * 4);
+    put(T.linepoint, m->linepoint, (size_t)(P->nline + 1) * 4);
+    put(T.linesize, m->linesize, (size_t)P->nline * 4);
+    put(T.woff, L.woff.data(), (size_t)(P->nline + 1) * 4);
+    put(T.fc, m->fc, (size_t)2 * m->nintface * 4);
+    put(T.lineface, m->lineface, (size_t)P->nle * 4);
+    put(T.src, src.data(), src.size() * sizeof(int2));
+    put(T.fidx, fidx.data(), fidx.size() * sizeof(int4));
+    put(T.nbr, nbr.data(), nbr.size() * 4);
+    put(T.slotc, slotc.data(), slotc.size() * 4);
+    put(T.ltab, ltab.data(), ltab.size() * sizeof(int4));
+    put(T.gtab, gtab.data(), gtab.size() * sizeof(int2));
     if (precision == TBS_FP32) {
-        SysDev<float> s = make_sysdev<float>(P);
-        TBS_TRY(up(P->d_sys, &s, sizeof(s)));
+        SysDev<float> sd = make_sysdev<float>(P);
+        put(T.sys, &sd, sizeof(sd));
     } else {
-        SysDev<double> s = make_sysdev<double>(P);
-        TBS_TRY(up(P->d_sys, &s, sizeof(s)));
+        SysDev<double> sd = make_sysdev<double>(P);
+        put(T.sys, &sd, sizeof(sd));
     }
-    TBS_HIP(hipStreamSynchronize(st));   // host vectors above go out of scope
+    hipStream_t st = ctx->stream;
+    TBS_HIP(hipMemcpyAsync(tb, h, T.total, hipMemcpyHostToDevice, st));
+    for (double *a : {P->a_rhs, P->a_U, P->a_R}) TBS_HIP(hipMemsetAsync(a, 0, sz[A_RHS], st));
     *out = p.release();
     return TBS_OK;
 }
@@ -1135,7 +1358,9 @@ int tbs_plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pla
 int tbs_plan_destroy(tbs_plan *plan) {
     if (!plan) return TBS_OK;
     hipSetDevice(plan->ctx->device);
-    hipStreamSynchronize(plan->ctx->stream);
+    // a region-backed plan frees no device memory; its owner retires it only
+    // after the plan's work has completed
+    if (!plan->in_region) hipStreamSynchronize(plan->ctx->stream);
     free_plan(plan);
     return TBS_OK;
 }
@@ -1187,56 +1412,24 @@ int tbs_plan_factor(tbs_plan *p, const double *jacD, const double *jacO1, const 
         }
         TBS_LAUNCHED("pack_blocks_kernel");
     }
-    // LU factors D' and DinvC (reference semantics), and the packed A
-    double *Dp = nullptr, *DinvC = nullptr, *A = nullptr;
-    auto cleanup = [&]() {
-        if (Dp) hipFree(Dp);
-        if (DinvC) hipFree(DinvC);
-        if (A) hipFree(A);
-    };
-    const size_t nb = (size_t)p->nelem * BS * sizeof(double);
-    hipError_t e1 = hipMalloc((void **)&Dp, nb);
-    hipError_t e2 = hipMalloc((void **)&DinvC, nb);
-    hipError_t e3 = hipMalloc((void **)&A, nb);
-    if (e1 || e2 || e3) {
-        cleanup();
-        hipGetLastError();
-        tbs::set_error("factor workspace allocation (%zu bytes) refused", 3 * nb);
-        return TBS_ERR_NOMEM;
-    }
-    int rc = TBS_OK;
-    do {
-        if ((rc = tbs_copyAtoBjac(ctx, p->nelem, jacD, Dp))) break;
-        if (hipMemsetAsync(DinvC, 0, nb, st) || hipMemsetAsync(A, 0, nb, st)) { rc = TBS_ERR_STATE; break; }
-        if (p->nintface > 0) {
-            if ((rc = tbs_assembleTriBlocks(ctx, p->nline, p->d_linesize, p->d_linepoint, p->d_lines,
-                                            p->d_lineface, p->d_fc, jacO1, jacO2, A))) break;
-        }
-        if ((rc = tbs_lineLU(ctx, p->nelem, p->nintface, p->nline, p->d_fc, p->d_linesize,
-                             p->d_linepoint, p->d_lines, p->d_lineface, Dp, jacO1, jacO2, DinvC))) break;
-        if (hipMemsetAsync(p->d_EF, 0, (size_t)p->nwin * win_stride(nv, p->prec) * tf, st) ||
-            hipMemsetAsync(p->d_GB, 0, (size_t)p->nwin * win_stride(nv, p->prec) * tf, st) ||
-            hipMemsetAsync(p->d_W, 0, G * BS * tf, st)) { rc = TBS_ERR_STATE; break; }
-        const long threads = (long)p->nle * nv;
-        switch (nv) {
+    // sweep-form factors W, E, Gamma straight from the blocks; padding stays zero
+    TBS_HIP(hipMemsetAsync(p->d_EF, 0, (size_t)p->nwin * win_stride(nv, p->prec) * tf, st));
+    TBS_HIP(hipMemsetAsync(p->d_GB, 0, (size_t)p->nwin * win_stride(nv, p->prec) * tf, st));
+    TBS_HIP(hipMemsetAsync(p->d_W, 0, G * BS * tf, st));
+    switch (nv) {
 #define X(v) case v: \
-            if (p->prec == TBS_FP32) hipLaunchKernelGGL((invform_kernel<v, float>), dim3((threads + 255) / 256), dim3(256), 0, st, \
-                p->nle, p->d_lines, p->d_mline, p->d_linepoint, p->d_linesize, p->d_woff, Dp, A, DinvC, \
-                (float *)p->d_W, (float *)p->d_EF, (float *)p->d_GB); \
-            else hipLaunchKernelGGL((invform_kernel<v, double>), dim3((threads + 255) / 256), dim3(256), 0, st, \
-                p->nle, p->d_lines, p->d_mline, p->d_linepoint, p->d_linesize, p->d_woff, Dp, A, DinvC, \
-                (double *)p->d_W, (double *)p->d_EF, (double *)p->d_GB); break;
-            TBS_PLAN_NVARS(X)
+        if (p->prec == TBS_FP32) hipLaunchKernelGGL((ffactor_kernel<v, float>), dim3(p->nline), dim3(64), 0, st, \
+            p->nline, p->d_linepoint, p->d_linesize, p->d_woff, p->d_fidx, jacD, jacO1, jacO2, \
+            (float *)p->d_W, (float *)p->d_EF, (float *)p->d_GB); \
+        else hipLaunchKernelGGL((ffactor_kernel<v, double>), dim3(p->nline), dim3(64), 0, st, \
+            p->nline, p->d_linepoint, p->d_linesize, p->d_woff, p->d_fidx, jacD, jacO1, jacO2, \
+            (double *)p->d_W, (double *)p->d_EF, (double *)p->d_GB); break;
+        TBS_PLAN_NVARS(X)
 #undef X
-        }
-        hipError_t le = hipGetLastError();
-        if (le) { rc = tbs::hip_fail(le, "invform_kernel", __FILE__, __LINE__); break; }
-        hipError_t se = hipStreamSynchronize(st);
-        if (se) { rc = tbs::hip_fail(se, "factor sync", __FILE__, __LINE__); break; }
-    } while (0);
-    cleanup();
-    if (rc == TBS_OK) p->factored = true;
-    return rc;
+    }
+    TBS_LAUNCHED("ffactor_kernel");
+    p->factored = true;
+    return TBS_OK;
 }
 
 int tbs_plan_begin(tbs_plan *p, const double *rhs, const double *U0) {
@@ -1473,53 +1666,99 @@ int tbs_plan_solve(tbs_plan *p, const double *rhs, double *U, int max_sweeps, do
 // ------------------------------------------------------------ batch --
 int tbs_batch_create(tbs_ctx *ctx, tbs_plan *const *plans, int nplans, tbs_batch **out) {
     TBS_TRY(tbs::use_ctx(ctx));
-    TBS_REQUIRE(out && plans && nplans >= 1, TBS_ERR_ARG, "bad batch arguments");
+    TBS_REQUIRE(out && plans && nplans >= 1 && plans[0], TBS_ERR_ARG, "bad batch arguments");
     *out = nullptr;
-    const int nv = plans[0]->nv, prec = plans[0]->prec;
+    std::unique_ptr<tbs_batch, int (*)(tbs_batch *)> b(new tbs_batch(), tbs_batch_destroy);
+    b->ctx = ctx;
+    b->nv = plans[0]->nv;
+    b->prec = plans[0]->prec;
+    TBS_HIP(hipEventCreateWithFlags(&b->staged, hipEventDisableTiming));
+    TBS_TRY(tbs_batch_set_plans(b.get(), plans, nplans));
+    *out = b.release();
+    return TBS_OK;
+}
+
+int tbs_batch_set_plans(tbs_batch *b, tbs_plan *const *plans, int nplans) {
+    TBS_REQUIRE(b && nplans >= 0 && (plans || nplans == 0), TBS_ERR_ARG, "bad batch arguments");
+    TBS_TRY(tbs::use_ctx(b->ctx));
     for (int i = 0; i < nplans; ++i) {
-        TBS_REQUIRE(plans[i] && plans[i]->ctx->device == ctx->device, TBS_ERR_ARG,
+        TBS_REQUIRE(plans[i] && plans[i]->ctx->device == b->ctx->device, TBS_ERR_ARG,
                     "plan %d is on another device", i);
-        TBS_REQUIRE(plans[i]->nv == nv && plans[i]->prec == prec, TBS_ERR_ARG,
+        TBS_REQUIRE(plans[i]->nv == b->nv && plans[i]->prec == b->prec, TBS_ERR_ARG,
                     "plans in a batch must share nvar and precision");
         TBS_REQUIRE(plans[i]->factored, TBS_ERR_STATE, "plan %d not factored", i);
     }
-    std::unique_ptr<tbs_batch> b(new tbs_batch());
-    b->ctx = ctx;
-    b->plans.assign(plans, plans + nplans);
-    b->nv = nv;
-    b->prec = prec;
-    std::vector<int4> ltab;
-    std::vector<int2> gtab;
-    // lines longest-first so the longest chains start first
+    hipStream_t st = b->ctx->stream;
+    if (!b->graphs.empty()) {
+        TBS_HIP(hipStreamSynchronize(st));
+        for (auto &kv : b->graphs) hipGraphExecDestroy(kv.second);
+        b->graphs.clear();
+    }
+    // tables: lines longest-first so the longest chains start first
     std::vector<std::pair<int, int4>> lines;
-    for (int s = 0; s < nplans; ++s) {
-        const tbs_plan *p = plans[s];
+    std::vector<int2> gtab;
+    for (int s2 = 0; s2 < nplans; ++s2) {
+        const tbs_plan *p = plans[s2];
         for (int l = 0; l < p->nline; ++l)
-            lines.push_back({p->h_linesize[l], make_int4(s, p->h_linepoint[l], p->h_linesize[l], p->h_woff[l])});
-        for (int g = 0; g < p->ngroups; ++g) gtab.push_back(make_int2(s, g));
+            lines.push_back({p->h_linesize[l], make_int4(s2, p->h_linepoint[l], p->h_linesize[l], p->h_woff[l])});
+        for (int g = 0; g < p->ngroups; ++g) gtab.push_back(make_int2(s2, g));
     }
     std::stable_sort(lines.begin(), lines.end(),
-                     [](const std::pair<int, int4> &a, const std::pair<int, int4> &c) { return a.first > c.first; });
-    for (auto &x : lines) ltab.push_back(x.second);
-    b->nl = (int)ltab.size();
-    b->ng = (int)gtab.size();
-    const size_t ssz = prec == TBS_FP32 ? sizeof(SysDev<float>) : sizeof(SysDev<double>);
-    std::vector<char> sysh(ssz * nplans);
-    for (int s = 0; s < nplans; ++s) {
-        if (prec == TBS_FP32) { SysDev<float> d = make_sysdev<float>(plans[s]); std::memcpy(&sysh[ssz * s], &d, ssz); }
-        else { SysDev<double> d = make_sysdev<double>(plans[s]); std::memcpy(&sysh[ssz * s], &d, ssz); }
+                     [](const std::pair<int, int4> &x, const std::pair<int, int4> &y) { return x.first > y.first; });
+    const size_t ssz = b->prec == TBS_FP32 ? sizeof(SysDev<float>) : sizeof(SysDev<double>);
+    const size_t nl = lines.size(), ng = gtab.size(), ns = (size_t)nplans;
+    // grow the device tables (after the work that reads them has finished)
+    if (ns > b->cap_sys || nl > b->cap_l || ng > b->cap_g) {
+        TBS_HIP(hipStreamSynchronize(st));
+        if (ns > b->cap_sys) {
+            const size_t c = std::max<size_t>(2 * ns, 16);
+            if (b->d_sys) TBS_HIP(hipFree(b->d_sys));
+            if (b->d_states) TBS_HIP(hipFree(b->d_states));
+            if (b->h_states) TBS_HIP(hipHostFree(b->h_states));
+            b->d_sys = nullptr; b->d_states = nullptr; b->h_states = nullptr; b->cap_sys = 0;
+            TBS_HIP(hipMalloc(&b->d_sys, c * ssz));
+            TBS_HIP(hipMalloc((void **)&b->d_states, c * 4 * sizeof(int)));
+            TBS_HIP(hipHostMalloc((void **)&b->h_states, c * 4 * sizeof(int), hipHostMallocDefault));
+            b->cap_sys = c;
+        }
+        if (nl > b->cap_l) {
+            const size_t c = std::max<size_t>(2 * nl, 256);
+            if (b->d_ltab) TBS_HIP(hipFree(b->d_ltab));
+            b->d_ltab = nullptr; b->cap_l = 0;
+            TBS_HIP(hipMalloc((void **)&b->d_ltab, c * sizeof(int4)));
+            b->cap_l = c;
+        }
+        if (ng > b->cap_g) {
+            const size_t c = std::max<size_t>(2 * ng, 256);
+            if (b->d_gtab) TBS_HIP(hipFree(b->d_gtab));
+            b->d_gtab = nullptr; b->cap_g = 0;
+            TBS_HIP(hipMalloc((void **)&b->d_gtab, c * sizeof(int2)));
+            b->cap_g = c;
+        }
     }
-    auto fail = [&](hipError_t e) { return tbs::hip_fail(e, "batch alloc", __FILE__, __LINE__); };
-    hipError_t e;
-    if ((e = hipMalloc(&b->d_sys, sysh.size()))) return fail(e);
-    if ((e = hipMalloc((void **)&b->d_ltab, ltab.size() * sizeof(int4)))) return fail(e);
-    if ((e = hipMalloc((void **)&b->d_gtab, gtab.size() * sizeof(int2)))) return fail(e);
-    if ((e = hipHostMalloc((void **)&b->h_state, 4 * sizeof(int), hipHostMallocDefault))) return fail(e);
-    if ((e = hipHostMalloc((void **)&b->h_states, 4 * sizeof(int) * nplans, hipHostMallocDefault))) return fail(e);
-    TBS_HIP(hipMemcpy(b->d_sys, sysh.data(), sysh.size(), hipMemcpyHostToDevice));
-    TBS_HIP(hipMemcpy(b->d_ltab, ltab.data(), ltab.size() * sizeof(int4), hipMemcpyHostToDevice));
-    TBS_HIP(hipMemcpy(b->d_gtab, gtab.data(), gtab.size() * sizeof(int2), hipMemcpyHostToDevice));
-    *out = b.release();
+    // stage through pinned memory; the previous upload must have left it
+    const size_t o_l = (ns * ssz + 15) / 16 * 16, o_g = o_l + nl * sizeof(int4), total = o_g + ng * sizeof(int2);
+    TBS_HIP(hipEventSynchronize(b->staged));
+    if (total > b->cap_stage) {
+        if (b->h_stage) TBS_HIP(hipHostFree(b->h_stage));
+        b->h_stage = nullptr; b->cap_stage = 0;
+        const size_t c = std::max<size_t>(2 * total, 4096);
+        TBS_HIP(hipHostMalloc((void **)&b->h_stage, c, hipHostMallocDefault));
+        b->cap_stage = c;
+    }
+    for (size_t s2 = 0; s2 < ns; ++s2) {
+        if (b->prec == TBS_FP32) { SysDev<float> d = make_sysdev<float>(plans[s2]); std::memcpy(b->h_stage + ssz * s2, &d, ssz); }
+        else { SysDev<double> d = make_sysdev<double>(plans[s2]); std::memcpy(b->h_stage + ssz * s2, &d, ssz); }
+    }
+    for (size_t i = 0; i < nl; ++i) std::memcpy(b->h_stage + o_l + i * sizeof(int4), &lines[i].second, sizeof(int4));
+    if (ng) std::memcpy(b->h_stage + o_g, gtab.data(), ng * sizeof(int2));
+    if (ns) TBS_HIP(hipMemcpyAsync(b->d_sys, b->h_stage, ns * ssz, hipMemcpyHostToDevice, st));
+    if (nl) TBS_HIP(hipMemcpyAsync(b->d_ltab, b->h_stage + o_l, nl * sizeof(int4), hipMemcpyHostToDevice, st));
+    if (ng) TBS_HIP(hipMemcpyAsync(b->d_gtab, b->h_stage + o_g, ng * sizeof(int2), hipMemcpyHostToDevice, st));
+    TBS_HIP(hipEventRecord(b->staged, st));
+    b->plans.assign(plans, plans + nplans);
+    b->nl = (int)nl;
+    b->ng = (int)ng;
     return TBS_OK;
 }
 
@@ -1549,6 +1788,7 @@ int tbs_batch_autotune(tbs_batch *b, int reps, int *chosen, double *us, int cap)
 int tbs_batch_launch_sweeps(tbs_batch *b, int nsweeps, int max_sweeps, double rtol) {
     TBS_REQUIRE(b && nsweeps >= 0 && max_sweeps >= 0 && max_sweeps < kHistCap, TBS_ERR_ARG, "bad arguments");
     TBS_TRY(tbs::use_ctx(b->ctx));
+    if (b->plans.empty()) return TBS_OK;
     hipStream_t st = b->ctx->stream;
     for (int i = 0; i < nsweeps; ++i)
         TBS_TRY(enqueue_sweep(b->nv, b->prec, st, b->d_sys, (int)b->plans.size(), b->d_ltab, b->nl, b->d_gtab,
@@ -1557,13 +1797,19 @@ int tbs_batch_launch_sweeps(tbs_batch *b, int nsweeps, int max_sweeps, double rt
 }
 
 int tbs_batch_states(tbs_batch *b, int *states) {
-    TBS_REQUIRE(b && states, TBS_ERR_ARG, "null argument");
+    TBS_REQUIRE(b && (states || b->plans.empty()), TBS_ERR_ARG, "null argument");
     TBS_TRY(tbs::use_ctx(b->ctx));
-    hipStream_t st = b->ctx->stream;
     const int ns = (int)b->plans.size();
-    for (int s2 = 0; s2 < ns; ++s2)
-        TBS_HIP(hipMemcpyAsync(&b->h_states[4 * s2], b->plans[s2]->d_state, 4 * sizeof(int),
-                               hipMemcpyDeviceToHost, st));
+    if (ns == 0) return TBS_OK;
+    hipStream_t st = b->ctx->stream;
+    if (b->prec == TBS_FP32)
+        hipLaunchKernelGGL((gather_states_kernel<float>), dim3((4 * ns + 255) / 256), dim3(256), 0, st,
+                           (const SysDev<float> *)b->d_sys, ns, b->d_states);
+    else
+        hipLaunchKernelGGL((gather_states_kernel<double>), dim3((4 * ns + 255) / 256), dim3(256), 0, st,
+                           (const SysDev<double> *)b->d_sys, ns, b->d_states);
+    TBS_LAUNCHED("gather_states_kernel");
+    TBS_HIP(hipMemcpyAsync(b->h_states, b->d_states, 4 * sizeof(int) * ns, hipMemcpyDeviceToHost, st));
     TBS_HIP(hipStreamSynchronize(st));
     std::memcpy(states, b->h_states, 4 * sizeof(int) * ns);
     return TBS_OK;
@@ -1577,8 +1823,10 @@ int tbs_batch_destroy(tbs_batch *b) {
     if (b->d_sys) hipFree(b->d_sys);
     if (b->d_ltab) hipFree(b->d_ltab);
     if (b->d_gtab) hipFree(b->d_gtab);
-    if (b->h_state) hipHostFree(b->h_state);
+    if (b->d_states) hipFree(b->d_states);
     if (b->h_states) hipHostFree(b->h_states);
+    if (b->h_stage) hipHostFree(b->h_stage);
+    if (b->staged) hipEventDestroy(b->staged);
     delete b;
     return TBS_OK;
 }
@@ -1621,11 +1869,7 @@ int tbs_batch_solve(tbs_batch *b, const double *const *rhs, double *const *U, in
         TBS_HIP(hipGraphLaunch(it->second, st));
         done += sweeps_per_check;
         bool any = false;
-        for (int s = 0; s < ns; ++s) {
-            TBS_HIP(hipMemcpyAsync(&st_h[4 * s], b->plans[s]->d_state, 4 * sizeof(int),
-                                   hipMemcpyDeviceToHost, st));
-        }
-        TBS_HIP(hipStreamSynchronize(st));
+        TBS_TRY(tbs_batch_states(b, st_h.data()));
         for (int s = 0; s < ns; ++s) any |= st_h[4 * s + 1] != 0;
         if (!any) break;
     }

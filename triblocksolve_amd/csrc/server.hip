@@ -1,16 +1,20 @@
 // server.hip -- continuous-batching LIJ solve service on one GPU.
 //
 // Requests (seeded structured systems, FP32-factor or FP64 plans) wait in a
-// FIFO queue.  Each engine step
-//   1. enqueues K sweeps of the current batch on the main stream,
-//   2. admits queued requests whose HBM footprint fits the budget: plan
-//      tables, device generation of the Jacobian, factorisation and the
-//      initial residual run on a second (admission) stream, overlapping the
-//      batch's sweeps on the device,
-//   3. waits for the batch's sweeps and reads every system's state,
-//   4. retires finished systems (U copied to host, HBM returned), and
-//   5. moves admitted requests whose admission work has completed into the
-//      batch, rebuilding only the descriptor tables.
+// FIFO queue.  The server owns an HBM arena, grown in segments of at least
+// kSegment bytes up to the budget; every admitted request gets one slab of
+// it (first fit), from which its plan, its rhs/U and its admission
+// temporaries are carved, so steady-state serving makes no hipMalloc/hipFree
+// calls.  Each engine step
+//   1. reads the states of the running batch (waits for its last sweeps),
+//   2. drops finished systems from the batch and adds admitted ones whose
+//      set-up has completed (descriptor tables rewritten in place),
+//   3. enqueues K sweeps of the batch,
+//   4. while those run, retires the finished systems on a third stream (U
+//      scattered to element order and copied to the host, slab returned), and
+//   5. admits queued requests that fit: mesh tables, Jacobian generation,
+//      factorisation and the initial residual are enqueued on the admission
+//      stream and overlap the batch's sweeps on the device.
 // A request's arithmetic depends on its own data alone (per-line Thomas,
 // per-element residual, per-system fixed-order norm), so its result is
 // bitwise the same whatever it shares the batch with.
@@ -28,34 +32,68 @@ namespace {
 
 using clk = std::chrono::steady_clock;
 
+constexpr size_t kAlign = tbs::kRegionAlign;
+constexpr size_t kSlack = 16 * kAlign;   // bump-alignment reserve per request
+// a request behind the queue head may be admitted first while the head has
+// been passed over fewer than this many times
+constexpr int kMaxSkips = 16;
+constexpr size_t kSegment = (size_t)16 << 30;   // arena growth step
+
+size_t align_up(size_t b) { return (std::max<size_t>(b, 1) + kAlign - 1) / kAlign * kAlign; }
+
+// First-fit allocator over the ranges released into it, coalescing
+// adjacent free ranges.
+class Arena {
+  public:
+    bool alloc(size_t n, size_t *off) {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            *off = it->first;
+            const size_t rest = it->second - n, at = it->first + n;
+            free_.erase(it);
+            if (rest) free_[at] = rest;
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t n) {
+        if (!n) return;
+        auto nx = free_.lower_bound(off);
+        if (nx != free_.end() && off + n == nx->first) {
+            n += nx->second;
+            nx = free_.erase(nx);
+        }
+        if (nx != free_.begin()) {
+            auto pv = std::prev(nx);
+            if (pv->first + pv->second == off) {
+                pv->second += n;
+                return;
+            }
+        }
+        free_[off] = n;
+    }
+
+  private:
+    std::map<size_t, size_t> free_;   // offset -> length
+};
+
 struct Request {
     long id = 0;
     tbs_server_request spec{};
     int nelem = 0, nintface = 0;
-    size_t footprint = 0;
-    // mesh (host; the plan keeps its own device copies)
+    size_t footprint = 0;   // slab bytes while admitting
+    size_t persist = 0;     // slab bytes kept once admitted (plan, rhs, U)
+    size_t slab_off = 0, slab_len = 0;
+    int skipped = 0;
     std::vector<int> epoint, ef, fc, linesize, linepoint, lines, lineface;
     tbs_plan *plan = nullptr;
-    double *d_rhs = nullptr, *d_U = nullptr;
+    double *d_rhs = nullptr, *d_U = nullptr;   // element order, in the slab
     hipEvent_t ready = nullptr;
-    // results
     std::vector<double> U, hist;
     int nsweeps = 0, status = 0;
     double t_submit = 0, t_admit = 0, t_done = 0;
-    bool done = false;
 };
 
-size_t request_footprint(const tbs_mesh_host &mh, int prec, int nvar) {
-    size_t plan = 0;
-    if (tbs_plan_footprint(&mh, prec, &plan) != TBS_OK) return (size_t)-1;
-    const size_t bs = (size_t)nvar * nvar * 8;
-    const size_t jac = bs * ((size_t)mh.nelem + 2 * (size_t)mh.nintface);   // generated D, O1, O2
-    const size_t work = 3 * bs * (size_t)mh.nelem;                          // factor workspace
-    const size_t vec = 2 * (size_t)nvar * mh.nelem * 8;                     // rhs, U
-    return plan + jac + work + vec;
-}
-
-// Structured NL x NK mesh of a request, in the request's host arrays.
 int build_mesh(Request *r) {
     const int NL = r->spec.NL, NK = r->spec.NK;
     int nif = 0, n = 0, eft = 0, nl = 0, nle = 0;
@@ -91,86 +129,132 @@ tbs_mesh_host mesh_view(const Request &r, int nvar) {
     return mh;
 }
 
+// Slab sizes: persistent part (plan arrays, rhs, U) and the whole slab
+// (plus generated D/O1/O2 and the factorisation workspace).
+int request_sizes(const Request &r, int prec, int nvar, size_t *persist, size_t *total) {
+    const tbs_mesh_host mh = mesh_view(r, nvar);
+    size_t plan = 0;
+    TBS_TRY(tbs_plan_footprint(&mh, prec, &plan));
+    const size_t bs = (size_t)nvar * nvar * 8, vec = (size_t)nvar * r.nelem * 8;
+    *persist = align_up(plan + 2 * align_up(vec) + kSlack);
+    *total = *persist + align_up(bs * r.nelem) + 2 * align_up(bs * std::max(r.nintface, 1)) +
+             tbs::plan_factor_workspace(nvar, r.nelem) + kSlack;
+    return TBS_OK;
+}
+
+enum { T_WAIT, T_MEMBERS, T_LAUNCH, T_RETIRE, T_ADMIT, T_NPHASE };
+enum { C_STEPS, C_ADMITTED, C_RETIRED, C_REBUILDS, C_NCOUNT };
+
 }  // namespace
 
 struct tbs_server {
-    tbs_ctx *ctx = nullptr;       // batch stream
-    tbs_ctx *adm = nullptr;       // admission stream (same device)
+    tbs_ctx *ctx = nullptr;   // batch stream
+    tbs_ctx *adm = nullptr;   // admission stream (same device)
+    tbs_ctx *ret = nullptr;   // retirement stream (same device)
     size_t budget = 0, used = 0;
     int prec = TBS_FP32, max_sweeps = 100, k = 4;
     double rtol = 1e-10;
     long next_id = 1;
     clk::time_point t0;
+    // arena segments; slab offsets are in a virtual space where segment i
+    // starts at seg_off[i] (segments never coalesce with each other)
+    std::vector<char *> seg_base;
+    std::vector<size_t> seg_off, seg_len;
+    size_t arena_bytes = 0;
+    Arena arena;
+    char *addr(size_t off) const {
+        for (size_t i = 0; i < seg_base.size(); ++i)
+            if (off >= seg_off[i] && off < seg_off[i] + seg_len[i]) return seg_base[i] + (off - seg_off[i]);
+        return nullptr;
+    }
     std::deque<std::unique_ptr<Request>> queue;
     std::vector<std::unique_ptr<Request>> admitting, active;
     std::map<long, std::unique_ptr<Request>> finished;
     std::vector<long> completed_unpolled;
     tbs_batch *batch = nullptr;
-    bool dirty = false;
+    double t_phase[T_NPHASE] = {};
+    long count[C_NCOUNT] = {};
     double now() const { return std::chrono::duration<double>(clk::now() - t0).count(); }
 };
 
 namespace {
 
-void free_request_device(Request *r) {
-    if (r->plan) tbs_plan_destroy(r->plan);
+void drop_device(tbs_server *s, Request *r) {
+    if (r->plan) tbs_plan_destroy(r->plan);   // region-backed: frees no device memory
     r->plan = nullptr;
-    if (r->d_rhs) hipFree(r->d_rhs);
-    if (r->d_U) hipFree(r->d_U);
-    r->d_rhs = r->d_U = nullptr;
     if (r->ready) hipEventDestroy(r->ready);
     r->ready = nullptr;
+    if (r->slab_len) s->arena.release(r->slab_off, r->slab_len);
+    r->slab_len = 0;
+    r->d_rhs = r->d_U = nullptr;
 }
 
+// Enqueue a request's set-up on the admission stream, inside its slab.
 int admit(tbs_server *s, Request *r) {
     const tbs_mesh_host mh = mesh_view(*r, s->ctx->nvar);
     const int nv = mh.nvar;
     const size_t bs = (size_t)nv * nv;
-    int rc = tbs_plan_create(s->adm, &mh, s->prec, &r->plan);
-    if (rc) return rc;
+    tbs_ctx *a = s->adm;
+    a->region = s->addr(r->slab_off);
+    a->region_size = r->slab_len;
+    a->region_used = 0;
     double *D = nullptr, *O1 = nullptr, *O2 = nullptr;
-    const size_t nD = bs * r->nelem, nO = bs * (size_t)std::max(r->nintface, 1);
-    hipError_t e = hipMalloc((void **)&D, nD * 8);
-    if (!e) e = hipMalloc((void **)&O1, nO * 8);
-    if (!e) e = hipMalloc((void **)&O2, nO * 8);
-    if (!e) e = hipMalloc((void **)&r->d_rhs, (size_t)nv * r->nelem * 8);
-    if (!e) e = hipMalloc((void **)&r->d_U, (size_t)nv * r->nelem * 8);
-    if (e) {
-        hipGetLastError();
-        if (D) hipFree(D);
-        if (O1) hipFree(O1);
-        if (O2) hipFree(O2);
-        tbs::set_error("admission allocation refused");
-        return TBS_ERR_NOMEM;
-    }
+    int rc = TBS_OK;
     do {
-        if ((rc = tbs_generate_device(s->adm, r->spec.seed, nv, r->nelem, r->nintface, r->spec.shift, D, O1,
-                                      O2, r->d_rhs)))
+        if ((rc = tbs_plan_create(a, &mh, s->prec, &r->plan))) break;
+        if ((rc = tbs::dev_alloc(a, (size_t)nv * r->nelem * 8, (void **)&r->d_rhs, nullptr))) break;
+        if ((rc = tbs::dev_alloc(a, (size_t)nv * r->nelem * 8, (void **)&r->d_U, nullptr))) break;
+        if (a->region_used > r->persist) { rc = TBS_ERR_STATE; tbs::set_error("slab layout overflow"); break; }
+        a->region_used = r->persist;   // temporaries start at the persistent boundary
+        if ((rc = tbs::dev_alloc(a, bs * r->nelem * 8, (void **)&D, nullptr))) break;
+        if ((rc = tbs::dev_alloc(a, bs * std::max(r->nintface, 1) * 8, (void **)&O1, nullptr))) break;
+        if ((rc = tbs::dev_alloc(a, bs * std::max(r->nintface, 1) * 8, (void **)&O2, nullptr))) break;
+        if ((rc = tbs_generate_device(a, r->spec.seed, nv, r->nelem, r->nintface, r->spec.shift, D, O1, O2,
+                                      r->d_rhs)))
             break;
-        if ((rc = tbs_memset(s->adm, r->d_U, 0, (size_t)nv * r->nelem * 8))) break;
-        if ((rc = tbs_plan_factor(r->plan, D, O1, O2))) break;      // synchronises the admission stream
+        if ((rc = tbs_memset(a, r->d_U, 0, (size_t)nv * r->nelem * 8))) break;
+        if ((rc = tbs_plan_factor(r->plan, D, O1, O2))) break;   // workspace from the region: no wait
         if ((rc = tbs_plan_begin(r->plan, r->d_rhs, r->d_U))) break;
         if (hipEventCreateWithFlags(&r->ready, hipEventDisableTiming) ||
-            hipEventRecord(r->ready, (hipStream_t)tbs_ctx_stream(s->adm))) {
+            hipEventRecord(r->ready, a->stream)) {
             rc = TBS_ERR_STATE;
+            tbs::set_error("admission event");
             break;
         }
     } while (0);
-    hipFree(D);
-    hipFree(O1);
-    hipFree(O2);
+    a->region = nullptr;
+    a->region_size = a->region_used = 0;
+    if (rc) hipStreamSynchronize(a->stream);   // nothing of the slab may still be in use
     return rc;
 }
 
-int rebuild_batch(tbs_server *s) {
-    if (s->batch) {
-        tbs_batch_destroy(s->batch);
-        s->batch = nullptr;
+// Add a segment able to hold `need` bytes, within the budget.
+int grow_arena(tbs_server *s, size_t need) {
+    const size_t room = s->budget - std::min(s->budget, s->arena_bytes);
+    size_t len = std::min(std::max(need, kSegment), room);
+    TBS_REQUIRE(len >= need, TBS_ERR_NOMEM, "arena cannot grow by %zu bytes", need);
+    char *base = nullptr;
+    hipError_t e = hipMalloc((void **)&base, len);
+    if (e) {
+        hipGetLastError();
+        tbs::set_error("server arena segment of %zu bytes refused", len);
+        return TBS_ERR_NOMEM;
     }
-    if (s->active.empty()) return TBS_OK;
-    std::vector<tbs_plan *> plans;
-    for (auto &r : s->active) plans.push_back(r->plan);
-    return tbs_batch_create(s->ctx, plans.data(), (int)plans.size(), &s->batch);
+    // a gap of one alignment unit keeps segments from coalescing in the free list
+    const size_t off = s->seg_off.empty() ? 0 : s->seg_off.back() + s->seg_len.back() + kAlign;
+    s->seg_base.push_back(base);
+    s->seg_off.push_back(off);
+    s->seg_len.push_back(len);
+    s->arena_bytes += len;
+    s->arena.release(off, len);
+    return TBS_OK;
+}
+
+void finish(tbs_server *s, std::unique_ptr<Request> r) {
+    r->t_done = s->now();
+    s->completed_unpolled.push_back(r->id);
+    const long id = r->id;
+    s->finished[id] = std::move(r);
 }
 
 }  // namespace
@@ -183,15 +267,16 @@ int tbs_server_create(tbs_ctx *ctx, size_t hbm_budget, int precision, double rto
     TBS_REQUIRE(out && (precision == TBS_FP32 || precision == TBS_FP64) && rtol >= 0 && max_sweeps >= 1 &&
                     max_sweeps < 4096 && sweeps_per_step >= 1, TBS_ERR_ARG, "bad server arguments");
     *out = nullptr;
-    std::unique_ptr<tbs_server> s(new tbs_server());
+    std::unique_ptr<tbs_server, int (*)(tbs_server *)> s(new tbs_server(), tbs_server_destroy);
     s->ctx = ctx;
     TBS_TRY(tbs_ctx_create(ctx->device, ctx->nvar, &s->adm));
+    TBS_TRY(tbs_ctx_create(ctx->device, ctx->nvar, &s->ret));
     if (hbm_budget == 0) {
         size_t fr = 0, tot = 0;
         TBS_HIP(hipMemGetInfo(&fr, &tot));
         hbm_budget = fr / 10 * 8;
     }
-    s->budget = hbm_budget;
+    s->budget = hbm_budget / kAlign * kAlign;
     s->prec = precision;
     s->rtol = rtol;
     s->max_sweeps = max_sweeps;
@@ -206,10 +291,11 @@ int tbs_server_destroy(tbs_server *s) {
     hipSetDevice(s->ctx->device);
     hipDeviceSynchronize();
     if (s->batch) tbs_batch_destroy(s->batch);
-    for (auto &r : s->active) free_request_device(r.get());
-    for (auto &r : s->admitting) free_request_device(r.get());
-    for (auto &kv : s->finished) free_request_device(kv.second.get());
-    tbs_ctx_destroy(s->adm);
+    for (auto &r : s->active) drop_device(s, r.get());
+    for (auto &r : s->admitting) drop_device(s, r.get());
+    for (char *b : s->seg_base) hipFree(b);
+    if (s->ret) tbs_ctx_destroy(s->ret);
+    if (s->adm) tbs_ctx_destroy(s->adm);
     delete s;
     return TBS_OK;
 }
@@ -219,8 +305,8 @@ int tbs_server_footprint(tbs_server *s, const tbs_server_request *req, size_t *b
     Request r;
     r.spec = *req;
     TBS_TRY(build_mesh(&r));
-    *bytes = request_footprint(mesh_view(r, s->ctx->nvar), s->prec, s->ctx->nvar);
-    return TBS_OK;
+    size_t persist = 0;
+    return request_sizes(r, s->prec, s->ctx->nvar, &persist, bytes);
 }
 
 int tbs_server_submit(tbs_server *s, const tbs_server_request *req, long *id) {
@@ -228,7 +314,7 @@ int tbs_server_submit(tbs_server *s, const tbs_server_request *req, long *id) {
     std::unique_ptr<Request> r(new Request());
     r->spec = *req;
     TBS_TRY(build_mesh(r.get()));
-    r->footprint = request_footprint(mesh_view(*r, s->ctx->nvar), s->prec, s->ctx->nvar);
+    TBS_TRY(request_sizes(*r, s->prec, s->ctx->nvar, &r->persist, &r->footprint));
     TBS_REQUIRE(r->footprint <= s->budget, TBS_ERR_NOMEM, "request needs %zu bytes, budget is %zu",
                 r->footprint, s->budget);
     r->id = s->next_id++;
@@ -241,73 +327,108 @@ int tbs_server_submit(tbs_server *s, const tbs_server_request *req, long *id) {
 int tbs_server_step(tbs_server *s, int *nactive, int *nqueued, int *nadmitting) {
     TBS_REQUIRE(s, TBS_ERR_ARG, "null server");
     TBS_TRY(tbs::use_ctx(s->ctx));
-    // 1. sweeps of the current batch (asynchronous)
-    if (s->batch) TBS_TRY(tbs_batch_launch_sweeps(s->batch, s->k, s->max_sweeps, s->rtol));
-    // 2. admission, FIFO with skip-ahead for requests that fit
-    for (auto it = s->queue.begin(); it != s->queue.end();) {
-        Request *r = it->get();
-        if (s->used + r->footprint > s->budget) { ++it; continue; }
-        s->used += r->footprint;
-        r->t_admit = s->now();
-        int rc = admit(s, r);
-        if (rc) {
-            s->used -= r->footprint;
-            free_request_device(r);
-            r->status = -1;
-            r->done = true;
-            r->t_done = s->now();
-            s->completed_unpolled.push_back(r->id);
-            s->finished[r->id] = std::move(*it);
-            it = s->queue.erase(it);
-            continue;
-        }
-        s->admitting.push_back(std::move(*it));
-        it = s->queue.erase(it);
-    }
-    // 3. states of the batch
-    if (s->batch) {
-        std::vector<int> st(4 * s->active.size());
-        TBS_TRY(tbs_batch_states(s->batch, st.data()));
-        // 4. retire
-        std::vector<std::unique_ptr<Request>> keep;
-        for (size_t i = 0; i < s->active.size(); ++i) {
-            std::unique_ptr<Request> &r = s->active[i];
-            if (st[4 * i + 1]) { keep.push_back(std::move(r)); continue; }
-            const int nv = s->ctx->nvar;
-            r->status = st[4 * i + 2];
-            r->U.resize((size_t)nv * r->nelem);
-            r->hist.resize(s->max_sweeps + 1);
-            int cnt = 0;
-            TBS_TRY(tbs_plan_end(r->plan, r->d_U, nullptr));
-            TBS_TRY(tbs_plan_history(r->plan, r->hist.data(), (int)r->hist.size(), &cnt));
-            TBS_TRY(tbs_memcpy_d2h(s->adm, r->U.data(), r->d_U, r->U.size() * 8));
-            r->hist.resize(cnt);
-            r->nsweeps = cnt > 0 ? cnt - 1 : 0;
-            r->t_done = s->now();
-            r->done = true;
-            free_request_device(r.get());
-            s->used -= r->footprint;
-            s->completed_unpolled.push_back(r->id);
-            s->finished[r->id] = std::move(r);
-            s->dirty = true;
-        }
-        s->active.swap(keep);
-    }
-    // 5. join admitted requests whose admission work is complete
-    for (auto it = s->admitting.begin(); it != s->admitting.end();) {
-        if (hipEventQuery((*it)->ready) == hipSuccess) {
-            s->active.push_back(std::move(*it));
-            it = s->admitting.erase(it);
-            s->dirty = true;
+    const int nv = s->ctx->nvar;
+    s->count[C_STEPS]++;
+    double tp = s->now();
+    auto lap = [&](int phase) { const double t = s->now(); s->t_phase[phase] += t - tp; tp = t; };
+
+    // 1. states of the running batch (waits for the sweeps of the last step)
+    std::vector<int> st(4 * s->active.size());
+    if (!s->active.empty()) TBS_TRY(tbs_batch_states(s->batch, st.data()));
+    lap(T_WAIT);
+
+    // 2. membership: finished systems leave, completed admissions join
+    std::vector<std::unique_ptr<Request>> keep, done;
+    for (size_t i = 0; i < s->active.size(); ++i) {
+        if (st[4 * i + 1]) {
+            keep.push_back(std::move(s->active[i]));
         } else {
-            ++it;
+            s->active[i]->status = st[4 * i + 2];
+            s->active[i]->nsweeps = std::max(std::min(st[4 * i], 4096) - 1, 0);
+            done.push_back(std::move(s->active[i]));
         }
+    }
+    bool changed = !done.empty();
+    for (auto it = s->admitting.begin(); it != s->admitting.end();) {
+        Request *r = it->get();
+        if (hipEventQuery(r->ready) != hipSuccess) { ++it; continue; }
+        // set-up complete: the temporaries of the slab go back to the arena
+        s->arena.release(r->slab_off + r->persist, r->slab_len - r->persist);
+        s->used -= r->slab_len - r->persist;
+        r->slab_len = r->persist;
+        keep.push_back(std::move(*it));
+        it = s->admitting.erase(it);
+        changed = true;
     }
     hipGetLastError();   // hipEventQuery reports "not ready" as an error code
-    if (s->dirty) {
-        TBS_TRY(rebuild_batch(s));
-        s->dirty = false;
+    s->active.swap(keep);
+    if (changed) {
+        std::vector<tbs_plan *> plans;
+        for (auto &r : s->active) plans.push_back(r->plan);
+        if (s->batch) TBS_TRY(tbs_batch_set_plans(s->batch, plans.data(), (int)plans.size()));
+        else if (!plans.empty()) TBS_TRY(tbs_batch_create(s->ctx, plans.data(), (int)plans.size(), &s->batch));
+        s->count[C_REBUILDS]++;
     }
+    lap(T_MEMBERS);
+
+    // 3. sweeps (asynchronous)
+    if (!s->active.empty()) TBS_TRY(tbs_batch_launch_sweeps(s->batch, s->k, s->max_sweeps, s->rtol));
+    lap(T_LAUNCH);
+
+    // 4. retire on the retirement stream; no queued kernel touches these systems
+    if (!done.empty()) {
+        hipStream_t rs = s->ret->stream;
+        for (auto &r : done) {
+            r->U.resize((size_t)nv * r->nelem);
+            r->hist.resize(r->nsweeps + 1);
+            TBS_TRY(tbs::plan_scatter_U(r->plan, rs, r->d_U));
+            TBS_HIP(hipMemcpyAsync(r->U.data(), r->d_U, r->U.size() * 8, hipMemcpyDeviceToHost, rs));
+            TBS_HIP(hipMemcpyAsync(r->hist.data(), tbs::plan_hist(r->plan), r->hist.size() * 8,
+                                   hipMemcpyDeviceToHost, rs));
+        }
+        TBS_HIP(hipStreamSynchronize(rs));
+        for (auto &r : done) {
+            s->used -= r->slab_len;
+            drop_device(s, r.get());
+            s->count[C_RETIRED]++;
+            finish(s, std::move(r));
+        }
+    }
+    lap(T_RETIRE);
+
+    // 5. admission: FIFO; later requests may pass a head that does not fit
+    //    until the head has been passed over kMaxSkips times
+    for (auto it = s->queue.begin(); it != s->queue.end();) {
+        Request *r = it->get();
+        const bool is_head = it == s->queue.begin();
+        bool fits = s->used + r->footprint <= s->budget;
+        if (fits && !s->arena.alloc(r->footprint, &r->slab_off)) {
+            fits = s->arena_bytes + r->footprint <= s->budget && grow_arena(s, r->footprint) == TBS_OK &&
+                   s->arena.alloc(r->footprint, &r->slab_off);
+        }
+        if (!fits) {
+            if (is_head && ++r->skipped > kMaxSkips) break;
+            ++it;
+            continue;
+        }
+        r->slab_len = r->footprint;
+        s->used += r->footprint;
+        r->t_admit = s->now();
+        const int rc = admit(s, r);
+        std::unique_ptr<Request> own = std::move(*it);
+        it = s->queue.erase(it);
+        if (rc) {
+            s->used -= own->slab_len;
+            drop_device(s, own.get());
+            own->status = -1;
+            finish(s, std::move(own));
+            continue;
+        }
+        s->count[C_ADMITTED]++;
+        s->admitting.push_back(std::move(own));
+    }
+    lap(T_ADMIT);
+
     if (nactive) *nactive = (int)s->active.size();
     if (nqueued) *nqueued = (int)s->queue.size();
     if (nadmitting) *nadmitting = (int)s->admitting.size();
@@ -337,7 +458,7 @@ int tbs_server_result(tbs_server *s, long id, double *U, size_t ucap, double *hi
         TBS_REQUIRE(ucap >= r->U.size(), TBS_ERR_ARG, "U capacity %zu < %zu", ucap, r->U.size());
         std::memcpy(U, r->U.data(), r->U.size() * 8);
     }
-    if (hist) std::memcpy(hist, r->hist.data(), std::min<size_t>(hcap, r->hist.size()) * 8);
+    if (hist && hcap > 0) std::memcpy(hist, r->hist.data(), std::min<size_t>(hcap, r->hist.size()) * 8);
     if (nsweeps) *nsweeps = r->nsweeps;
     if (status) *status = r->status;
     if (times) {
@@ -359,6 +480,13 @@ int tbs_server_info(tbs_server *s, size_t *budget, size_t *used, double *now) {
     if (budget) *budget = s->budget;
     if (used) *used = s->used;
     if (now) *now = s->now();
+    return TBS_OK;
+}
+
+int tbs_server_stats(tbs_server *s, double *seconds, int nseconds, long *counts, int ncounts) {
+    TBS_REQUIRE(s, TBS_ERR_ARG, "null server");
+    for (int i = 0; seconds && i < nseconds && i < T_NPHASE; ++i) seconds[i] = s->t_phase[i];
+    for (int i = 0; counts && i < ncounts && i < C_NCOUNT; ++i) counts[i] = s->count[i];
     return TBS_OK;
 }
 

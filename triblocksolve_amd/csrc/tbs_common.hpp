@@ -20,6 +20,11 @@ struct tbs_ctx {
     double *red_buf = nullptr;
     size_t red_cap = 0;
     double *red_host = nullptr;   // pinned, 1 value
+    // Optional device region that plan allocations are carved from (bump,
+    // 256-byte aligned) instead of hipMalloc; set by the serving engine around
+    // one request's admission.  Region memory is never freed piecewise.
+    char *region = nullptr;
+    size_t region_size = 0, region_used = 0;
 };
 
 namespace tbs {
@@ -52,6 +57,20 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
 
 // Make the context's device current (every entry point does this first).
 int use_ctx(tbs_ctx *ctx);
+
+// Device allocation for library-owned arrays: from the context's region when
+// one is set (*from_region = true), else hipMalloc.  TBS_ERR_NOMEM when
+// neither can hold `bytes`.
+int dev_alloc(tbs_ctx *ctx, size_t bytes, void **ptr, bool *from_region);
+constexpr size_t kRegionAlign = 256;
+
+// Internal access to a plan for the serving engine (same library).
+// Scatter the plan's solution into U (device, element order) on stream st.
+int plan_scatter_U(tbs_plan *p, hipStream_t st, double *U);
+// Device residual history [kHistCap] of a plan.
+const double *plan_hist(const tbs_plan *p);
+// Bytes the plan's factorisation draws from a context region.
+size_t plan_factor_workspace(int nvar, int nelem);
 
 inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
 

@@ -70,7 +70,9 @@ class DeviceBuffer:
         return out
 
     def zero(self):
+        """Zero the buffer; complete on return, so work on any stream sees it."""
         A.check(self.ctx._lib.tbs_memset(self.ctx.handle, C.c_void_p(self.ptr), 0, self.nbytes))
+        A.check(self.ctx._lib.tbs_ctx_synchronize(self.ctx.handle))
 
     def close(self):
         if self.ptr and self.ctx.handle:
@@ -469,6 +471,12 @@ class Batch:
         A.check(self.ctx._lib.tbs_batch_autotune(self.handle, reps, C.byref(ch), _ptr(us), nvariants))
         return ch.value, us
 
+    def set_plans(self, plans):
+        """Replace the batch's systems (tables rewritten in place, stream-ordered)."""
+        self.plans = list(plans)
+        arr = (C.c_void_p * max(len(self.plans), 1))(*[p.handle.value for p in self.plans])
+        A.check(self.ctx._lib.tbs_batch_set_plans(self.handle, arr, len(self.plans)))
+
     def launch_sweeps(self, n, max_sweeps, rtol=0.0):
         """Enqueue n sweeps of every system (each plan begun with Plan.begin)."""
         A.check(self.ctx._lib.tbs_batch_launch_sweeps(self.handle, n, max_sweeps, float(rtol)))
@@ -558,3 +566,11 @@ class Server:
         b, u, t = C.c_size_t(), C.c_size_t(), C.c_double()
         A.check(self.ctx._lib.tbs_server_info(self.handle, C.byref(b), C.byref(u), C.byref(t)))
         return dict(budget=b.value, used=u.value, now=t.value)
+
+    def stats(self):
+        """Cumulative host seconds per engine phase and event counts."""
+        t = np.zeros(5)
+        c = np.zeros(4, np.int64)
+        A.check(self.ctx._lib.tbs_server_stats(self.handle, _ptr(t), 5, _ptr(c), 4))
+        return dict(zip(("wait", "membership", "launch", "retire", "admit"), t.tolist()),
+                    **dict(zip(("steps", "admitted", "retired", "membership_updates"), c.tolist())))
