@@ -65,6 +65,24 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ u32x4 bload16(rsrc_t r, uint32_t voff, uint32_t soff) {
     return __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0);
 }
+// Streaming (non-temporal) variants for data read once per sweep: the cache
+// policy bit nt asks L2 not to keep the lines, so reused data (the solution
+// vector's neighbour tiles) stays resident.
+constexpr int kCacheNT = 2;
+__device__ __forceinline__ u32x4 bload16_nt(rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, kCacheNT);
+}
+template <typename T>
+__device__ __forceinline__ T bload_nt(rsrc_t r, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ double bload_nt<double>(rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, kCacheNT));
+}
+template <>
+__device__ __forceinline__ float bload_nt<float>(rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, kCacheNT));
+}
+
 template <typename T>
 __device__ __forceinline__ T lane_elem(const u32x4 &v, int k);
 template <>
@@ -90,6 +108,18 @@ __device__ __forceinline__ void bstore<double>(rsrc_t r, double v, uint32_t voff
 template <>
 __device__ __forceinline__ void bstore<float>(rsrc_t r, float v, uint32_t voff, uint32_t soff) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)voff, (int)soff, 0);
+}
+
+template <typename T>
+__device__ __forceinline__ void bstore_nt(rsrc_t r, T v, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ void bstore_nt<double>(rsrc_t r, double v, uint32_t voff, uint32_t soff) {
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)voff, (int)soff, kCacheNT);
+}
+template <>
+__device__ __forceinline__ void bstore_nt<float>(rsrc_t r, float v, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)voff, (int)soff, kCacheNT);
 }
 
 // Lane-interleaved block layout ("SoA-64"): a group of 64 elements stores each

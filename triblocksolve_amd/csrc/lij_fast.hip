@@ -83,7 +83,7 @@ __device__ __forceinline__ void slot_matvec(tbs::rsrc_t r, uint32_t slot_soff, i
     using L = tbs::Soa64<NV * NV, T>;
 #pragma unroll
     for (int c = 0; c < L::NC; ++c) {
-        const tbs::u32x4 v = tbs::bload16(r, (uint32_t)lane * 16u, slot_soff + (uint32_t)c * 1024u);
+        const tbs::u32x4 v = tbs::bload16_nt(r, (uint32_t)lane * 16u, slot_soff + (uint32_t)c * 1024u);
 #pragma unroll
         for (int k = 0; k < L::VW; ++k) {
             const int q = c * L::VW + k;
@@ -93,8 +93,8 @@ __device__ __forceinline__ void slot_matvec(tbs::rsrc_t r, uint32_t slot_soff, i
 #pragma unroll
     for (int k = 0; k < L::REM; ++k) {
         const int q = L::NC * L::VW + k;
-        const T b = tbs::bload<T>(r, (uint32_t)lane * sizeof(T),
-                                  slot_soff + (uint32_t)(L::NC * L::VW * 64 + k * 64) * sizeof(T));
+        const T b = tbs::bload_nt<T>(r, (uint32_t)lane * sizeof(T),
+                                     slot_soff + (uint32_t)(L::NC * L::VW * 64 + k * 64) * sizeof(T));
         acc[q % NV] += (double)b * x[q / NV];
     }
 }
@@ -107,21 +107,24 @@ __device__ __forceinline__ void load_vec(tbs::rsrc_t r, uint32_t byte_off, doubl
 
 // Copy the vector tile of elements [m_base, m_base + 64) (AoS, NV doubles per
 // element, contiguous) into this wave's LDS tile with 16-byte loads; elements
-// outside the vector read as 0.  Each lane then reads its own element.
-template <int NV>
+// outside the vector read as 0.  Each lane then reads its own element.  NT:
+// the vector is read once (streaming cache policy).
+template <int NV, bool NT = false>
 __device__ __forceinline__ void stage_tile(tbs::rsrc_t r, int m_base, int lane, double *tile) {
     constexpr int TB = kGroup * NV * 8;          // tile bytes
     constexpr int NC = TB / 1024;                // full 1 KiB wave loads
     const uint32_t base = (uint32_t)(m_base * NV * 8);
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
-        const tbs::u32x4 v = tbs::bload16(r, base + 16u * lane + 1024u * k, 0);
+        const tbs::u32x4 v = NT ? tbs::bload16_nt(r, base + 16u * lane + 1024u * k, 0)
+                                : tbs::bload16(r, base + 16u * lane + 1024u * k, 0);
         tile[2 * lane + 128 * k] = tbs::lane_elem<double>(v, 0);
         tile[2 * lane + 128 * k + 1] = tbs::lane_elem<double>(v, 1);
     }
     static_assert(TB - NC * 1024 == 0 || TB - NC * 1024 == 512, "tile = 512*NV bytes");
     if (TB - NC * 1024 == 512)   // odd NV: one more 8-byte load per lane
-        tile[NC * 128 + lane] = tbs::bload<double>(r, base + (uint32_t)(NC * 1024) + 8u * lane, 0);
+        tile[NC * 128 + lane] = NT ? tbs::bload_nt<double>(r, base + (uint32_t)(NC * 1024) + 8u * lane, 0)
+                                   : tbs::bload<double>(r, base + (uint32_t)(NC * 1024) + 8u * lane, 0);
     __builtin_amdgcn_wave_barrier();
 }
 
@@ -194,7 +197,7 @@ __global__ __launch_bounds__(256) void fres_kernel(const SysDev<TF> *__restrict_
         }
     }
     double rh[NV], r[NV], nr[NV];
-    stage_tile<NV>(rRhs, m0 + kPad, lane, tile);
+    stage_tile<NV, true>(rRhs, m0 + kPad, lane, tile);
     read_tile<NV>(tile, lane, rh);
     double rr = 0.0;
 #pragma unroll
@@ -232,11 +235,13 @@ struct GroupIO {
     static constexpr int LDS = NCH * 64 * VW;            // padded LDS length
 };
 
-template <int V, typename T>
+template <int V, typename T, bool NT = false>
 __device__ __forceinline__ void bulk_load(tbs::rsrc_t r, uint32_t base, int lane,
                                           tbs::u32x4 (&reg)[GroupIO<V, T>::NCH]) {
 #pragma unroll
-    for (int c = 0; c < GroupIO<V, T>::NCH; ++c) reg[c] = tbs::bload16(r, base + (uint32_t)(c * 1024 + lane * 16), 0);
+    for (int c = 0; c < GroupIO<V, T>::NCH; ++c)
+        reg[c] = NT ? tbs::bload16_nt(r, base + (uint32_t)(c * 1024 + lane * 16), 0)
+                    : tbs::bload16(r, base + (uint32_t)(c * 1024 + lane * 16), 0);
 }
 
 template <int V, typename T>
@@ -252,7 +257,7 @@ __device__ __forceinline__ void bulk_commit(const tbs::u32x4 (&reg)[GroupIO<V, T
 
 // Store exactly V values from LDS (16-byte stores for whole chunks, per-value
 // stores for the tail so nothing past the group is overwritten).
-template <int V, typename T>
+template <int V, typename T, bool NT = false>
 __device__ __forceinline__ void bulk_store(tbs::rsrc_t r, uint32_t base, int lane, const T *lds) {
     using IO = GroupIO<V, T>;
     __builtin_amdgcn_wave_barrier();
@@ -272,13 +277,15 @@ __device__ __forceinline__ void bulk_store(tbs::rsrc_t r, uint32_t base, int lan
 #pragma unroll
                 for (int k = 0; k < 4; ++k) v[k] = __builtin_bit_cast(unsigned, (float)lds[first + k]);
             }
-            __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)(base + (uint32_t)(c * 1024 + lane * 16)), 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)(base + (uint32_t)(c * 1024 + lane * 16)), 0,
+                                                   NT ? tbs::kCacheNT : 0);
         } else {
 #pragma unroll
             for (int k = 0; k < IO::VW; ++k) {
                 const int q = first + k;
                 const uint32_t off = q < V ? base + (uint32_t)q * sizeof(T) : tbs::kOOB;
-                tbs::bstore<T>(r, lds[q], off, 0);
+                if (NT) tbs::bstore_nt<T>(r, lds[q], off, 0);
+                else tbs::bstore<T>(r, lds[q], off, 0);
             }
         }
     }
@@ -326,13 +333,13 @@ __global__ __launch_bounds__(64) void fthomas_kernel(const SysDev<TF> *__restric
         const uint32_t b = wbase(w);
 #pragma unroll
         for (int c = 0; c < WL::NC; ++c) {
-            const tbs::u32x4 v = tbs::bload16(r, b + (uint32_t)(c * LW * 16 + lane * 16), 0);
+            const tbs::u32x4 v = tbs::bload16_nt(r, b + (uint32_t)(c * LW * 16 + lane * 16), 0);
 #pragma unroll
             for (int k = 0; k < WL::VW; ++k) e[c * WL::VW + k] = tbs::lane_elem<TF>(v, k);
         }
 #pragma unroll
         for (int k = 0; k < WL::REM; ++k)
-            e[WL::NC * WL::VW + k] = tbs::bload<TF>(r, b + (uint32_t)((WL::NC * WL::VW * LW + k * LW + lane) * sizeof(TF)), 0);
+            e[WL::NC * WL::VW + k] = tbs::bload_nt<TF>(r, b + (uint32_t)((WL::NC * WL::VW * LW + k * LW + lane) * sizeof(TF)), 0);
     };
 
     TF d[NV];
@@ -344,13 +351,13 @@ __global__ __launch_bounds__(64) void fthomas_kernel(const SysDev<TF> *__restric
     for (int p = 0; p < PF; ++p) load_row(rEF, p, eb[p]);
     {
         tbs::u32x4 vr[IOT::NCH];
-        bulk_load<VG, TF>(rg, gbase(0, sizeof(TF)), lane, vr);
+        bulk_load<VG, TF, true>(rg, gbase(0, sizeof(TF)), lane, vr);
         bulk_commit<VG, TF>(vr, lane, s_vin[0]);
     }
     for (int jg = 0; jg < ngrp; ++jg) {
         const int cur = jg & 1;
         tbs::u32x4 vr[IOT::NCH];
-        bulk_load<VG, TF>(rg, gbase(jg + 1, sizeof(TF)), lane, vr);   // next group's g
+        bulk_load<VG, TF, true>(rg, gbase(jg + 1, sizeof(TF)), lane, vr);   // next group's g
 #pragma unroll
         for (int bb = 0; bb < G; ++bb) {
             const int w = jg * G + bb;
@@ -391,7 +398,7 @@ __global__ __launch_bounds__(64) void fthomas_kernel(const SysDev<TF> *__restric
     {
         tbs::u32x4 vr[IOT::NCH];
         tbs::u32x4 ur[IOD::NCH];
-        bulk_load<VG, TF>(rd, gbase(ngrp - 1, sizeof(TF)), lane, vr);
+        bulk_load<VG, TF, true>(rd, gbase(ngrp - 1, sizeof(TF)), lane, vr);
         bulk_load<VG, double>(rU, gbase(ngrp - 1, 8), lane, ur);
         bulk_commit<VG, TF>(vr, lane, s_vin[0]);
         bulk_commit<VG, double>(ur, lane, s_u[0]);
@@ -402,7 +409,7 @@ __global__ __launch_bounds__(64) void fthomas_kernel(const SysDev<TF> *__restric
         tbs::u32x4 vr[IOT::NCH];
         tbs::u32x4 ur[IOD::NCH];
         if (jg > 0) {                                                   // previous group's d', U
-            bulk_load<VG, TF>(rd, gbase(jg - 1, sizeof(TF)), lane, vr);
+            bulk_load<VG, TF, true>(rd, gbase(jg - 1, sizeof(TF)), lane, vr);
             bulk_load<VG, double>(rU, gbase(jg - 1, 8), lane, ur);
         }
 #pragma unroll
