@@ -36,6 +36,9 @@ def parse():
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="only the FP64 timed region")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="before timing, copy jacD once (copy_kernel, 81*8*nelem bytes read and written): "
+                         "the known-size launch that rocprofv3 FETCH_SIZE/WRITE_SIZE counts are scaled by")
     return ap.parse_args()
 
 
@@ -211,6 +214,11 @@ def main():
     D, O1, O2, rhs = z(81 * n), z(81 * nif), z(81 * nif), z(9 * n)
     ctx.generate_device(args.seed, n, nif, args.shift, D, O1, O2, rhs)
     ctx.sync()
+    if args.calibrate_copy:
+        Dc = z(81 * n)
+        ctx.copyAtoBjac(n, D, Dc)
+        ctx.sync()
+        Dc.close()
 
     def timed_sweeps(prec, steps, warmup):
         plan = tbs.Plan(ctx, m, prec)

@@ -542,6 +542,52 @@ __global__ void pack_blocks_kernel(long total, int S, const int2 *__restrict__ s
     }
 }
 
+// Same packing through LDS: one 256-thread workgroup per (group, slot) reads
+// the group's 64 source blocks whole (consecutive threads read consecutive
+// entries of a block, so every load is contiguous), then writes the slot in
+// SoA-64 order (consecutive threads write consecutive addresses).
+template <int NV>
+__global__ __launch_bounds__(256) void pack_lds_kernel(int S, const int2 *__restrict__ src,
+                                                       const double *__restrict__ D,
+                                                       const double *__restrict__ O1,
+                                                       const double *__restrict__ O2,
+                                                       double *__restrict__ blk) {
+    constexpr int BS = NV * NV;
+    using L = tbs::Soa64<BS, double>;
+    __shared__ double s[kGroup * BS];          // [element lane][entry]
+    __shared__ const double *s_src[kGroup];
+    const long w = blockIdx.x;                 // (group, slot)
+    const long g = w / S;
+    const int slot = (int)(w - g * S);
+    const int t = threadIdx.x;
+    if (t < kGroup) {
+        const int2 c = src[(g * kGroup + t) * S + slot];
+        const double *a = c.x == 1 ? D : (c.x == 2 ? O1 : (c.x == 3 ? O2 : nullptr));
+        s_src[t] = a ? a + (size_t)BS * c.y : nullptr;
+    }
+    __syncthreads();
+    for (int i = t; i < kGroup * BS; i += 256) {
+        const int e = i / BS, q = i - e * BS;
+        const double *p = s_src[e];
+        s[i] = p ? p[q] : 0.0;
+    }
+    __syncthreads();
+    double *out = blk + w * L::ELEMS;
+    for (int i = t; i < L::ELEMS; i += 256) {
+        int q, lane;
+        if (i < L::NC * L::VW * 64) {
+            const int c = i / (64 * L::VW), rem = i - c * 64 * L::VW;
+            lane = rem / L::VW;
+            q = c * L::VW + (rem - lane * L::VW);
+        } else {
+            const int r = i - L::NC * L::VW * 64;
+            q = L::NC * L::VW + r / 64;
+            lane = r % 64;
+        }
+        out[i] = s[lane * BS + q];
+    }
+}
+
 // ------------------------------------------------------------- factor --
 // 1/x from the hardware reciprocal estimate refined by two Newton steps
 // (each doubles the correct bits), for the Gauss-Jordan pivots.
@@ -1410,14 +1456,25 @@ int tbs_plan_factor(tbs_plan *p, const double *jacD, const double *jacO1, const 
     const size_t tf = tf_size(p->prec);
     // residual blocks
     {
-        const long total = (long)G * p->S * BS;
-        switch (nv) {
-#define X(v) case v: hipLaunchKernelGGL(pack_blocks_kernel<v>, dim3(gs_grid(total)), dim3(256), 0, st, \
-                   total, p->S, p->d_src, jacD, jacO1, jacO2, p->d_blk); break;
-            TBS_PLAN_NVARS(X)
+        const long nws = (long)p->ngroups * p->S;
+        if (nws < INT_MAX) {
+            switch (nv) {
+#define X(v) case v: hipLaunchKernelGGL(pack_lds_kernel<v>, dim3((unsigned)nws), dim3(256), 0, st, \
+                       p->S, p->d_src, jacD, jacO1, jacO2, p->d_blk); break;
+                TBS_PLAN_NVARS(X)
 #undef X
+            }
+            TBS_LAUNCHED("pack_lds_kernel");
+        } else {
+            const long total = (long)G * p->S * BS;
+            switch (nv) {
+#define X(v) case v: hipLaunchKernelGGL(pack_blocks_kernel<v>, dim3(gs_grid(total)), dim3(256), 0, st, \
+                       total, p->S, p->d_src, jacD, jacO1, jacO2, p->d_blk); break;
+                TBS_PLAN_NVARS(X)
+#undef X
+            }
+            TBS_LAUNCHED("pack_blocks_kernel");
         }
-        TBS_LAUNCHED("pack_blocks_kernel");
     }
     // sweep-form factors W, E, Gamma straight from the blocks; padding stays zero
     TBS_HIP(hipMemsetAsync(p->d_EF, 0, (size_t)p->nwin * win_stride(nv, p->prec) * tf, st));
