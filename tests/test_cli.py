@@ -1,0 +1,113 @@
+"""triblock.exe: the reference's driver and host classes (Platform, Mesh,
+Jacobian; src/main.cxx, src/Mesh.cxx, src/Jacobian.cxx) on the HIP library,
+run end to end and checked against the oracle."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import closed_forms as cf
+from conftest import ROOT, system
+from triblocksolve_amd.solver import HostMesh
+
+EXE = os.path.join(ROOT, "triblocksolve_amd", "triblock.exe")
+
+
+def run(args, cwd, expect_ok=True):
+    print("case triblock.exe", " ".join(args), flush=True)
+    p = subprocess.run([EXE] + args, cwd=cwd, capture_output=True, text=True, timeout=120)
+    if expect_ok:
+        assert p.returncode == 0, p.stdout + p.stderr
+    hist = np.array([float(x) for x in re.findall(r"\|\|R_\d+\|\| = (\S+)", p.stdout)])
+    return p, hist
+
+
+def test_other_compute_modes_are_errors(tmp_path):
+    """Modes other than HIP raise before touching a device (src/main.cxx:38-54 lists them)."""
+    for mode in ("0", "4"):
+        p, _ = run([mode, "0", "9", "--synthetic", "4x4"], tmp_path, expect_ok=False)
+        assert p.returncode == 1 and "compute_mode " + mode in p.stderr
+
+
+def ref_resize_blocks(v, nblk, old, new):
+    """src/Jacobian.cxx:153-238 on one block array: std::vector::resize keeps
+    the flat prefix and zero-fills growth, diagonals become 10, then the
+    leading min(old,new) square of every block is copied at the new stride."""
+    nn, oo, mn = new * new, old * old, min(old, new)
+    out = np.zeros(nn * nblk)
+    k = min(v.size, out.size)
+    out[:k] = v[:k]
+    o = out.reshape(nblk, new, new)            # [block][column j][row i]
+    idx = np.arange(new)
+    o[:, idx, idx] = 10.0
+    o[:, :mn, :mn] = v.reshape(nblk, old, old)[:, :mn, :mn]
+    return out
+
+
+def ref_resize_vec(v, nelem, new, mn):
+    out = np.zeros(new * nelem)
+    out[:nelem * mn] = v[:nelem * mn]
+    return out
+
+
+@pytest.mark.gpu
+def test_synthetic_dropin_history(oracle, tmp_path):
+    _, hist = run(["1", "0", "9", "--synthetic", "60x64", "--shift", "20", "--seed", "1", "--iters", "10"],
+                  tmp_path)
+    m, jac = system(60, 64, 9, seed=1, shift=20.0)
+    ref = oracle.lij_run(m, jac, 10)
+    assert hist.size == 11
+    ok, worst = cf.history_close(hist, ref["hist"])
+    assert ok, worst
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_synthetic_fused(oracle, tmp_path, prec):
+    _, hist = run(["1", "0", "9", "--synthetic", "40x96", "--shift", "12", "--seed", "5", "--iters", "40",
+                   "--fused", "--precision", prec, "--rtol", "1e-10"], tmp_path)
+    assert hist[-1] <= 1e-10 * hist[0] and hist.size <= 12
+    if prec == "fp64":
+        m, jac = system(40, 96, 9, seed=5, shift=12.0)
+        ref = oracle.lij_run(m, jac, hist.size - 1)
+        assert cf.history_close(hist, ref["hist"])[0]
+
+
+@pytest.mark.gpu
+def test_reads_reference_files_from_cwd(tbs, oracle, tmp_path):
+    m, jac = system(12, 40, 9, seed=7, shift=12.0)
+    tbs.write_mesh(tmp_path / "gpuline.mesh.data.bin", m)
+    tbs.write_jacobian(tmp_path / "gpuline.jacobian.data.bin", 9, m.nelem, m.nintface, jac)
+    _, hist = run(["1", "0", "9", "--iters", "8"], tmp_path)
+    ref = oracle.lij_run(m, jac, 8)
+    assert cf.history_close(hist, ref["hist"])[0]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("new", [5, 11])
+def test_resize_block_size(tbs, oracle, tmp_path, new):
+    """Block size different from the file's nvar: the resized system is the
+    reference's (bit-exact), and the sweep history on it matches the oracle.
+    Growing keeps the old flat data in the new blocks' extra rows and
+    columns (the reference's resize semantics); on this data that system's
+    block LU breaks down and both runs give the same non-finite history."""
+    m, jac = system(5, 8, 9, seed=3, shift=20.0)
+    tbs.write_mesh(tmp_path / "gpuline.mesh.data.bin", m)
+    tbs.write_jacobian(tmp_path / "gpuline.jacobian.data.bin", 9, m.nelem, m.nintface, jac)
+    _, hist = run(["1", "0", str(new), "--iters", "3", "--write-resized", "resized.bin"], tmp_path)
+    nv, ne, nif, r = tbs.read_jacobian(tmp_path / "resized.bin")
+    assert (nv, ne, nif) == (new, m.nelem, m.nintface)
+    mn = min(9, new)
+    exp = dict(jacD=ref_resize_blocks(jac["jacD"], m.nelem, 9, new),
+               jacO1=ref_resize_blocks(jac["jacO1"], m.nintface, 9, new),
+               jacO2=ref_resize_blocks(jac["jacO2"], m.nintface, 9, new),
+               rhs=ref_resize_vec(jac["rhs"], m.nelem, new, mn))
+    for k, v in exp.items():
+        assert np.array_equal(r[k], v), k
+    mr = HostMesh(new, m.epoint, m.ef, m.fc, m.linesize, m.linepoint, m.lines, m.lineface)
+    ref = oracle.lij_run(mr, exp, 3)
+    assert hist.size == 4 and np.allclose(hist, ref["hist"], rtol=1e-10, atol=0, equal_nan=True)
+    if new < 9:
+        assert np.isfinite(hist).all() and hist[-1] < 1e-3 * hist[0]

@@ -71,40 +71,46 @@ void Jacobian::assembleTriBlocks(Mesh &mesh) {
 
 void Jacobian::resizeBlockSize(int nvar_new) {
     if (nvar_new == nvar) return;
-    // Pad (diagonal 10) or truncate every block and vector to nvar_new, as
-    // src/Jacobian.cxx:153-238 does; A and DinvC are re-sized too (the
-    // reference leaves DinvC at the old size), so call assembleTriBlocks
-    // again afterwards.
+    // src/Jacobian.cxx:153-238, step for step in effect:
+    //  - every array is resized in place (std::vector::resize keeps the old
+    //    flat prefix and zero-fills what grows; the reference's zeroing loops
+    //    iterate over copies and change nothing);
+    //  - the diagonal of every D, Of1 and Of2 block is set to 10;
+    //  - the leading min(old, new) square of each block is copied from the
+    //    old block (old stride) into the new block (new stride);
+    //  - rhs and U0 get a flat copy of their first nelem*min(old, new) values.
+    // A and DinvC keep their old contents; the driver re-assembles A after a
+    // resize (the reference leaves A at the old stride).
     printf("Resizing NVAR: old=%d new=%d ...", nvar, nvar_new);
     const int old = nvar, mn = std::min(nvar, nvar_new);
-    auto reblock = [&](const std::vector<double> &src, size_t nblk) {
-        std::vector<double> dst((size_t)nvar_new * nvar_new * nblk, 0.0);
-        for (size_t b = 0; b < nblk; ++b) {
-            for (int j = 0; j < nvar_new; ++j) dst[b * nvar_new * nvar_new + nvar_new * j + j] = 10.0;
+    const size_t nn = (size_t)nvar_new * nvar_new, oo = (size_t)old * old;
+    auto reblock = [&](std::vector<double> &v, size_t nblk) {
+        const std::vector<double> prev(v);
+        v.resize(nn * nblk);
+        for (size_t b = 0; b < nblk; ++b)
+            for (int j = 0; j < nvar_new; ++j) v[b * nn + (size_t)nvar_new * j + j] = 10.0;
+        for (size_t b = 0; b < nblk; ++b)
             for (int j = 0; j < mn; ++j)
-                for (int i = 0; i < mn; ++i)
-                    dst[b * nvar_new * nvar_new + nvar_new * j + i] = src[b * old * old + old * j + i];
-        }
-        return dst;
+                for (int i = 0; i < mn; ++i) v[b * nn + (size_t)nvar_new * j + i] = prev[b * oo + (size_t)old * j + i];
     };
-    auto revec = [&](const std::vector<double> &src) {
-        std::vector<double> dst((size_t)nvar_new * nelem, 0.0);
-        for (size_t i = 0; i < (size_t)nelem * mn; ++i) dst[i] = src[i];
-        return dst;
+    auto revec = [&](std::vector<double> &v) {
+        const std::vector<double> prev(v);
+        v.resize((size_t)nvar_new * nelem);
+        for (size_t i = 0; i < (size_t)nelem * mn; ++i) v[i] = prev[i];
     };
-    jacD = reblock(jacD, nelem);
-    jacO1 = reblock(jacO1, nintface);
-    jacO2 = reblock(jacO2, nintface);
-    rhs = revec(rhs);
-    U0 = revec(U0);
+    reblock(jacD, nelem);
+    reblock(jacO1, nintface);
+    reblock(jacO2, nintface);
+    revec(rhs);
+    revec(U0);
     nvar = nvar_new;
     const size_t bs = (size_t)nvar * nvar;
-    jacDLU.assign(bs * nelem, 0.0);
-    DinvC.assign(bs * nelem, 0.0);
-    A.assign(bs * nelem, 0.0);
+    jacDLU.resize(bs * nelem);
+    A.resize(bs * nelem);
+    DinvC.resize(bs * nelem);   // the device copy needs the new size
     U.assign((size_t)nvar * nelem, 0.0);
-    dU.assign((size_t)nvar * nelem, 0.0);
-    res.assign((size_t)nvar * nelem, 0.0);
+    dU.resize((size_t)nvar * nelem);
+    res.resize((size_t)nvar * nelem);
     printf(" done!\n");
 }
 

@@ -9,7 +9,7 @@
 // run the same iteration through the graph-captured fused path.
 //
 // Options: --synthetic NLxNK  --shift S  --seed N  --iters N  --fused
-//          --precision fp64|fp32  --rtol R  --write-data
+//          --precision fp64|fp32  --rtol R  --write-data  --write-resized FILE
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -54,7 +54,7 @@ void usage() {
            "  device_id:    Device ID on node\n"
            "  block_size:   Size of the matrix sub-block (e.g. 9 for 9x9 blocks)\n"
            "Options: --synthetic NLxNK --shift S --seed N --iters N --fused --precision fp64|fp32\n"
-           "         --rtol R --write-data\n");
+           "         --rtol R --write-data --write-resized FILE\n");
 }
 
 }  // namespace
@@ -64,6 +64,7 @@ int main(int argc, char **argv) {
     int NL = 0, NK = 0, seed = 1, precision = TBS_FP64;
     double shift = 20.0, rtol = 0.0;
     bool fused = false, write_data = false;
+    std::string resized_path;
     std::vector<std::string> pos;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
@@ -79,6 +80,7 @@ int main(int argc, char **argv) {
         else if (a == "--rtol") rtol = std::stod(next());
         else if (a == "--fused") fused = true;
         else if (a == "--write-data") write_data = true;
+        else if (a == "--write-resized") resized_path = next();
         else if (a == "--precision") precision = (next() == "fp32") ? TBS_FP32 : TBS_FP64;
         else pos.push_back(a);
     }
@@ -105,6 +107,7 @@ int main(int argc, char **argv) {
         Jac.assembleTriBlocks(mesh);
         Jac.resizeBlockSize(block_size);
         if (Jac.nvar != mesh.nvar) { mesh.nvar = Jac.nvar; Jac.assembleTriBlocks(mesh); }
+        if (!resized_path.empty() && !Jac.toFile(resized_path)) { fprintf(stderr, "cannot write %s\n", resized_path.c_str()); return 1; }
         mesh.setupDevice(gpu);
         mesh.toDevice();
         Jac.setupDevice(gpu);
@@ -194,7 +197,7 @@ int main(int argc, char **argv) {
         printf("-----------------------------------------\n");
         printf("Iterations: %d\nTotal Time: %f\n", iters, dU_time + cp_time + LR_time + LU_time);
         for (size_t k = 0; k < hist.size(); ++k)
-            printf("  ||R_%zu|| = %.6e  (%.3e relative)\n", k, hist[k], hist[k] / hist[0]);
+            printf("  ||R_%zu|| = %.17e  (%.3e relative)\n", k, hist[k], hist[k] / hist[0]);
         Jac.fromDevice();
     } catch (const std::exception &e) {
         fprintf(stderr, "ERROR: %s\n", e.what());
