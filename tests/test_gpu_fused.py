@@ -83,6 +83,23 @@ def test_variants_fp64(tbs, ctxs, oracle, name, make):
     assert relerr(out["U"], ref["U"]) < 1e-12
 
 
+@pytest.mark.parametrize("nv", [2, 3, 6, 7, 8, 10, 11, 12, 13, 14, 15, 16])
+def test_block_sizes(tbs, ctxs, oracle, nv):
+    """Every compiled block size of the fused path: FP64 history and U against
+    the oracle, FP32 factors converge to the same solution."""
+    print(f"case block size {nv}", flush=True)
+    m = ragged_lines([9, 1, 40, 23, 64, 70], nv)
+    jac = synth_np.generate(20 + nv, nv, m.nelem, m.nintface, 2.0 * nv + 2)
+    ref = oracle.lij_run(m, jac, 8)
+    out = solve_plan(tbs, ctxs(nv), m, jac, 8)
+    ok, worst = cf.history_close(out["hist"], ref["hist"])
+    assert ok, worst
+    assert relerr(out["U"], ref["U"]) < 1e-12
+    o32 = solve_plan(tbs, ctxs(nv), m, jac, 40, rtol=1e-10, prec=1)
+    assert o32["hist"][-1] <= 1e-10 * o32["hist"][0]
+    assert relerr(o32["U"], ref["U"]) < 1e-8
+
+
 def test_fp32_factors_reach_fp64_accuracy(tbs, ctxs, oracle):
     """FP32 factors and Thomas sweeps, FP64 residual/update/norm: same 1e-10 target."""
     m, jac = system(60, 64, 9, seed=4, shift=12.0)

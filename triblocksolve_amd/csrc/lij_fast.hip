@@ -308,7 +308,14 @@ __global__ __launch_bounds__(64) void fthomas_kernel(const SysDev<TF> *__restric
     __shared__ double s_u[2][IOD::LDS];     // backward: U (updated in place)
     const int l = blockIdx.x;
     if (l >= nl) return;
-    const int4 lt = ltab[l];          // {system, m0, n, window offset}
+    // {system, m0, n, window offset}, wave-uniform by construction: the
+    // group loops below must be scalar loops (the chain broadcasts read
+    // lanes that every step writes)
+    int4 lt = ltab[l];
+    lt.x = __builtin_amdgcn_readfirstlane(lt.x);
+    lt.y = __builtin_amdgcn_readfirstlane(lt.y);
+    lt.z = __builtin_amdgcn_readfirstlane(lt.z);
+    lt.w = __builtin_amdgcn_readfirstlane(lt.w);
     const SysDev<TF> &S = sys[lt.x];
     if (!S.state[1]) return;
     const int lane = threadIdx.x;
@@ -641,9 +648,11 @@ __global__ __launch_bounds__(64) void ffactor_kernel(int nline, const int *__res
     const int l = blockIdx.x;
     if (l >= nline) return;
     const int lane = threadIdx.x;
-    const int m0 = linepoint[l], n = linesize[l];
-    const long wbase = (long)woff[l] * WL::STRIDE;
-    const int nch = (n + CH - 1) / CH;
+    // wave-uniform by construction: the chunk and step loops must be scalar
+    // loops (the pivot broadcasts read lanes that every step writes)
+    const int m0 = __builtin_amdgcn_readfirstlane(linepoint[l]);
+    const int n = __builtin_amdgcn_readfirstlane(linesize[l]);
+    const long wbase = (long)__builtin_amdgcn_readfirstlane(woff[l]) * WL::STRIDE;
 
     auto load_idx = [&](int c) -> int4 {
         const int k = c * CH + lane;
@@ -767,22 +776,21 @@ __global__ __launch_bounds__(64) void ffactor_kernel(int nline, const int *__res
         }
     };
 
-    for (int c = 0; c < nch; ++c) {
-        const int b = c & 1;
-        for (int s = 0; s < CH; s += 2) {
-            const int k = c * CH + s;
-            if (k >= n) break;
-            step(std::integral_constant<int, 0>(), b, s, k);
-            if (k + 1 >= n) break;
-            step(std::integral_constant<int, 1>(), b, s + 1, k + 1);
+    // one uniform loop over step pairs (CH is even); at the first step of
+    // chunk c > 0: chunk c into LDS, chunk c+1's blocks and chunk c+2's
+    // indices into flight
+    for (int k = 0; k < n; k += 2) {
+        const int c = k / CH, s = k - c * CH;
+        if (s == 0 && c > 0) {
+            __syncthreads();                            // LDS of chunk c-1 read
+            store_blocks(c & 1, R);                     // chunk c
+            if (lane <= CH) sidx[(c - 1) & 1][lane] = xnext;   // indices of chunk c+1
+            __syncthreads();
+            load_blocks(c + 1, sidx[(c - 1) & 1], R);
+            xnext = load_idx(c + 2);
         }
-        if (c + 1 >= nch) break;
-        __syncthreads();                       // LDS of chunk c read
-        store_blocks(b ^ 1, R);                // chunk c+1
-        if (lane <= CH) sidx[b][lane] = xnext; // indices of chunk c+2
-        __syncthreads();
-        load_blocks(c + 2, sidx[b], R);
-        xnext = load_idx(c + 3);
+        step(std::integral_constant<int, 0>(), c & 1, s, k);
+        if (k + 1 < n) step(std::integral_constant<int, 1>(), c & 1, s + 1, k + 1);
     }
 }
 
@@ -1150,7 +1158,7 @@ int launch_final(hipStream_t st, const void *sys, int nsys, int max_sweeps, doub
 }
 
 // Dispatch on (nvar, precision) for the fused kernels.
-#define TBS_PLAN_NVARS(X) X(4) X(5) X(9)
+#define TBS_PLAN_NVARS(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
 
 int k_residual(int nv, int prec, hipStream_t st, const void *sys, const int2 *gtab, int ng,
                int want_R, int force) {
