@@ -54,26 +54,53 @@ def measured_traffic(workload, kernel):
         return None
 
 
-def cpu_baseline(tbs, args, nelem_full):
-    """The oracle (OpenMP over lines, all host cores) on a bounded slice of the
-    workload: the first lines of the same grid, same generator, 3 timed sweeps."""
+def cpu_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    flags = ""
+    try:
+        for line in open(os.path.join(ROOT, "oracle", "Makefile")):
+            if line.startswith("CXXFLAGS"):
+                flags = line.split("=", 1)[1].strip()
+    except OSError:
+        pass
+    return model, flags
+
+
+def cpu_baseline(tbs, args, nelem_full, sweeps=10):
+    """The oracle (the reference's Serial/OpenMP path restated in C++, OpenMP
+    over lines) on a bounded slice of the workload: the first tenth of the
+    lines of the same grid and generator, `sweeps` timed sweeps after one
+    warm-up, with all host cores (up to 16) and with one thread."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_py import Oracle
-    import numpy as np
-    cores = os.cpu_count() or 1
-    cores = min(cores, 16)
+    cores = min(os.cpu_count() or 1, 16)
     nl = max(1, args.nl // 10)
     m = tbs.structured_mesh(nl, args.nk)
     jac = tbs.generate_host(args.seed, 9, m.nelem, m.nintface, args.shift)
     o = Oracle()
-    o.lij_run(m, jac, 1, nthreads=cores)                 # warm-up
-    out = o.lij_run(m, jac, 3, nthreads=cores)
-    sweeps_per_s_sample = 3.0 / out["t_sweeps"]
     scale = m.nelem / nelem_full
-    return {"value": sweeps_per_s_sample * scale, "unit": "sweeps/s", "cores": cores, "kind": "port",
+    res = {}
+    for threads in (cores, 1):
+        o.lij_run(m, jac, 1, nthreads=threads)                 # warm-up
+        out = o.lij_run(m, jac, sweeps, nthreads=threads)
+        res[threads] = (out["t_sweeps"] / sweeps, out["t_factor"])
+    model, flags = cpu_info()
+    t_all, f_all = res[cores]
+    t_one, f_one = res[1]
+    return {"value": scale / t_all, "unit": "sweeps/s", "cores": cores, "kind": "port",
             "sample": f"{nl} x {args.nk} slice ({m.nelem} elements, {scale:.3f} of the workload), "
-                      f"3 sweeps after 1 warm-up, OpenMP over lines; sweeps/s scaled by the element ratio",
-            "t_sweep_sample_s": out["t_sweeps"] / 3.0, "t_factor_sample_s": out["t_factor"]}
+                      f"{sweeps} sweeps after 1 warm-up, OpenMP over lines with {cores} threads; "
+                      f"sweeps/s scaled by the element ratio",
+            "serial_sweeps_per_s": scale / t_one, "t_sweep_sample_s": t_all, "t_sweep_sample_serial_s": t_one,
+            "t_factor_sample_s": f_all, "t_factor_sample_serial_s": f_one, "cpu_model": model,
+            "nproc": os.cpu_count(), "compiler_flags": flags}
 
 
 def bench_batched(tbs, ctx, args, seed=3):
@@ -136,6 +163,50 @@ def bench_batched(tbs, ctx, args, seed=3):
         for x in t:
             x.close()
     return res
+
+
+def bench_gmres(tbs, ctx, m, args, restart=30):
+    """Line-preconditioned GMRES(30) against plain LIJ, both to 1e-10 on the
+    headline shape with one FP64 factorisation each: at the headline shift
+    (one LIJ sweep per preconditioner application) and at shift 3, where
+    LIJ alone diverges (two sweeps per application)."""
+    n, nif = m.nelem, m.nintface
+    out = {}
+    for shift, sweeps in ((args.shift, 1), (3.0, 2)):
+        z = lambda k: ctx.array(size=k, zero=False)
+        D, O1, O2, rhs = z(81 * n), z(81 * nif), z(81 * nif), z(9 * n)
+        ctx.generate_device(args.seed, n, nif, shift, D, O1, O2, rhs)
+        U = ctx.array(size=9 * n)
+        ctx.sync()
+        t0 = time.perf_counter()
+        p = tbs.Plan(ctx, m, tbs.TBS_FP64)
+        p.factor(D, O1, O2)
+        ctx.sync()
+        t_factor = time.perf_counter() - t0
+        res = {"factor_s": t_factor}
+        t0 = time.perf_counter()
+        try:
+            hist, ns = p.solve(rhs, U, 200, 1e-10)
+            ctx.sync()
+            res["lij"] = {"sweeps": int(ns), "solve_s": time.perf_counter() - t0,
+                          "final_rel_residual": float(hist[-1] / hist[0])}
+        except tbs.TbsError as e:
+            res["lij"] = {"error": str(e)}
+        U.zero()
+        g = tbs.Gmres(p, restart)
+        t0 = time.perf_counter()
+        hist, it = g.solve(rhs, U, 300, 1e-10, sweeps)
+        ctx.sync()
+        t = time.perf_counter() - t0
+        st = g.stats()
+        res["gmres"] = {"iterations": int(it), "sweeps_per_preconditioner": sweeps, "solve_s": t, "ms_per_iteration": t / max(it, 1) * 1e3,
+                        "final_rel_residual": st["true_residual"] / float(hist[0]), "restarts": st["restarts"]}
+        out[f"shift{shift:g}"] = res
+        g.close()
+        p.close()
+        for b_ in (D, O1, O2, rhs, U):
+            b_.close()
+    return out
 
 
 def bench_serving(tbs, ctx, args, nreq=160, seed=7):
@@ -304,6 +375,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
+        # BASELINE.md's V100 figure is derived (Table 5 time / 5 sweeps), not published as sweeps/s
+        "v100_derived_sweeps_per_s": 65.16,
         "dtype": "f64",
         "data": "synthetic (seeded structured-grid generator, generated in HBM)",
         "config": {"workload": f"lij_{args.nl}x{args.nk}_nvar9_shift{args.shift:g}_seed{args.seed}",
@@ -337,12 +410,13 @@ def main():
         for d in (a, b):
             d.pop("U")
         out["time_to_1e-10"] = {"fp64": a, "fp32_factor": b, "fp32_vs_fp64_U_relerr": err}
+        out["gmres"] = bench_gmres(tbs, ctx, m, args)
         out["batched"] = bench_batched(tbs, ctx, args)
         out["serving"] = bench_serving(tbs, ctx, args)
     if not args.no_cpu:
         cb = cpu_baseline(tbs, args, n)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
-        out["cpu_baseline_detail"] = {k: cb[k] for k in ("t_sweep_sample_s", "t_factor_sample_s")}
+        out["cpu_baseline_detail"] = {k: v for k, v in cb.items() if k not in out["cpu_baseline"]}
     ctx.close()
     print(json.dumps(out))
 

@@ -222,6 +222,26 @@ int tbs_batch_solve(tbs_batch *batch, const double *const *rhs,
                     int sweeps_per_check, double *hist, int *nsweeps);
 
 /* ---------------------------------------------------------------------- */
+/* Line-preconditioned GMRES (the paper's Krylov solver around the LIJ      */
+/* smoother, kirby-mavriplis-2025 Table 6; the reference repository ships   */
+/* no GMRES code).  Flexible right-preconditioned GMRES(restart) on a       */
+/* factored plan solves J U = -rhs; the preconditioner is `sweeps` LIJ      */
+/* sweeps from zero (1 = the block-tridiagonal line solve).  The Krylov     */
+/* basis (2*restart+1 vectors of nelem*nvar doubles) is allocated at create. */
+/* ---------------------------------------------------------------------- */
+typedef struct tbs_gmres tbs_gmres;
+int tbs_gmres_create(tbs_plan *plan, int restart, tbs_gmres **out);
+int tbs_gmres_destroy(tbs_gmres *gmres);
+/* rhs, U: device, element order; U is the initial guess and the result.
+ * hist (host, hist_cap entries): ||R_0|| then the GMRES residual estimate
+ * after each iteration; iters: iterations run.                             */
+int tbs_gmres_solve(tbs_gmres *gmres, const double *rhs, double *U, int max_iters, double rtol,
+                    int sweeps, double *hist, int hist_cap, int *iters);
+/* [0] operator applications, [1] preconditioner sweeps, [2] restart cycles,
+ * [3] true residual norm ||rhs + J U|| at the end of the last solve.       */
+int tbs_gmres_stats(tbs_gmres *gmres, double *stats, int n);
+
+/* ---------------------------------------------------------------------- */
 /* Serving: continuous batching of solve requests on one GPU.  Requests    */
 /* wait in a FIFO; each tbs_server_step runs K sweeps of the active batch, */
 /* admits queued requests whose HBM footprint fits the budget (generation, */

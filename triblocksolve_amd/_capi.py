@@ -105,6 +105,10 @@ SIGNATURES = {
     "tbs_batch_solve": [P, P, P, I, D, I, P, P],
     "tbs_batch_launch_sweeps": [P, I, I, D],
     "tbs_batch_states": [P, P],
+    "tbs_gmres_create": [P, I, PP],
+    "tbs_gmres_destroy": [P],
+    "tbs_gmres_solve": [P, P, P, I, D, I, P, I, C.POINTER(C.c_int)],
+    "tbs_gmres_stats": [P, P, I],
     "tbs_server_create": [P, C.c_size_t, I, D, I, I, PP],
     "tbs_server_destroy": [P],
     "tbs_server_footprint": [P, C.POINTER(ServerRequest), C.POINTER(C.c_size_t)],

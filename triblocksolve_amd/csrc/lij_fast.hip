@@ -154,7 +154,10 @@ __global__ __launch_bounds__(256) void fres_kernel(const SysDev<TF> *__restrict_
     const int lane = threadIdx.x & 63;
     const int2 sg = gtab[wid];
     const SysDev<TF> &S = sys[sg.x];
-    if (!force && !S.state[1]) return;
+    // force bit 0: run even if the system is inactive; bit 1: U is zero
+    // (the first residual of a solve from zero is R = rhs: skip J U)
+    if (!(force & 1) && !S.state[1]) return;
+    const bool u_zero = (force & 2) != 0;
     const int g = sg.y;
     const int F = S.F;
     const int m = g * kGroup + lane;
@@ -172,10 +175,11 @@ __global__ __launch_bounds__(256) void fres_kernel(const SysDev<TF> *__restrict_
     const uint32_t moff = (uint32_t)m * NV * 8u;
 
     double u[NV], acc[NV];
-    stage_tile<NV>(rU, m0 + kPad, lane, tile);
-    read_tile<NV>(tile, lane, u);
 #pragma unroll
     for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+    if (!u_zero) {
+    stage_tile<NV>(rU, m0 + kPad, lane, tile);
+    read_tile<NV>(tile, lane, u);
     slot_matvec<NV, double>(rB, 0u, lane, u, acc);                     // D U
     for (int f = 0; f < F; ++f) {
         const int c = S.slotc[g * F + f];
@@ -195,6 +199,7 @@ __global__ __launch_bounds__(256) void fres_kernel(const SysDev<TF> *__restrict_
 #pragma unroll
             for (int i = 0; i < NV; ++i) acc[i] += t[i];
         }
+    }
     }
     double rh[NV], r[NV], nr[NV];
     stage_tile<NV, true>(rRhs, m0 + kPad, lane, tile);
@@ -1243,6 +1248,44 @@ int plan_scatter_U(tbs_plan *p, hipStream_t st, double *U) {
 
 const double *plan_hist(const tbs_plan *p) { return p->d_hist; }
 
+int plan_vecs(tbs_plan *p, PlanVecs *out) {
+    TBS_REQUIRE(p && out && p->factored, TBS_ERR_STATE, "plan not factored");
+    out->ctx = p->ctx;
+    out->rhs = p->d_rhs;
+    out->U = p->d_U;
+    out->R = p->d_R;
+    out->n = p->nle;
+    out->nv = p->nv;
+    out->lines = p->d_lines;
+    return TBS_OK;
+}
+
+int plan_activate(tbs_plan *p, hipStream_t st) { return k_reset(p->prec, st, p->d_sys, 1); }
+
+int plan_residual(tbs_plan *p, hipStream_t st, int want_R, bool u_zero) {
+    return k_residual(p->nv, p->prec, st, p->d_sys, p->d_gtab, p->ngroups, want_R, u_zero ? 3 : 1);
+}
+
+int plan_thomas(tbs_plan *p, hipStream_t st) {
+    return k_thomas(p->nv, p->prec, st, p->d_sys, p->d_ltab, p->nline, p->variant);
+}
+
+int plan_gather(tbs_plan *p, hipStream_t st, const double *src, double *dst) {
+    const long total = (long)p->nle * p->nv;
+    hipLaunchKernelGGL(gather_vec_kernel, dim3(gs_grid(total)), dim3(256), 0, st, total, p->nv, p->d_lines, src,
+                       dst);
+    TBS_LAUNCHED("gather");
+    return TBS_OK;
+}
+
+int plan_scatter(tbs_plan *p, hipStream_t st, const double *src, double *dst) {
+    const long total = (long)p->nle * p->nv;
+    hipLaunchKernelGGL(scatter_vec_kernel, dim3(gs_grid(total)), dim3(256), 0, st, total, p->nv, p->d_lines,
+                       src, dst);
+    TBS_LAUNCHED("scatter");
+    return TBS_OK;
+}
+
 size_t plan_factor_workspace(int nvar, int nelem) { return factor_workspace_bytes(nvar, nelem); }
 
 }  // namespace tbs
@@ -1518,8 +1561,9 @@ int tbs_plan_begin(tbs_plan *p, const double *rhs, const double *U0) {
                        p->d_lines, U0, p->d_U);
     TBS_LAUNCHED("gather U0");
     TBS_TRY(k_reset(p->prec, st, p->d_sys, 1));
-    // R_0 = rhs + J U_0 (= rhs when U_0 = 0, the reference's start) and ||R_0||
-    TBS_TRY(k_residual(p->nv, p->prec, st, p->d_sys, p->d_gtab, p->ngroups, 0, 1));
+    // R_0 = rhs + J U_0 (= rhs when U_0 = 0, the reference's start; then J U
+    // is skipped) and ||R_0||
+    TBS_TRY(k_residual(p->nv, p->prec, st, p->d_sys, p->d_gtab, p->ngroups, 0, U0 ? 1 : 3));
     TBS_TRY(k_final(p->prec, st, p->d_sys, 1, p->max_sweeps, p->rtol, 1));
     return TBS_OK;
 }

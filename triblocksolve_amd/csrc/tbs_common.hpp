@@ -72,6 +72,28 @@ const double *plan_hist(const tbs_plan *p);
 // Bytes the plan's factorisation draws from a context region.
 size_t plan_factor_workspace(int nvar, int nelem);
 
+// Internal-order vectors of a factored plan (n elements x nvar values each,
+// line-major order) for solvers built on the plan's kernels.
+struct PlanVecs {
+    tbs_ctx *ctx;      // the plan's context (device, stream)
+    double *rhs, *U, *R;
+    long n;          // elements
+    int nv;
+    const int *lines;  // internal element m holds element lines[m]
+};
+int plan_vecs(tbs_plan *p, PlanVecs *out);
+// Mark the plan active so its sweep kernels run (state reset, no history).
+int plan_activate(tbs_plan *p, hipStream_t st);
+// R = rhs + J U over the plan's internal vectors; also g = D'^-1(-R) for the
+// next Thomas pass.  R is stored only when want_R; u_zero: U is known to be
+// zero, so R = rhs and the J U pass is skipped.
+int plan_residual(tbs_plan *p, hipStream_t st, int want_R, bool u_zero = false);
+// U += d with M d = -R (block-Thomas on the g of the last residual pass).
+int plan_thomas(tbs_plan *p, hipStream_t st);
+// Element order <-> internal order for one vector.
+int plan_gather(tbs_plan *p, hipStream_t st, const double *src, double *dst_internal);
+int plan_scatter(tbs_plan *p, hipStream_t st, const double *src_internal, double *dst);
+
 inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
 }  // namespace tbs

@@ -498,6 +498,43 @@ class Batch:
         return [hist[i, : ns[i] + 1] for i in range(n)], ns
 
 
+class Gmres:
+    """Flexible line-preconditioned GMRES(restart) on a factored plan (csrc/gmres.hip)."""
+
+    def __init__(self, plan, restart=20):
+        self.plan = plan
+        self.ctx = plan.ctx
+        h = C.c_void_p()
+        A.check(self.ctx._lib.tbs_gmres_create(plan.handle, restart, C.byref(h)))
+        self.handle = h
+        self.ctx._children.add(self)
+
+    def close(self):
+        if self.handle and self.ctx.handle:
+            self.ctx._lib.tbs_gmres_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, rhs, U, max_iters=100, rtol=1e-10, sweeps=1):
+        """Solve J U = -rhs from the initial guess in U; returns (history, iterations)."""
+        hist = np.zeros(max_iters + 2)
+        it = C.c_int()
+        A.check(self.ctx._lib.tbs_gmres_solve(self.handle, _ptr(rhs), _ptr(U), max_iters, float(rtol), sweeps,
+                                              _ptr(hist), hist.size, C.byref(it)))
+        return hist[: it.value + 1], it.value
+
+    def stats(self):
+        s = np.zeros(4)
+        A.check(self.ctx._lib.tbs_gmres_stats(self.handle, _ptr(s), 4))
+        return dict(operator_applications=int(s[0]), preconditioner_sweeps=int(s[1]), restarts=int(s[2]),
+                    true_residual=float(s[3]))
+
+
 class Server:
     """Continuous-batching solve service (the C engine in csrc/server.hip)."""
 
