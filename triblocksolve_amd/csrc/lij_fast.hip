@@ -1823,7 +1823,10 @@ int tbs_batch_set_plans(tbs_batch *b, tbs_plan *const *plans, int nplans) {
                      [](const std::pair<int, int4> &x, const std::pair<int, int4> &y) { return x.first > y.first; });
     const size_t ssz = b->prec == TBS_FP32 ? sizeof(SysDev<float>) : sizeof(SysDev<double>);
     const size_t nl = lines.size(), ng = gtab.size(), ns = (size_t)nplans;
-    // grow the device tables (after the work that reads them has finished)
+    // grow the device tables (after the work that reads them has finished);
+    // on any failure below the batch is left empty, so launches are no-ops
+    b->plans.clear();
+    b->nl = b->ng = 0;
     if (ns > b->cap_sys || nl > b->cap_l || ng > b->cap_g) {
         TBS_HIP(hipStreamSynchronize(st));
         if (ns > b->cap_sys) {
@@ -1880,6 +1883,7 @@ int tbs_batch_set_plans(tbs_batch *b, tbs_plan *const *plans, int nplans) {
 
 int tbs_batch_autotune(tbs_batch *b, int reps, int *chosen, double *us, int cap) {
     TBS_REQUIRE(b && reps >= 1, TBS_ERR_ARG, "bad arguments");
+    TBS_REQUIRE(!b->plans.empty(), TBS_ERR_STATE, "batch has no systems");
     TBS_TRY(tbs::use_ctx(b->ctx));
     std::vector<std::pair<double *, long>> zv;
     const size_t tf = tf_size(b->prec);
@@ -1950,6 +1954,7 @@ int tbs_batch_destroy(tbs_batch *b) {
 int tbs_batch_solve(tbs_batch *b, const double *const *rhs, double *const *U, int max_sweeps,
                     double rtol, int sweeps_per_check, double *hist, int *nsweeps) {
     TBS_REQUIRE(b && rhs && U, TBS_ERR_ARG, "null argument");
+    TBS_REQUIRE(!b->plans.empty(), TBS_ERR_STATE, "batch has no systems");
     TBS_REQUIRE(max_sweeps >= 0 && max_sweeps < kHistCap, TBS_ERR_ARG, "bad max_sweeps");
     TBS_TRY(tbs::use_ctx(b->ctx));
     const int ns = (int)b->plans.size();

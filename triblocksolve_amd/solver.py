@@ -411,7 +411,10 @@ class Plan:
         A.check(self.ctx._lib.tbs_plan_set_variant(self.handle, v))
 
     def factor(self, jacD, jacO1, jacO2):
+        """Factorise; returns after the device has finished reading the Jacobian
+        (the C call is asynchronous, and the caller's buffers may be temporaries)."""
         A.check(self.ctx._lib.tbs_plan_factor(self.handle, _ptr(jacD), _ptr(jacO1), _ptr(jacO2)))
+        self.ctx.sync()
 
     def solve(self, rhs, U, max_sweeps, rtol=0.0, R_out=None):
         hist = np.zeros(max_sweeps + 1)
