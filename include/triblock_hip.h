@@ -63,13 +63,16 @@ int tbs_ctx_synchronize(tbs_ctx *ctx);
 /* Wait for all work on the context's device (every stream). */
 int tbs_device_synchronize(tbs_ctx *ctx);
 int tbs_ctx_nvar(tbs_ctx *ctx);
-/* Device memory helpers for hosts without another allocator. */
+/* Device memory helpers for hosts without another allocator (the reference
+ * allocates through occa::device::malloc, src/Mesh.cxx:75-84,
+ * src/Jacobian.cxx:240-259). */
 int tbs_malloc(tbs_ctx *ctx, size_t bytes, void **dptr);
 int tbs_free(tbs_ctx *ctx, void *dptr);
 int tbs_memcpy_h2d(tbs_ctx *ctx, void *dst, const void *src, size_t bytes);
 int tbs_memcpy_d2h(tbs_ctx *ctx, void *dst, const void *src, size_t bytes);
 int tbs_memset(tbs_ctx *ctx, void *dst, int value, size_t bytes);
-/* Timing events on the context's stream (hipEvent_t behind an opaque handle). */
+/* Timing events on the context's stream (hipEvent_t behind an opaque handle;
+ * the reference times with OCCA stream tags, src/main.cxx:181-228). */
 int tbs_event_create(tbs_ctx *ctx, void **ev);
 int tbs_event_destroy(tbs_ctx *ctx, void *ev);
 int tbs_event_record(tbs_ctx *ctx, void *ev);
@@ -199,6 +202,8 @@ int tbs_plan_internal(tbs_plan *plan, int which, void **dptr, size_t *bytes);
 /* ---------------------------------------------------------------------- */
 /* Batched solves: many independent plans (uneven line lengths) advanced   */
 /* by one launch per kernel per sweep, each with its own convergence test. */
+/* No counterpart in the reference (it solves one system per run); for     */
+/* callers that solve many small systems at once.                          */
 /* ---------------------------------------------------------------------- */
 int tbs_batch_create(tbs_ctx *ctx, tbs_plan *const *plans, int nplans,
                      tbs_batch **out);
@@ -242,7 +247,8 @@ int tbs_gmres_solve(tbs_gmres *gmres, const double *rhs, double *U, int max_iter
 int tbs_gmres_stats(tbs_gmres *gmres, double *stats, int n);
 
 /* ---------------------------------------------------------------------- */
-/* Serving: continuous batching of solve requests on one GPU.  Requests    */
+/* Serving: continuous batching of solve requests on one GPU (no          */
+/* counterpart in the reference; for request-driven callers).  Requests    */
 /* wait in a FIFO; each tbs_server_step runs K sweeps of the active batch, */
 /* admits queued requests whose HBM footprint fits the budget (generation, */
 /* factorisation and first residual on a second stream, overlapping the   */
@@ -259,9 +265,10 @@ typedef struct tbs_server_request {
 int tbs_server_create(tbs_ctx *ctx, size_t hbm_budget, int precision, double rtol,
                       int max_sweeps, int sweeps_per_step, tbs_server **out);
 int tbs_server_destroy(tbs_server *srv);
-/* HBM bytes a request holds from admission to retirement (plan, generated
- * Jacobian, factor workspace, rhs and U); admission keeps the sum of the
- * admitted requests' footprints within the budget. */
+/* HBM bytes a request needs when admitted: its plan, rhs and U (held until
+ * retirement) and the generated Jacobian (returned once the request's
+ * set-up has completed).  Admission keeps the sum of what admitted
+ * requests hold within the budget. */
 int tbs_server_footprint(tbs_server *srv, const tbs_server_request *req, size_t *bytes);
 /* TBS_ERR_NOMEM if the request alone exceeds the budget. */
 int tbs_server_submit(tbs_server *srv, const tbs_server_request *req, long *id);
