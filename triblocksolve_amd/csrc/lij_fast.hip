@@ -867,7 +867,7 @@ struct tbs_plan {
     int *h_state = nullptr;      // pinned unless region-backed
     bool in_region = false;      // device arrays carved from a context region
     std::vector<void *> allocs;  // hipMalloc'd arrays (not region-backed)
-    std::vector<char> h_tables;  // host image of the packed tables (source of an async upload)
+    std::unique_ptr<char[]> h_tables;  // host image of the packed tables (source of an async upload)
 };
 
 struct tbs_batch {
@@ -1336,60 +1336,6 @@ int tbs_plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pla
     p->h_woff = L.woff;
     const size_t G = (size_t)L.ngroups * kGroup;
 
-    // host tables
-    std::vector<int> mline(p->nle), inv(p->nelem, -1);
-    for (int l = 0; l < p->nline; ++l)
-        for (int k = 0; k < m->linesize[l]; ++k) mline[m->linepoint[l] + k] = l;
-    for (int i = 0; i < p->nle; ++i) inv[m->lines[i]] = i;
-    std::vector<int2> src(G * L.S, make_int2(0, 0));
-    std::vector<int> nbr(G * std::max(L.F, 1), -1);
-    for (int mm = 0; mm < p->nle; ++mm) {
-        const int e = m->lines[mm];
-        const int g = mm / kGroup, lane = mm % kGroup;
-        src[(size_t)mm * L.S] = make_int2(1, e);
-        int s = 0;
-        for (int k2 = m->epoint[e]; k2 < m->epoint[e + 1]; ++k2, ++s) {
-            const int f = m->ef[k2];
-            if (f < 0) continue;
-            const int e1 = m->fc[2 * f], e2 = m->fc[2 * f + 1];
-            src[(size_t)mm * L.S + 1 + s] = make_int2(e == e1 ? 3 : 2, f);
-            nbr[((size_t)g * L.F + s) * kGroup + lane] = inv[e == e1 ? e2 : e1];
-        }
-    }
-    // slot maps: affine shift when every lane with a face has nbr = m + c
-    std::vector<int> slotc((size_t)L.ngroups * std::max(L.F, 1), kSlotNone);
-    for (int g = 0; g < L.ngroups; ++g)
-        for (int f = 0; f < L.F; ++f) {
-            int c = kSlotNone;
-            for (int lane = 0; lane < kGroup; ++lane) {
-                const int mm = g * kGroup + lane;
-                const int nb = nbr[((size_t)g * L.F + f) * kGroup + lane];
-                if (mm >= p->nle || nb < 0) continue;
-                if (c == kSlotNone) c = nb - mm;
-                else if (c != nb - mm) { c = kSlotGather; break; }
-            }
-            slotc[(size_t)g * L.F + f] = c;
-        }
-    std::vector<int4> ltab(p->nline);
-    for (int l = 0; l < p->nline; ++l) ltab[l] = make_int4(0, m->linepoint[l], m->linesize[l], L.woff[l]);
-    // factor sources: A_k = Of2, C_k = Of1 at face f_k when element k is the face's
-    // first element, else the reverse (src/okl/lineLU_v2.okl:116-117)
-    std::vector<int4> fidx(p->nle);
-    for (int l = 0; l < p->nline; ++l)
-        for (int k = 0; k < m->linesize[l]; ++k) {
-            const int mm = m->linepoint[l] + k, e = m->lines[mm];
-            int a = -1, c = -1;
-            if (k >= 1) {
-                const int f = m->lineface[mm];
-                const bool first = e == m->fc[2 * f];
-                a = 2 * f + (first ? 1 : 0);
-                c = 2 * f + (first ? 0 : 1);
-            }
-            fidx[mm] = make_int4(e, a, c, 0);
-        }
-    std::vector<int2> gtab(L.ngroups);
-    for (int g = 0; g < L.ngroups; ++g) gtab[g] = make_int2(0, g);
-
     tbs_plan *P = p.get();
     const std::array<size_t, A_COUNT> sz = array_sizes(m, L, precision);
     void *arr[A_COUNT];
@@ -1428,23 +1374,77 @@ int tbs_plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pla
     if (P->in_region) P->h_state = new int[4]();
     else TBS_HIP(hipHostMalloc((void **)&P->h_state, 4 * sizeof(int), hipHostMallocDefault));
 
-    // host image of the tables, kept by the plan so the upload stays asynchronous
-    P->h_tables.assign(T.total, 0);
-    char *h = P->h_tables.data();
+    // host image of the tables, built in place (every entry written once) and
+    // kept by the plan so the upload stays asynchronous
+    P->h_tables.reset(new char[T.total]);
+    char *h = P->h_tables.get();
+    const int nle = P->nle, F = L.F, Fm = std::max(F, 1);
     auto put = [h](size_t off, const void *src_, size_t bytes) { if (bytes) std::memcpy(h + off, src_, bytes); };
-    put(T.lines, m->lines, (size_t)P->nle * 4);
-    put(T.mline, mline.data(), (size_t)P->nle * 4);
+    put(T.lines, m->lines, (size_t)nle * 4);
     put(T.linepoint, m->linepoint, (size_t)(P->nline + 1) * 4);
     put(T.linesize, m->linesize, (size_t)P->nline * 4);
     put(T.woff, L.woff.data(), (size_t)(P->nline + 1) * 4);
     put(T.fc, m->fc, (size_t)2 * m->nintface * 4);
-    put(T.lineface, m->lineface, (size_t)P->nle * 4);
-    put(T.src, src.data(), src.size() * sizeof(int2));
-    put(T.fidx, fidx.data(), fidx.size() * sizeof(int4));
-    put(T.nbr, nbr.data(), nbr.size() * 4);
-    put(T.slotc, slotc.data(), slotc.size() * 4);
-    put(T.ltab, ltab.data(), ltab.size() * sizeof(int4));
-    put(T.gtab, gtab.data(), gtab.size() * sizeof(int2));
+    put(T.lineface, m->lineface, (size_t)nle * 4);
+    int *mline = (int *)(h + T.mline);
+    int4 *fidx = (int4 *)(h + T.fidx);
+    int4 *ltab = (int4 *)(h + T.ltab);
+    for (int l = 0; l < P->nline; ++l) {
+        const int m0 = m->linepoint[l], n = m->linesize[l];
+        ltab[l] = make_int4(0, m0, n, L.woff[l]);
+        for (int k = 0; k < n; ++k) {
+            // factor sources: A_k = Of2, C_k = Of1 at face f_k when element k is
+            // the face's first element, else the reverse (src/okl/lineLU_v2.okl:116-117)
+            const int mm = m0 + k, e = m->lines[mm];
+            int a = -1, c = -1;
+            if (k >= 1) {
+                const int f = m->lineface[mm];
+                const bool first = e == m->fc[2 * f];
+                a = 2 * f + (first ? 1 : 0);
+                c = 2 * f + (first ? 0 : 1);
+            }
+            mline[mm] = l;
+            fidx[mm] = make_int4(e, a, c, 0);
+        }
+    }
+    std::vector<int> inv(P->nelem);
+    for (int i = 0; i < nle; ++i) inv[m->lines[i]] = i;
+    // residual sources and neighbour table, [group][slot][lane]; pad lanes empty
+    int2 *src = (int2 *)(h + T.src);
+    int *nbr = (int *)(h + T.nbr);
+    for (size_t mm = 0; mm < G; ++mm) {
+        const size_t g = mm / kGroup, lane = mm % kGroup;
+        int2 *row = src + mm * L.S;
+        for (int q = 0; q < L.S; ++q) row[q] = make_int2(0, 0);
+        for (int s2 = 0; s2 < Fm; ++s2) nbr[(g * Fm + s2) * kGroup + lane] = -1;
+        if ((long)mm >= nle) continue;
+        const int e = m->lines[mm];
+        row[0] = make_int2(1, e);
+        int s2 = 0;
+        for (int k2 = m->epoint[e]; k2 < m->epoint[e + 1]; ++k2, ++s2) {
+            const int f = m->ef[k2];
+            if (f < 0) continue;
+            const int e1 = m->fc[2 * f], e2 = m->fc[2 * f + 1];
+            row[1 + s2] = make_int2(e == e1 ? 3 : 2, f);
+            nbr[(g * Fm + s2) * kGroup + lane] = inv[e == e1 ? e2 : e1];
+        }
+    }
+    // slot maps: affine shift when every lane with a face has nbr = m + c
+    int *slotc = (int *)(h + T.slotc);
+    for (int g = 0; g < L.ngroups; ++g)
+        for (int f = 0; f < Fm; ++f) {
+            int c = kSlotNone;
+            for (int lane = 0; lane < kGroup && f < F; ++lane) {
+                const int mm = g * kGroup + lane;
+                const int nb = nbr[((size_t)g * Fm + f) * kGroup + lane];
+                if (mm >= nle || nb < 0) continue;
+                if (c == kSlotNone) c = nb - mm;
+                else if (c != nb - mm) { c = kSlotGather; break; }
+            }
+            slotc[(size_t)g * Fm + f] = c;
+        }
+    int2 *gtab = (int2 *)(h + T.gtab);
+    for (int g = 0; g < L.ngroups; ++g) gtab[g] = make_int2(0, g);
     if (precision == TBS_FP32) {
         SysDev<float> sd = make_sysdev<float>(P);
         put(T.sys, &sd, sizeof(sd));

@@ -142,7 +142,7 @@ int request_sizes(const Request &r, int prec, int nvar, size_t *persist, size_t 
     return TBS_OK;
 }
 
-enum { T_WAIT, T_MEMBERS, T_LAUNCH, T_RETIRE, T_ADMIT, T_NPHASE };
+enum { T_WAIT, T_MEMBERS, T_LAUNCH, T_RETIRE, T_ADMIT, T_PLAN, T_NPHASE };
 enum { C_STEPS, C_ADMITTED, C_RETIRED, C_REBUILDS, C_NCOUNT };
 
 }  // namespace
@@ -201,7 +201,10 @@ int admit(tbs_server *s, Request *r) {
     double *D = nullptr, *O1 = nullptr, *O2 = nullptr;
     int rc = TBS_OK;
     do {
-        if ((rc = tbs_plan_create(a, &mh, s->prec, &r->plan))) break;
+        const double tp0 = s->now();
+        rc = tbs_plan_create(a, &mh, s->prec, &r->plan);
+        s->t_phase[T_PLAN] += s->now() - tp0;
+        if (rc) break;
         if ((rc = tbs::dev_alloc(a, (size_t)nv * r->nelem * 8, (void **)&r->d_rhs, nullptr))) break;
         if ((rc = tbs::dev_alloc(a, (size_t)nv * r->nelem * 8, (void **)&r->d_U, nullptr))) break;
         if (a->region_used > r->persist) { rc = TBS_ERR_STATE; tbs::set_error("slab layout overflow"); break; }

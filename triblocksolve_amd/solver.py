@@ -609,8 +609,8 @@ class Server:
 
     def stats(self):
         """Cumulative host seconds per engine phase and event counts."""
-        t = np.zeros(5)
+        t = np.zeros(6)
         c = np.zeros(4, np.int64)
-        A.check(self.ctx._lib.tbs_server_stats(self.handle, _ptr(t), 5, _ptr(c), 4))
-        return dict(zip(("wait", "membership", "launch", "retire", "admit"), t.tolist()),
+        A.check(self.ctx._lib.tbs_server_stats(self.handle, _ptr(t), 6, _ptr(c), 4))
+        return dict(zip(("wait", "membership", "launch", "retire", "admit", "admit_plan_build"), t.tolist()),
                     **dict(zip(("steps", "admitted", "retired", "membership_updates"), c.tolist())))
