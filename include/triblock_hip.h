@@ -159,6 +159,8 @@ typedef struct tbs_plan_info {
 
 /* Footprint the plan would need, without allocating (admission control). */
 int tbs_plan_footprint(const tbs_mesh_host *mesh, int precision, size_t *bytes);
+/* TBS_ERR_UNSUPPORTED if one solution vector would exceed 4 GiB (about
+ * 59 million elements at nvar 9): split such systems over several plans. */
 int tbs_plan_create(tbs_ctx *ctx, const tbs_mesh_host *mesh, int precision,
                     tbs_plan **out);
 int tbs_plan_destroy(tbs_plan *plan);

@@ -1317,6 +1317,11 @@ int tbs_plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pla
     TBS_REQUIRE(m->nvar == ctx->nvar, TBS_ERR_ARG, "mesh nvar %d != context nvar %d", m->nvar, ctx->nvar);
     TBS_REQUIRE(plan_nvar_ok(m->nvar), TBS_ERR_UNSUPPORTED, "nvar %d not built for the fused path", m->nvar);
     TBS_REQUIRE(precision == TBS_FP64 || precision == TBS_FP32, TBS_ERR_ARG, "bad precision");
+    // whole-system vectors are addressed through buffer descriptors with a
+    // 32-bit byte range
+    TBS_REQUIRE(((size_t)m->nlineelem + 2 * kPad) * m->nvar * 8 < ((size_t)1 << 32) - 4096, TBS_ERR_UNSUPPORTED,
+                "system too large for one plan: %d elements x %d values exceeds 4 GiB per vector",
+                m->nlineelem, m->nvar);
     const int nv = m->nvar;
     Layout L = layout_of(m);
     std::unique_ptr<tbs_plan, void (*)(tbs_plan *)> p(new tbs_plan(), free_plan);
