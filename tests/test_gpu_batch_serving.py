@@ -163,3 +163,39 @@ def test_batch_membership_changes_in_place(tbs, ctx):
     b.close()
     for s_ in systems:
         s_[2].close()
+
+
+def test_server_fp64_mixed_sizes(tbs, ctx):
+    """FP64 server with tiny (one element, one line) and larger requests in one
+    batch: every result is bitwise its solo FP64 solve."""
+    print("case server fp64 mixed", flush=True)
+    specs = [(1, 1, 51), (1, 300, 52), (90, 8, 53), (200, 150, 54), (3, 2, 55)]
+    srv = tbs.Server(ctx, precision=tbs.TBS_FP64, rtol=1e-10, max_sweeps=60, sweeps_per_step=3)
+    ids = {srv.submit(NL, NK, seed, 12.0): (NL, NK, seed) for NL, NK, seed in specs}
+    done = {}
+    for _ in range(3000):
+        srv.step()
+        for rid in srv.poll():
+            done[rid] = srv.result(rid)
+        if len(done) == len(specs):
+            break
+    assert len(done) == len(specs)
+    for rid, (NL, NK, seed) in ids.items():
+        mesh = tbs.structured_mesh(NL, NK)
+        jac = tbs.generate_host(seed, 9, mesh.nelem, mesh.nintface, 12.0)
+        h, n, U = _solve_alone(tbs, ctx, mesh, jac, tbs.TBS_FP64, 60, 1e-10)
+        r = done[rid]
+        assert r["status"] == 1 and r["nsweeps"] == n
+        assert np.array_equal(r["hist"], h) and np.array_equal(r["U"], U), (NL, NK)
+    srv.close()
+
+
+def test_server_rejects_bad_requests(tbs, ctx):
+    srv = tbs.Server(ctx, hbm_budget=1 << 30, precision=tbs.TBS_FP32)
+    for NL, NK in ((0, 10), (10, 0), (-1, 5)):
+        with pytest.raises(tbs.TbsError):
+            srv.submit(NL, NK, 1)
+    with pytest.raises(tbs.TbsError) as ei:
+        srv.result(12345)
+    assert ei.value.code == -1
+    srv.close()

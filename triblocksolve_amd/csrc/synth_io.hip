@@ -120,7 +120,7 @@ int tbs_structured_mesh(int NL, int NK, int *epoint, int *ef, int *fc, int *line
     int n, nif;
     int rc = tbs_structured_sizes(NL, NK, &n, &nif, nullptr, nullptr, nullptr);
     if (rc) return rc;
-    TBS_REQUIRE(epoint && ef && fc && linesize && linepoint && lines && lineface, TBS_ERR_ARG,
+    TBS_REQUIRE(epoint && ef && (fc || nif == 0) && linesize && linepoint && lines && lineface, TBS_ERR_ARG,
                 "null mesh array");
     // face ids of the in-line face (e+1,e) and cross face (e+NK,e) created at e
     std::vector<int> f_in(n, -1), f_cross(n, -1);
@@ -175,7 +175,7 @@ int tbs_ragged_mesh(int nline, const int *lens, int *epoint, int *ef, int *fc, i
     int n, nif, eft;
     int rc = tbs_ragged_sizes(nline, lens, &n, &nif, &eft);
     if (rc) return rc;
-    TBS_REQUIRE(epoint && ef && fc && linesize && linepoint && lines && lineface, TBS_ERR_ARG,
+    TBS_REQUIRE(epoint && ef && (fc || nif == 0) && linesize && linepoint && lines && lineface, TBS_ERR_ARG,
                 "null mesh array");
     std::vector<int> start(nline + 1, 0);
     for (int l = 0; l < nline; ++l) start[l + 1] = start[l] + lens[l];

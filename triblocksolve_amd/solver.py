@@ -588,8 +588,8 @@ class Server:
         return [int(x) for x in ids[: n.value]]
 
     def result(self, rid, want_U=True):
-        n = self.sizes[rid]
-        U = np.zeros(self.nvar * n) if want_U else None
+        n = self.sizes.get(rid)           # unknown ids: the library reports the error
+        U = np.zeros(self.nvar * n) if (want_U and n is not None) else None
         hist = np.zeros(self.max_sweeps + 1)
         ns, st = C.c_int(), C.c_int()
         times = np.zeros(3)
