@@ -15,6 +15,12 @@ import numpy as np
 from . import _capi as A
 
 
+def variant_name(code):
+    """Name of a Thomas variant from its code (100 * prefetch windows + ring cycles)."""
+    code = int(code)
+    return f"pf{code // 100}_k{code % 100}"
+
+
 def _ptr(x):
     """c_void_p of a numpy array, a DeviceBuffer or an int address (None -> NULL)."""
     if x is None:
@@ -398,14 +404,20 @@ class Plan:
         return {k: getattr(inf, k) for k, _ in A.PlanInfo._fields_}
 
     def autotune(self, reps=3):
-        """Pick the fastest Thomas variant; returns (chosen index, {"pf<P>_k<K>": us})."""
-        n = self.ctx._lib.tbs_plan_variants(self.handle, None, 0)
-        pf = np.zeros(n, np.int32)
-        self.ctx._lib.tbs_plan_variants(self.handle, _ptr(pf), n)
+        """Pick the fastest Thomas variant; returns (chosen index, {variant name: us})."""
+        pf = self.variants()
+        n = len(pf)
         us = np.zeros(n)
         ch = C.c_int()
         A.check(self.ctx._lib.tbs_plan_autotune(self.handle, reps, C.byref(ch), _ptr(us), n))
-        return ch.value, {f"pf{int(p) // 100}_k{int(p) % 100}": float(t) for p, t in zip(pf, us)}
+        return ch.value, {variant_name(p): float(t) for p, t in zip(pf, us)}
+
+    def variants(self):
+        """Codes of the compiled Thomas variants (see variant_name)."""
+        n = self.ctx._lib.tbs_plan_variants(self.handle, None, 0)
+        pf = np.zeros(n, np.int32)
+        self.ctx._lib.tbs_plan_variants(self.handle, _ptr(pf), n)
+        return [int(x) for x in pf]
 
     def set_variant(self, v):
         A.check(self.ctx._lib.tbs_plan_set_variant(self.handle, v))
