@@ -558,46 +558,59 @@ __global__ void pack_blocks_kernel(long total, int S, const int2 *__restrict__ s
 // the group's 64 source blocks whole (consecutive threads read consecutive
 // entries of a block, so every load is contiguous), then writes the slot in
 // SoA-64 order (consecutive threads write consecutive addresses).
+// One 128-thread workgroup per half of a (group, slot): the 32 source blocks
+// of lanes h*32 .. h*32+31 come into LDS whole, in 16-byte pieces (consecutive
+// threads read consecutive pieces of a block; a block is contiguous in the
+// source array, 8-byte aligned), and leave as those lanes' part of the SoA-64
+// slot: 512-byte runs of 16-byte pieces.  (Half groups keep the LDS of a
+// workgroup at 21 KB, so seven fit a CU.)
+typedef double f64x2u __attribute__((ext_vector_type(2), aligned(8)));
+constexpr int kPackLanes = 32;
 template <int NV>
-__global__ __launch_bounds__(256) void pack_lds_kernel(int S, const int2 *__restrict__ src,
+__global__ __launch_bounds__(128) void pack_lds_kernel(int S, const int2 *__restrict__ src,
                                                        const double *__restrict__ D,
                                                        const double *__restrict__ O1,
                                                        const double *__restrict__ O2,
                                                        double *__restrict__ blk) {
     constexpr int BS = NV * NV;
+    constexpr int PB = (BS + 1) / 2;           // 16-byte pieces per block (the last may be 8 bytes)
+    constexpr int EL = kPackLanes;
     using L = tbs::Soa64<BS, double>;
-    __shared__ double s[kGroup * BS];          // [element lane][entry]
-    __shared__ const double *s_src[kGroup];
-    const long w = blockIdx.x;                 // (group, slot)
+    static_assert(L::VW == 2, "SoA-64 slots of doubles hold 2 entries per 16-byte chunk");
+    __shared__ double s[EL * BS];              // [element lane - h*EL][entry]
+    __shared__ const double *s_src[EL];
+    const long w = blockIdx.x >> 1;            // (group, slot)
+    const int h = (int)(blockIdx.x & 1u);      // half of the group's lanes
     const long g = w / S;
     const int slot = (int)(w - g * S);
     const int t = threadIdx.x;
-    if (t < kGroup) {
-        const int2 c = src[(g * kGroup + t) * S + slot];
+    if (t < EL) {
+        const int2 c = src[(g * kGroup + h * EL + t) * S + slot];
         const double *a = c.x == 1 ? D : (c.x == 2 ? O1 : (c.x == 3 ? O2 : nullptr));
         s_src[t] = a ? a + (size_t)BS * c.y : nullptr;
     }
     __syncthreads();
-    for (int i = t; i < kGroup * BS; i += 256) {
-        const int e = i / BS, q = i - e * BS;
+    for (int i = t; i < EL * PB; i += 128) {
+        const int e = i / PB, k = i - e * PB;
         const double *p = s_src[e];
-        s[i] = p ? p[q] : 0.0;
+        double *d = s + e * BS + 2 * k;
+        if (2 * k + 1 < BS) {
+            const f64x2u v = p ? *reinterpret_cast<const f64x2u *>(p + 2 * k) : f64x2u{0.0, 0.0};
+            d[0] = v[0];
+            d[1] = v[1];
+        } else {
+            d[0] = p ? p[2 * k] : 0.0;
+        }
     }
     __syncthreads();
     double *out = blk + w * L::ELEMS;
-    for (int i = t; i < L::ELEMS; i += 256) {
-        int q, lane;
-        if (i < L::NC * L::VW * 64) {
-            const int c = i / (64 * L::VW), rem = i - c * 64 * L::VW;
-            lane = rem / L::VW;
-            q = c * L::VW + (rem - lane * L::VW);
-        } else {
-            const int r = i - L::NC * L::VW * 64;
-            q = L::NC * L::VW + r / 64;
-            lane = r % 64;
-        }
-        out[i] = s[lane * BS + q];
+    for (int i = t; i < L::NC * EL; i += 128) {              // chunk c, lane h*EL + e: entries 2c, 2c+1
+        const int c = i / EL, e = i - c * EL;
+        const double *q = s + e * BS + 2 * c;
+        *reinterpret_cast<f64x2u *>(out + c * 128 + (h * EL + e) * 2) = f64x2u{q[0], q[1]};
     }
+    for (int i = t; i < L::REM * EL; i += 128)               // odd BS: the last entry, [lane]
+        out[L::NC * 128 + h * EL + i] = s[i * BS + BS - 1];
 }
 
 // ------------------------------------------------------------- factor --
@@ -626,10 +639,18 @@ __device__ __forceinline__ double recip(double x) {
 // pivot columns), X..X+NV-1 hold C_{k+1} and end with Gamma_k, 2NV.. hold A_k
 // then E_k, 3NV.. hold I then W_k.  P and X swap every step, so the lanes
 // that end a step holding Gamma_k form S_{k+1} in place; each pivot step
-// broadcasts one column with readlane.  The D, A, C blocks of 4 steps at a
-// time are prefetched into double-buffered LDS one chunk ahead, from the
-// per-element source table fidx = {e, A source, C source of the face before}
-// (source = 2 * face + 1 for Of2, 2 * face for Of1, -1 none).
+// broadcasts one column with readlane.  A lone wave issues one instruction
+// every four cycles whatever its kind, so the step is written as one
+// instruction stream for every lane, with no branches on the lane's role:
+// each lane reads its new column through a per-lane LDS pointer (D, C or A
+// of the step, or a resident identity block), every lane applies the
+// S_k update with its old column scaled by 1 (pivot lanes) or 0 (the rest),
+// and the outputs go out through per-lane destination offsets
+// u_i + s * v_i precomputed for the lane's role.  The D, A, C blocks of 4
+// steps at a time are prefetched into double-buffered LDS one chunk ahead;
+// their sources come from the per-element table fidx = {e, A source, C
+// source of the face before} (source = 2 * face + 1 for Of2, 2 * face for
+// Of1, -1 none: the zero block).
 template <int NV, typename TF>
 __global__ __launch_bounds__(64) void ffactor_kernel(int nline, const int *__restrict__ linepoint,
                                                      const int *__restrict__ linesize,
@@ -637,18 +658,23 @@ __global__ __launch_bounds__(64) void ffactor_kernel(int nline, const int *__res
                                                      const int4 *__restrict__ fidx,
                                                      const double *__restrict__ jacD,
                                                      const double *__restrict__ O1,
-                                                     const double *__restrict__ O2, TF *__restrict__ W,
-                                                     TF *__restrict__ EF, TF *__restrict__ GB) {
+                                                     const double *__restrict__ O2,
+                                                     const double *__restrict__ zero_blk,
+                                                     TF *__restrict__ W, TF *__restrict__ EF,
+                                                     TF *__restrict__ GB) {
     constexpr int BS = NV * NV;
     constexpr int CH = 4;                       // steps per chunk
-    constexpr int SB = 3 * BS;                  // doubles per step: D_k, A_k, C_{k+1}
+    constexpr int NB = 3 * CH;                  // blocks per chunk: D_k, A_k, C_{k+1} of each step
+    constexpr int SB = 3 * BS;                  // doubles per step
     constexpr int NLD = (CH * SB + 63) / 64;    // loads per lane per chunk
     constexpr int SPW = 64 / NV, LW = SPW * NV;
     using WL = tbs::WinLayout<NV, LW, TF>;
     using WS = tbs::Soa64<BS, TF>;
     static_assert(4 * NV <= 64, "lane roles need 4*NV lanes");
+    static_assert(BS <= 256, "entry index kept in 8 bits");
     __shared__ double sblk[2][CH * SB];
-    __shared__ int4 sidx[2][CH + 1];
+    __shared__ double sI[BS];                   // identity: the W lanes' starting column
+    __shared__ const double *sptr[2][NB];       // sources of a chunk's blocks
 
     const int l = blockIdx.x;
     if (l >= nline) return;
@@ -659,125 +685,143 @@ __global__ __launch_bounds__(64) void ffactor_kernel(int nline, const int *__res
     const int n = __builtin_amdgcn_readfirstlane(linesize[l]);
     const long wbase = (long)__builtin_amdgcn_readfirstlane(woff[l]) * WL::STRIDE;
 
-    auto load_idx = [&](int c) -> int4 {
-        const int k = c * CH + lane;
-        int4 x = make_int4(0, -1, -1, 0);
-        if (lane <= CH && k < n) x = fidx[m0 + k];
-        return x;
+    // ---- per-lane constants
+    const int grp = lane / NV;                  // role group 0..3 (4+: idle lanes)
+    const int j = lane - grp * NV;              // the lane's column
+    for (int t = lane; t < BS; t += 64) sI[t] = (t % NV == t / NV) ? 1.0 : 0.0;
+    // chunk entry t = lane + 64 r: block (step * 3 + which) << 8 | entry
+    int code[NLD];
+#pragma unroll
+    for (int r = 0; r < NLD; ++r) {
+        const int t = lane + 64 * r;
+        const int st = t / SB, rem = t - st * SB;
+        const int which = rem / BS;
+        code[r] = t < CH * SB ? ((st * 3 + which) << 8) | (rem - which * BS) : -1;
+    }
+    // outputs: lanes of group 0/1 (X, parity-dependent) and 2 write window
+    // entries WL::index(j, st * NV + i) = u_i + st * v_i; group 3 writes W
+    // entries WS::index(i + NV j, lane of the element) = u_i + elane * v_i
+    int u[NV], v[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        if (grp == 3) {
+            u[i] = (int)WS::index(i + NV * j, 0);
+            v[i] = (int)(WS::index(i + NV * j, 1) - u[i]);
+        } else {
+            u[i] = (int)WL::index(j < NV ? j : 0, i);
+            v[i] = (int)(WL::index(j < NV ? j : 0, NV) - WL::index(j < NV ? j : 0, 0));
+        }
+    }
+    const double fP[2] = {grp == 0 ? 1.0 : 0.0, grp == 1 ? 1.0 : 0.0};   // pivot lanes at parity 0 / 1
+
+    // lane t < NB: block t of a chunk is (step t / 3, which = t % 3) and its
+    // source is component `which` of fidx[k] ({element, A source, C source}),
+    // with k the step for D and A, the next step for C_{k+1}; -1: none
+    const int lst = lane / 3, lwhich = lane - 3 * lst;
+    auto load_idx = [&](int c) -> int {
+        const int k = c * CH + lst + (lwhich == 2 ? 1 : 0);
+        int code_src = -1;
+        if (lane < NB && k < n) code_src = reinterpret_cast<const int *>(fidx + m0 + k)[lwhich];
+        return code_src;
     };
-    auto load_blocks = [&](int c, const int4 *ix, double (&R)[NLD]) {
+    // (pointers are chosen by arithmetic on their values: a select between
+    // two captured pointers becomes a select of their addresses, which puts
+    // them in scratch memory)
+    const uintptr_t aD = (uintptr_t)jacD, aO1 = (uintptr_t)O1, aZ = (uintptr_t)zero_blk;
+    const uintptr_t dO = (uintptr_t)O2 - aO1;
+    auto set_ptrs = [&](int b, int cs) {
+        if (lane < NB) {
+            const uintptr_t face = aO1 + (uintptr_t)(cs & 1) * dO + (uintptr_t)((long)(cs >> 1) * BS * 8);
+            const uintptr_t elem = aD + (uintptr_t)((long)cs * BS * 8);
+            const uintptr_t p = lwhich == 0 ? elem : face;
+            sptr[b][lane] = (const double *)(cs >= 0 ? p : aZ);
+        }
+    };
+    // (the sources are global memory: loads through a global-space pointer,
+    // so they count only against vmcnt and the step's LDS waits do not wait
+    // for the prefetch)
+    using gptr = const __attribute__((address_space(1))) double *;
+    auto load_blocks = [&](int b, double (&R)[NLD]) {
 #pragma unroll
         for (int r = 0; r < NLD; ++r) {
-            const int t = lane + 64 * r;
-            R[r] = 0.0;
-            if (t < CH * SB) {
-                const int st = t / SB, rem = t - st * SB;
-                const int which = rem / BS, q = rem - which * BS;
-                const int k = c * CH + st;
-                if (k < n) {
-                    const double *src = nullptr;
-                    if (which == 0) {
-                        src = jacD + (size_t)BS * ix[st].x;
-                    } else {
-                        const int code = which == 1 ? ix[st].y : (k + 1 < n ? ix[st + 1].z : -1);
-                        if (code >= 0) src = ((code & 1) ? O2 : O1) + (size_t)BS * (code >> 1);
-                    }
-                    if (src) R[r] = src[q];
-                }
-            }
+            const int cd = code[r] < 0 ? 0 : code[r];
+            const gptr src = (gptr)sptr[b][cd >> 8];
+            R[r] = src[cd & 255];
         }
     };
     auto store_blocks = [&](int b, const double (&R)[NLD]) {
 #pragma unroll
         for (int r = 0; r < NLD; ++r) {
             const int t = lane + 64 * r;
-            if (t < CH * SB) sblk[b][t] = R[r];
+            if (r < NLD - 1 || t < CH * SB) sblk[b][t] = R[r];
         }
     };
 
     double R[NLD];
     // prologue: chunk 0 in LDS, chunk 1 in flight, indices of chunk 2 in flight
-    {
-        int4 x = load_idx(0);
-        if (lane <= CH) sidx[0][lane] = x;
-        __syncthreads();
-        load_blocks(0, sidx[0], R);
-        store_blocks(0, R);
-        x = load_idx(1);
-        if (lane <= CH) sidx[1][lane] = x;
-        __syncthreads();
-        load_blocks(1, sidx[1], R);
-    }
-    int4 xnext = load_idx(2);
+    set_ptrs(0, load_idx(0));
+    __syncthreads();
+    load_blocks(0, R);
+    store_blocks(0, R);
+    set_ptrs(1, load_idx(1));
+    __syncthreads();
+    load_blocks(1, R);
+    int xnext = load_idx(2);
 
     double col[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) col[i] = 0.0;
 
+    const uintptr_t aW = (uintptr_t)W, aEF = (uintptr_t)EF, dGB = (uintptr_t)GB - (uintptr_t)EF;
     auto step = [&](auto par, int b, int s, int k) {
-        constexpr int P = decltype(par)::value ? NV : 0;
-        constexpr int X = decltype(par)::value ? 0 : NV;
-        const double *D = &sblk[b][s * SB];
-        const double *A = D + BS;
-        const double *C = D + 2 * BS;
-        if (lane >= P && lane < P + NV) {
-            const int j = lane - P;
-            if (k == 0) {
+        constexpr int PAR = decltype(par)::value;
+        constexpr int P = PAR ? NV : 0;
+        const double *Dk = &sblk[b][s * SB];
+        const double *Ak = Dk + BS;
+        // the lane's new column: pivot lanes D_k, X lanes C_{k+1}, A lanes
+        // A_k, W lanes the identity (idle lanes read the identity too)
+        const int xg = PAR ? 0 : 1;
+        const double *cp0 = grp == (PAR ? 1 : 0) ? Dk : (grp == xg ? Dk + 2 * BS : (grp == 2 ? Ak : sI));
+        const double *bcol = cp0 + (grp < 4 ? NV * j : 0);
+        double vold[NV];
 #pragma unroll
-                for (int i = 0; i < NV; ++i) col[i] = D[i + NV * j];
-            } else {
-                double nc[NV];
+        for (int m = 0; m < NV; ++m) vold[m] = col[m] * fP[PAR];
+        // S_k = D_k - A_k Gamma_{k-1} on the pivot lanes (A_k uniform); the
+        // other lanes subtract exact zeros
 #pragma unroll
-                for (int i = 0; i < NV; ++i) {
-                    double tot = 0.0;
+        for (int i = 0; i < NV; ++i) {
+            double tot = 0.0;
 #pragma unroll
-                    for (int m = 0; m < NV; ++m) tot += A[i + NV * m] * col[m];
-                    nc[i] = D[i + NV * j] - tot;
-                }
-#pragma unroll
-                for (int i = 0; i < NV; ++i) col[i] = nc[i];
-            }
-        } else if (lane >= X && lane < X + NV) {
-#pragma unroll
-            for (int i = 0; i < NV; ++i) col[i] = C[i + NV * (lane - X)];
-        } else if (lane >= 2 * NV && lane < 3 * NV) {
-#pragma unroll
-            for (int i = 0; i < NV; ++i) col[i] = A[i + NV * (lane - 2 * NV)];
-        } else {
-#pragma unroll
-            for (int i = 0; i < NV; ++i) col[i] = (lane - 3 * NV == i) ? 1.0 : 0.0;
+            for (int m = 0; m < NV; ++m) tot += Ak[i + NV * m] * vold[m];
+            col[i] = bcol[i] - tot;
         }
         // Gauss-Jordan on the pivot columns
 #pragma unroll
         for (int p = 0; p < NV; ++p) {
-            double cp[NV];
+            double cpv[NV];
 #pragma unroll
-            for (int i = 0; i < NV; ++i) cp[i] = tbs::rdlane(col[i], P + p);
-            const double rp = recip(cp[p]);
+            for (int i = 0; i < NV; ++i) cpv[i] = tbs::rdlane(col[i], P + p);
+            const double rp = recip(cpv[p]);
             const double x = col[p] * rp;
 #pragma unroll
             for (int i = 0; i < NV; ++i)
-                if (i != p) col[i] -= cp[i] * x;
+                if (i != p) col[i] -= cpv[i] * x;
             col[p] = x;
         }
-        // outputs
+        // outputs: Gamma_k (X lanes, k <= n-2), E_k (A lanes, k >= 1), W_k
         const int w = k / SPW, st = k - w * SPW;
-        const long wb = wbase + (long)w * WL::STRIDE;
-        if (lane >= X && lane < X + NV) {
-            if (k <= n - 2) {
+        const int mm = m0 + k;
+        const bool isW = grp == 3;
+        const bool act = (grp == xg && k <= n - 2) || (grp == 2 && k >= 1) || isW;
+        const uintptr_t wdst = aW + (uintptr_t)((size_t)(mm / kGroup) * BS * kGroup * sizeof(TF));
+        const uintptr_t fdst = aEF + (uintptr_t)(grp == xg) * dGB +
+                               (uintptr_t)((wbase + (long)w * WL::STRIDE) * (long)sizeof(TF));
+        using gdst = __attribute__((address_space(1))) TF *;
+        const gdst dst = (gdst)(TF *)(isW ? wdst : fdst);
+        const int sc = isW ? mm % kGroup : st;
+        if (act) {
 #pragma unroll
-                for (int i = 0; i < NV; ++i) GB[wb + WL::index(lane - X, st * NV + i)] = (TF)col[i];
-            }
-        } else if (lane >= 2 * NV && lane < 3 * NV) {
-            if (k >= 1) {
-#pragma unroll
-                for (int i = 0; i < NV; ++i) EF[wb + WL::index(lane - 2 * NV, st * NV + i)] = (TF)col[i];
-            }
-        } else if (lane >= 3 * NV && lane < 4 * NV) {
-            const int mm = m0 + k, j = lane - 3 * NV;
-            TF *Wg = W + (size_t)(mm / kGroup) * BS * kGroup;
-#pragma unroll
-            for (int i = 0; i < NV; ++i) Wg[WS::index(i + NV * j, mm % kGroup)] = (TF)col[i];
+            for (int i = 0; i < NV; ++i) dst[u[i] + sc * v[i]] = (TF)col[i];
         }
     };
 
@@ -789,9 +833,9 @@ __global__ __launch_bounds__(64) void ffactor_kernel(int nline, const int *__res
         if (s == 0 && c > 0) {
             __syncthreads();                            // LDS of chunk c-1 read
             store_blocks(c & 1, R);                     // chunk c
-            if (lane <= CH) sidx[(c - 1) & 1][lane] = xnext;   // indices of chunk c+1
+            set_ptrs((c + 1) & 1, xnext);               // sources of chunk c+1
             __syncthreads();
-            load_blocks(c + 1, sidx[(c - 1) & 1], R);
+            load_blocks((c + 1) & 1, R);
             xnext = load_idx(c + 2);
         }
         step(std::integral_constant<int, 0>(), c & 1, s, k);
@@ -1513,9 +1557,9 @@ int tbs_plan_factor(tbs_plan *p, const double *jacD, const double *jacO1, const 
     // residual blocks
     {
         const long nws = (long)p->ngroups * p->S;
-        if (nws < INT_MAX) {
+        if (2 * nws < INT_MAX) {
             switch (nv) {
-#define X(v) case v: hipLaunchKernelGGL(pack_lds_kernel<v>, dim3((unsigned)nws), dim3(256), 0, st, \
+#define X(v) case v: hipLaunchKernelGGL(pack_lds_kernel<v>, dim3((unsigned)(2 * nws)), dim3(128), 0, st, \
                        p->S, p->d_src, jacD, jacO1, jacO2, p->d_blk); break;
                 TBS_PLAN_NVARS(X)
 #undef X
@@ -1539,10 +1583,10 @@ int tbs_plan_factor(tbs_plan *p, const double *jacD, const double *jacO1, const 
     switch (nv) {
 #define X(v) case v: \
         if (p->prec == TBS_FP32) hipLaunchKernelGGL((ffactor_kernel<v, float>), dim3(p->nline), dim3(64), 0, st, \
-            p->nline, p->d_linepoint, p->d_linesize, p->d_woff, p->d_fidx, jacD, jacO1, jacO2, \
+            p->nline, p->d_linepoint, p->d_linesize, p->d_woff, p->d_fidx, jacD, jacO1, jacO2, p->a_rhs, \
             (float *)p->d_W, (float *)p->d_EF, (float *)p->d_GB); \
         else hipLaunchKernelGGL((ffactor_kernel<v, double>), dim3(p->nline), dim3(64), 0, st, \
-            p->nline, p->d_linepoint, p->d_linesize, p->d_woff, p->d_fidx, jacD, jacO1, jacO2, \
+            p->nline, p->d_linepoint, p->d_linesize, p->d_woff, p->d_fidx, jacD, jacO1, jacO2, p->a_rhs, \
             (double *)p->d_W, (double *)p->d_EF, (double *)p->d_GB); break;
         TBS_PLAN_NVARS(X)
 #undef X
