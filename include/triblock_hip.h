@@ -169,7 +169,9 @@ int tbs_plan_info_get(tbs_plan *plan, tbs_plan_info *info);
  * 58-182) carried out in inverse form by one fused kernel that writes the
  * sweep-form factors W = D'^-1, E = D'^-1 A and Gamma = D'^-1 C directly.
  * Asynchronous on the context's stream: the Jacobian arrays must stay valid
- * until the stream has run it. */
+ * until the stream has run it.  (The residual blocks are packed on a second
+ * stream of the context at the same time; the context's stream waits for
+ * them, so work queued after the call sees the whole factorisation.) */
 int tbs_plan_factor(tbs_plan *plan, const double *jacD, const double *jacO1,
                     const double *jacO2);
 /* Solve J U = -rhs by LIJ sweeps from the initial U (device, reference

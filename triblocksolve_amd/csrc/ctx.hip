@@ -54,6 +54,16 @@ int dev_alloc(tbs_ctx *ctx, size_t bytes, void **ptr, bool *from_region) {
     return TBS_OK;
 }
 
+int aux_stream(tbs_ctx *ctx, hipStream_t *out) {
+    if (!ctx->aux) {
+        TBS_HIP(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking));
+        TBS_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+        TBS_HIP(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+    }
+    *out = ctx->aux;
+    return TBS_OK;
+}
+
 }  // namespace tbs
 
 extern "C" {
@@ -94,6 +104,12 @@ int tbs_ctx_destroy(tbs_ctx *ctx) {
     if (!ctx) return TBS_OK;
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    if (ctx->aux) {
+        hipStreamSynchronize(ctx->aux);
+        hipStreamDestroy(ctx->aux);
+    }
+    if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
     if (ctx->red_buf) hipFree(ctx->red_buf);
     if (ctx->red_host) hipHostFree(ctx->red_host);
     if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);

@@ -1554,32 +1554,14 @@ int tbs_plan_factor(tbs_plan *p, const double *jacD, const double *jacO1, const 
     const int nv = p->nv, BS = nv * nv;
     const size_t G = (size_t)p->ngroups * kGroup;
     const size_t tf = tf_size(p->prec);
-    // residual blocks
-    {
-        const long nws = (long)p->ngroups * p->S;
-        if (2 * nws < INT_MAX) {
-            switch (nv) {
-#define X(v) case v: hipLaunchKernelGGL(pack_lds_kernel<v>, dim3((unsigned)(2 * nws)), dim3(128), 0, st, \
-                       p->S, p->d_src, jacD, jacO1, jacO2, p->d_blk); break;
-                TBS_PLAN_NVARS(X)
-#undef X
-            }
-            TBS_LAUNCHED("pack_lds_kernel");
-        } else {
-            const long total = (long)G * p->S * BS;
-            switch (nv) {
-#define X(v) case v: hipLaunchKernelGGL(pack_blocks_kernel<v>, dim3(gs_grid(total)), dim3(256), 0, st, \
-                       total, p->S, p->d_src, jacD, jacO1, jacO2, p->d_blk); break;
-                TBS_PLAN_NVARS(X)
-#undef X
-            }
-            TBS_LAUNCHED("pack_blocks_kernel");
-        }
-    }
     // sweep-form factors W, E, Gamma straight from the blocks; padding stays zero
     TBS_HIP(hipMemsetAsync(p->d_EF, 0, (size_t)p->nwin * win_stride(nv, p->prec) * tf, st));
     TBS_HIP(hipMemsetAsync(p->d_GB, 0, (size_t)p->nwin * win_stride(nv, p->prec) * tf, st));
     TBS_HIP(hipMemsetAsync(p->d_W, 0, G * BS * tf, st));
+    hipStream_t pst;
+    TBS_TRY(tbs::aux_stream(ctx, &pst));
+    TBS_HIP(hipEventRecord(ctx->ev_fork, st));
+    TBS_HIP(hipStreamWaitEvent(pst, ctx->ev_fork, 0));
     switch (nv) {
 #define X(v) case v: \
         if (p->prec == TBS_FP32) hipLaunchKernelGGL((ffactor_kernel<v, float>), dim3(p->nline), dim3(64), 0, st, \
@@ -1592,6 +1574,34 @@ int tbs_plan_factor(tbs_plan *p, const double *jacD, const double *jacO1, const 
 #undef X
     }
     TBS_LAUNCHED("ffactor_kernel");
+    // residual blocks, packed on the context's second stream while the
+    // factor kernel (latency-bound, one wave per line) runs on the main one:
+    // queued after the factor kernel (its waves take their CUs first) and
+    // forked after the memsets; the main stream waits for the packing before
+    // anything after the call
+    {
+        const long nws = (long)p->ngroups * p->S;
+        if (2 * nws < INT_MAX) {
+            switch (nv) {
+#define X(v) case v: hipLaunchKernelGGL(pack_lds_kernel<v>, dim3((unsigned)(2 * nws)), dim3(128), 0, pst, \
+                       p->S, p->d_src, jacD, jacO1, jacO2, p->d_blk); break;
+                TBS_PLAN_NVARS(X)
+#undef X
+            }
+            TBS_LAUNCHED("pack_lds_kernel");
+        } else {
+            const long total = (long)G * p->S * BS;
+            switch (nv) {
+#define X(v) case v: hipLaunchKernelGGL(pack_blocks_kernel<v>, dim3(gs_grid(total)), dim3(256), 0, pst, \
+                       total, p->S, p->d_src, jacD, jacO1, jacO2, p->d_blk); break;
+                TBS_PLAN_NVARS(X)
+#undef X
+            }
+            TBS_LAUNCHED("pack_blocks_kernel");
+        }
+    }
+    TBS_HIP(hipEventRecord(ctx->ev_join, pst));
+    TBS_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
     p->factored = true;
     return TBS_OK;
 }

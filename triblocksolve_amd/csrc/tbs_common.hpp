@@ -25,11 +25,19 @@ struct tbs_ctx {
     // one request's admission.  Region memory is never freed piecewise.
     char *region = nullptr;
     size_t region_size = 0, region_used = 0;
+    // Second stream (default priority, created on first use) for work that
+    // runs beside the main stream inside one call and joins back before the
+    // call's later work on the main stream (the factorisation's block packing).
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace tbs {
 
 void set_error(const char *fmt, ...);
+
+// The context's second stream and its fork/join events, created on first use.
+int aux_stream(tbs_ctx *ctx, hipStream_t *out);
 
 // Record a HIP failure and return its positive status.
 int hip_fail(hipError_t e, const char *what, const char *file, int line);
