@@ -4,6 +4,7 @@
 // and owns the launch queue; there is no JIT and no MPI.
 #include "tbs_common.hpp"
 
+#include <algorithm>
 #include <cstring>
 
 namespace tbs {
@@ -64,6 +65,31 @@ int aux_stream(tbs_ctx *ctx, hipStream_t *out) {
     return TBS_OK;
 }
 
+int stage_acquire(tbs_ctx *ctx, size_t bytes, char **out) {
+    const int i = ctx->stage_cur;
+    if (ctx->stage_busy[i]) TBS_HIP(hipEventSynchronize(ctx->stage_ev[i]));
+    ctx->stage_busy[i] = false;
+    if (bytes > ctx->stage_cap[i]) {
+        const size_t cap = std::max(bytes, ctx->stage_cap[i] + ctx->stage_cap[i] / 2);
+        if (ctx->stage[i]) TBS_HIP(hipHostFree(ctx->stage[i]));
+        ctx->stage[i] = nullptr;
+        ctx->stage_cap[i] = 0;
+        TBS_HIP(hipHostMalloc((void **)&ctx->stage[i], cap, hipHostMallocDefault));
+        ctx->stage_cap[i] = cap;
+    }
+    if (!ctx->stage_ev[i]) TBS_HIP(hipEventCreateWithFlags(&ctx->stage_ev[i], hipEventDisableTiming));
+    *out = ctx->stage[i];
+    return TBS_OK;
+}
+
+int stage_release(tbs_ctx *ctx) {
+    const int i = ctx->stage_cur;
+    TBS_HIP(hipEventRecord(ctx->stage_ev[i], ctx->stream));
+    ctx->stage_busy[i] = true;
+    ctx->stage_cur = (i + 1) % tbs_ctx::kStages;
+    return TBS_OK;
+}
+
 }  // namespace tbs
 
 extern "C" {
@@ -110,6 +136,10 @@ int tbs_ctx_destroy(tbs_ctx *ctx) {
     }
     if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
+    for (int i = 0; i < tbs_ctx::kStages; ++i) {
+        if (ctx->stage_ev[i]) hipEventDestroy(ctx->stage_ev[i]);
+        if (ctx->stage[i]) hipHostFree(ctx->stage[i]);
+    }
     if (ctx->red_buf) hipFree(ctx->red_buf);
     if (ctx->red_host) hipHostFree(ctx->red_host);
     if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);

@@ -843,6 +843,91 @@ __global__ __launch_bounds__(64) void ffactor_kernel(int nline, const int *__res
     }
 }
 
+// ------------------------------------------------------- plan tables --
+// The derived tables of a plan, built on the device from the mesh arrays
+// (uploaded as they are): per line element the factor sources fidx and the
+// line index; per group lane the residual block sources src and neighbour
+// indices nbr; per group slot the neighbour map slotc.
+
+// One block per line.  fidx = {e, A source, C source}: A_k = Of2, C_k = Of1
+// at face f_k when element k is the face's first element, else the reverse
+// (src/okl/lineLU_v2.okl:116-117); source = 2f + 1 for Of2, 2f for Of1.
+__global__ void tab_lines_kernel(const int *__restrict__ linepoint, const int *__restrict__ linesize,
+                                 const int *__restrict__ woff, const int *__restrict__ lines,
+                                 const int *__restrict__ lineface, const int *__restrict__ fc,
+                                 int *__restrict__ mline, int4 *__restrict__ fidx, int4 *__restrict__ ltab) {
+    const int l = blockIdx.x;
+    const int m0 = linepoint[l], n = linesize[l];
+    if (threadIdx.x == 0) ltab[l] = make_int4(0, m0, n, woff[l]);
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const int mm = m0 + k, e = lines[mm];
+        int a = -1, c = -1;
+        if (k >= 1) {
+            const int f = lineface[mm];
+            const bool first = e == fc[2 * f];
+            a = 2 * f + (first ? 1 : 0);
+            c = 2 * f + (first ? 0 : 1);
+        }
+        mline[mm] = l;
+        fidx[mm] = make_int4(e, a, c, 0);
+    }
+}
+
+// inv[lines[i]] = i: internal position of every element.
+__global__ void tab_inv_kernel(int nle, const int *__restrict__ lines, int *__restrict__ inv) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nle; i += gridDim.x * blockDim.x) inv[lines[i]] = i;
+}
+
+// One thread per group lane mm < G: src row [S] = {0 none, 1 D, 2 Of1, 3 Of2;
+// index} (slot 0 = D, slot 1 + s = the element's s-th face in ef order: Of2
+// when the element is the face's first element), nbr[g][s][lane] = internal
+// index of the neighbour across that face or -1; lanes past the last
+// element stay empty.
+__global__ void tab_src_kernel(long G, int nle, int S, int Fm, const int *__restrict__ lines,
+                               const int *__restrict__ epoint, const int *__restrict__ ef,
+                               const int *__restrict__ fc, const int *__restrict__ inv, int2 *__restrict__ src,
+                               int *__restrict__ nbr) {
+    for (long mm = (long)blockIdx.x * blockDim.x + threadIdx.x; mm < G; mm += (long)gridDim.x * blockDim.x) {
+        const long g = mm / kGroup;
+        const int lane = (int)(mm % kGroup);
+        int2 *row = src + mm * S;
+        for (int q = 0; q < S; ++q) row[q] = make_int2(0, 0);
+        for (int s2 = 0; s2 < Fm; ++s2) nbr[(g * Fm + s2) * kGroup + lane] = -1;
+        if (mm >= nle) continue;
+        const int e = lines[mm];
+        row[0] = make_int2(1, e);
+        int s2 = 0;
+        for (int k2 = epoint[e]; k2 < epoint[e + 1]; ++k2, ++s2) {
+            const int f = ef[k2];
+            if (f < 0) continue;
+            const int e1 = fc[2 * f], e2 = fc[2 * f + 1];
+            row[1 + s2] = make_int2(e == e1 ? 3 : 2, f);
+            nbr[(g * Fm + s2) * kGroup + lane] = inv[e == e1 ? e2 : e1];
+        }
+    }
+}
+
+// One thread per (group, slot): the slot's neighbour map is the shift c when
+// every lane with a face there has nbr = m + c, kSlotGather when they differ,
+// kSlotNone when no lane has the face.  Also the group table.
+__global__ void tab_slot_kernel(int ngroups, int F, int Fm, int nle, const int *__restrict__ nbr,
+                                int *__restrict__ slotc, int2 *__restrict__ gtab) {
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < (long)ngroups * Fm;
+         t += (long)gridDim.x * blockDim.x) {
+        const int g = (int)(t / Fm), f = (int)(t % Fm);
+        int c = kSlotNone;
+        for (int lane = 0; lane < kGroup && f < F; ++lane) {
+            const int mm = g * kGroup + lane;
+            const int nb = nbr[((long)g * Fm + f) * kGroup + lane];
+            if (mm >= nle || nb < 0) continue;
+            if (c == kSlotNone) c = nb - mm;
+            else if (c != nb - mm) { c = kSlotGather; break; }
+        }
+        slotc[t] = c;
+        if (f == 0) gtab[g] = make_int2(0, g);
+    }
+}
+
 // Vector gathers between reference order (element id) and internal order.
 __global__ void gather_vec_kernel(long total, int nv, const int *__restrict__ lines,
                                   const double *__restrict__ src, double *__restrict__ dst) {
@@ -1032,8 +1117,11 @@ Layout layout_of(const tbs_mesh_host *m) {
 
 // Packed integer tables of a plan (one device array): byte offsets, each
 // 16-byte aligned, and the total.
+// The arrays up to host_end come from the host (mesh arrays as given, the
+// window offsets and the system descriptor); the rest are built on the device.
 struct TableLayout {
-    size_t lines, mline, linepoint, linesize, woff, fc, lineface, src, fidx, nbr, slotc, ltab, gtab, sys, total;
+    size_t lines, linepoint, linesize, woff, fc, lineface, epoint, ef, sys, host_end;
+    size_t mline, src, fidx, nbr, slotc, ltab, gtab, total;
 };
 
 TableLayout table_layout(const tbs_mesh_host *m, const Layout &L, int prec) {
@@ -1043,19 +1131,22 @@ TableLayout table_layout(const tbs_mesh_host *m, const Layout &L, int prec) {
     const size_t G = (size_t)L.ngroups * kGroup, Fm = (size_t)std::max(L.F, 1);
     const size_t nle = m->nlineelem, nl = m->nline;
     t.lines = take(4 * nle);
-    t.mline = take(4 * nle);
     t.linepoint = take(4 * (nl + 1));
     t.linesize = take(4 * nl);
     t.woff = take(4 * (nl + 1));
     t.fc = take(4 * std::max<size_t>(2 * (size_t)m->nintface, 1));
     t.lineface = take(4 * nle);
+    t.epoint = take(4 * ((size_t)m->nelem + 1));
+    t.ef = take(4 * std::max<size_t>((size_t)m->eftot, 1));
+    t.sys = take(prec == TBS_FP32 ? sizeof(SysDev<float>) : sizeof(SysDev<double>));
+    t.host_end = o;
+    t.mline = take(4 * nle);
     t.src = take(sizeof(int2) * G * L.S);
     t.fidx = take(sizeof(int4) * nle);
     t.nbr = take(4 * G * Fm);
     t.slotc = take(4 * (size_t)L.ngroups * Fm);
     t.ltab = take(sizeof(int4) * nl);
     t.gtab = take(sizeof(int2) * L.ngroups);
-    t.sys = take(prec == TBS_FP32 ? sizeof(SysDev<float>) : sizeof(SysDev<double>));
     t.total = o;
     return t;
 }
@@ -1345,19 +1436,34 @@ int tbs_plan_supported_nvar(int *out, int cap) {
 }
 
 int tbs_plan_footprint(const tbs_mesh_host *mesh, int precision, size_t *bytes) {
+    return tbs::plan_footprint(mesh, precision, bytes, true);
+}
+
+int tbs_plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_plan **out) {
+    return tbs::plan_create(ctx, m, precision, out, true);
+}
+
+}  // extern "C"
+
+int tbs::plan_footprint(const tbs_mesh_host *mesh, int precision, size_t *bytes, bool validate) {
     TBS_REQUIRE(bytes, TBS_ERR_ARG, "null output");
-    TBS_TRY(check_mesh(mesh));
+    if (validate) TBS_TRY(check_mesh(mesh));
+    else TBS_REQUIRE(mesh != nullptr && mesh->nelem >= 1 && mesh->nline >= 1, TBS_ERR_ARG, "bad mesh sizes");
     TBS_REQUIRE(precision == TBS_FP64 || precision == TBS_FP32, TBS_ERR_ARG, "bad precision");
     Layout L = layout_of(mesh);
     *bytes = footprint_of(mesh, L, precision);
     return TBS_OK;
 }
 
-int tbs_plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_plan **out) {
+// Plan creation; validate = false skips the O(mesh) consistency checks for
+// meshes the library built itself (a served request's structured grid).
+int tbs::plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_plan **out, bool validate) {
     TBS_TRY(tbs::use_ctx(ctx));
     TBS_REQUIRE(out, TBS_ERR_ARG, "null output");
     *out = nullptr;
-    TBS_TRY(check_mesh(m));
+    if (validate) TBS_TRY(check_mesh(m));
+    else TBS_REQUIRE(m != nullptr && m->nelem >= 1 && m->nline >= 1 && m->nlineelem == m->nelem, TBS_ERR_ARG,
+                     "bad mesh sizes");
     TBS_REQUIRE(m->nvar == ctx->nvar, TBS_ERR_ARG, "mesh nvar %d != context nvar %d", m->nvar, ctx->nvar);
     TBS_REQUIRE(plan_nvar_ok(m->nvar), TBS_ERR_UNSUPPORTED, "nvar %d not built for the fused path", m->nvar);
     TBS_REQUIRE(precision == TBS_FP64 || precision == TBS_FP32, TBS_ERR_ARG, "bad precision");
@@ -1423,10 +1529,19 @@ int tbs_plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pla
     if (P->in_region) P->h_state = new int[4]();
     else TBS_HIP(hipHostMalloc((void **)&P->h_state, 4 * sizeof(int), hipHostMallocDefault));
 
-    // host image of the tables, built in place (every entry written once) and
-    // kept by the plan so the upload stays asynchronous
-    P->h_tables.reset(new char[T.total]);
-    char *h = P->h_tables.get();
+    // host part of the tables (the mesh arrays as given): small plans (a
+    // served request's) go through the context's pinned staging buffer,
+    // reused from plan to plan, so the upload is one DMA; large ones through
+    // a pageable image kept by the plan so the upload stays asynchronous
+    constexpr size_t kStageMax = (size_t)64 << 20;
+    const bool staged = T.host_end <= kStageMax;
+    char *h = nullptr;
+    if (staged) {
+        TBS_TRY(tbs::stage_acquire(ctx, T.host_end, &h));
+    } else {
+        P->h_tables.reset(new char[T.host_end]);
+        h = P->h_tables.get();
+    }
     const int nle = P->nle, F = L.F, Fm = std::max(F, 1);
     auto put = [h](size_t off, const void *src_, size_t bytes) { if (bytes) std::memcpy(h + off, src_, bytes); };
     put(T.lines, m->lines, (size_t)nle * 4);
@@ -1435,65 +1550,8 @@ int tbs_plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pla
     put(T.woff, L.woff.data(), (size_t)(P->nline + 1) * 4);
     put(T.fc, m->fc, (size_t)2 * m->nintface * 4);
     put(T.lineface, m->lineface, (size_t)nle * 4);
-    int *mline = (int *)(h + T.mline);
-    int4 *fidx = (int4 *)(h + T.fidx);
-    int4 *ltab = (int4 *)(h + T.ltab);
-    for (int l = 0; l < P->nline; ++l) {
-        const int m0 = m->linepoint[l], n = m->linesize[l];
-        ltab[l] = make_int4(0, m0, n, L.woff[l]);
-        for (int k = 0; k < n; ++k) {
-            // factor sources: A_k = Of2, C_k = Of1 at face f_k when element k is
-            // the face's first element, else the reverse (src/okl/lineLU_v2.okl:116-117)
-            const int mm = m0 + k, e = m->lines[mm];
-            int a = -1, c = -1;
-            if (k >= 1) {
-                const int f = m->lineface[mm];
-                const bool first = e == m->fc[2 * f];
-                a = 2 * f + (first ? 1 : 0);
-                c = 2 * f + (first ? 0 : 1);
-            }
-            mline[mm] = l;
-            fidx[mm] = make_int4(e, a, c, 0);
-        }
-    }
-    std::vector<int> inv(P->nelem);
-    for (int i = 0; i < nle; ++i) inv[m->lines[i]] = i;
-    // residual sources and neighbour table, [group][slot][lane]; pad lanes empty
-    int2 *src = (int2 *)(h + T.src);
-    int *nbr = (int *)(h + T.nbr);
-    for (size_t mm = 0; mm < G; ++mm) {
-        const size_t g = mm / kGroup, lane = mm % kGroup;
-        int2 *row = src + mm * L.S;
-        for (int q = 0; q < L.S; ++q) row[q] = make_int2(0, 0);
-        for (int s2 = 0; s2 < Fm; ++s2) nbr[(g * Fm + s2) * kGroup + lane] = -1;
-        if ((long)mm >= nle) continue;
-        const int e = m->lines[mm];
-        row[0] = make_int2(1, e);
-        int s2 = 0;
-        for (int k2 = m->epoint[e]; k2 < m->epoint[e + 1]; ++k2, ++s2) {
-            const int f = m->ef[k2];
-            if (f < 0) continue;
-            const int e1 = m->fc[2 * f], e2 = m->fc[2 * f + 1];
-            row[1 + s2] = make_int2(e == e1 ? 3 : 2, f);
-            nbr[(g * Fm + s2) * kGroup + lane] = inv[e == e1 ? e2 : e1];
-        }
-    }
-    // slot maps: affine shift when every lane with a face has nbr = m + c
-    int *slotc = (int *)(h + T.slotc);
-    for (int g = 0; g < L.ngroups; ++g)
-        for (int f = 0; f < Fm; ++f) {
-            int c = kSlotNone;
-            for (int lane = 0; lane < kGroup && f < F; ++lane) {
-                const int mm = g * kGroup + lane;
-                const int nb = nbr[((size_t)g * Fm + f) * kGroup + lane];
-                if (mm >= nle || nb < 0) continue;
-                if (c == kSlotNone) c = nb - mm;
-                else if (c != nb - mm) { c = kSlotGather; break; }
-            }
-            slotc[(size_t)g * Fm + f] = c;
-        }
-    int2 *gtab = (int2 *)(h + T.gtab);
-    for (int g = 0; g < L.ngroups; ++g) gtab[g] = make_int2(0, g);
+    put(T.epoint, m->epoint, (size_t)(P->nelem + 1) * 4);
+    put(T.ef, m->ef, (size_t)m->eftot * 4);
     if (precision == TBS_FP32) {
         SysDev<float> sd = make_sysdev<float>(P);
         put(T.sys, &sd, sizeof(sd));
@@ -1502,11 +1560,31 @@ int tbs_plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pla
         put(T.sys, &sd, sizeof(sd));
     }
     hipStream_t st = ctx->stream;
-    TBS_HIP(hipMemcpyAsync(tb, h, T.total, hipMemcpyHostToDevice, st));
+    TBS_HIP(hipMemcpyAsync(tb, h, T.host_end, hipMemcpyHostToDevice, st));
+    if (staged) TBS_TRY(tbs::stage_release(ctx));
+    // derived tables on the device; the R vector (zeroed below) holds the
+    // inverse permutation meanwhile
+    {
+        const int *d_epoint = (const int *)(tb + T.epoint), *d_ef = (const int *)(tb + T.ef);
+        int *inv = (int *)P->a_R;
+        hipLaunchKernelGGL(tab_lines_kernel, dim3(P->nline), dim3(64), 0, st, P->d_linepoint, P->d_linesize,
+                           P->d_woff, P->d_lines, P->d_lineface, P->d_fc, P->d_mline, P->d_fidx, P->d_ltab);
+        TBS_LAUNCHED("tab_lines_kernel");
+        hipLaunchKernelGGL(tab_inv_kernel, dim3(gs_grid(nle)), dim3(256), 0, st, nle, P->d_lines, inv);
+        TBS_LAUNCHED("tab_inv_kernel");
+        hipLaunchKernelGGL(tab_src_kernel, dim3(gs_grid((long)G)), dim3(256), 0, st, (long)G, nle, L.S, Fm,
+                           P->d_lines, d_epoint, d_ef, P->d_fc, inv, P->d_src, P->d_nbr);
+        TBS_LAUNCHED("tab_src_kernel");
+        hipLaunchKernelGGL(tab_slot_kernel, dim3(gs_grid((long)L.ngroups * Fm)), dim3(256), 0, st, L.ngroups, F,
+                           Fm, nle, P->d_nbr, P->d_slotc, P->d_gtab);
+        TBS_LAUNCHED("tab_slot_kernel");
+    }
     for (double *a : {P->a_rhs, P->a_U, P->a_R}) TBS_HIP(hipMemsetAsync(a, 0, sz[A_RHS], st));
     *out = p.release();
     return TBS_OK;
 }
+
+extern "C" {
 
 int tbs_plan_destroy(tbs_plan *plan) {
     if (!plan) return TBS_OK;

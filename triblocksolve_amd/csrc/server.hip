@@ -89,7 +89,9 @@ struct Request {
     tbs_plan *plan = nullptr;
     double *d_rhs = nullptr, *d_U = nullptr;   // element order, in the slab
     hipEvent_t ready = nullptr;
-    std::vector<double> U, hist;
+    double *hU = nullptr;           // result U (pinned, from the server's pool)
+    size_t nU = 0, hU_cls = 0;
+    std::vector<double> hist;
     int nsweeps = 0, status = 0;
     double t_submit = 0, t_admit = 0, t_done = 0;
 };
@@ -134,7 +136,7 @@ tbs_mesh_host mesh_view(const Request &r, int nvar) {
 int request_sizes(const Request &r, int prec, int nvar, size_t *persist, size_t *total) {
     const tbs_mesh_host mh = mesh_view(r, nvar);
     size_t plan = 0;
-    TBS_TRY(tbs_plan_footprint(&mh, prec, &plan));
+    TBS_TRY(tbs::plan_footprint(&mh, prec, &plan, false));   // the server's own mesh
     const size_t bs = (size_t)nvar * nvar * 8, vec = (size_t)nvar * r.nelem * 8;
     *persist = align_up(plan + 2 * align_up(vec) + kSlack);
     *total = *persist + align_up(bs * r.nelem) + 2 * align_up(bs * std::max(r.nintface, 1)) +
@@ -146,6 +148,36 @@ enum { T_WAIT, T_MEMBERS, T_LAUNCH, T_RETIRE, T_ADMIT, T_PLAN, T_NPHASE };
 enum { C_STEPS, C_ADMITTED, C_RETIRED, C_REBUILDS, C_NCOUNT };
 
 }  // namespace
+
+// Pinned host buffers for results, in power-of-two size classes, reused
+// from request to request (a result copied into pageable memory costs page
+// faults and a staged copy on the serving thread).
+struct PinnedPool {
+    std::map<size_t, std::vector<double *>> free;
+    std::vector<double *> all;
+    int get(size_t bytes, double **p, size_t *cls) {
+        size_t c = 4096;
+        while (c < bytes) c <<= 1;
+        std::vector<double *> &v = free[c];
+        if (!v.empty()) {
+            *p = v.back();
+            v.pop_back();
+        } else {
+            TBS_HIP(hipHostMalloc((void **)p, c, hipHostMallocDefault));
+            all.push_back(*p);
+        }
+        *cls = c;
+        return TBS_OK;
+    }
+    void put(double *p, size_t cls) {
+        if (p) free[cls].push_back(p);
+    }
+    void clear() {
+        for (double *p : all) hipHostFree(p);
+        all.clear();
+        free.clear();
+    }
+};
 
 struct tbs_server {
     tbs_ctx *ctx = nullptr;   // batch stream
@@ -172,6 +204,7 @@ struct tbs_server {
     std::map<long, std::unique_ptr<Request>> finished;
     std::vector<long> completed_unpolled;
     tbs_batch *batch = nullptr;
+    PinnedPool pinned;
     double t_phase[T_NPHASE] = {};
     long count[C_NCOUNT] = {};
     double now() const { return std::chrono::duration<double>(clk::now() - t0).count(); }
@@ -202,7 +235,7 @@ int admit(tbs_server *s, Request *r) {
     int rc = TBS_OK;
     do {
         const double tp0 = s->now();
-        rc = tbs_plan_create(a, &mh, s->prec, &r->plan);
+        rc = tbs::plan_create(a, &mh, s->prec, &r->plan, false);   // the server's own mesh
         s->t_phase[T_PLAN] += s->now() - tp0;
         if (rc) break;
         if ((rc = tbs::dev_alloc(a, (size_t)nv * r->nelem * 8, (void **)&r->d_rhs, nullptr))) break;
@@ -297,6 +330,8 @@ int tbs_server_destroy(tbs_server *s) {
     for (auto &r : s->active) drop_device(s, r.get());
     for (auto &r : s->admitting) drop_device(s, r.get());
     for (char *b : s->seg_base) hipFree(b);
+    s->finished.clear();
+    s->pinned.clear();
     if (s->ret) tbs_ctx_destroy(s->ret);
     if (s->adm) tbs_ctx_destroy(s->adm);
     delete s;
@@ -382,10 +417,11 @@ int tbs_server_step(tbs_server *s, int *nactive, int *nqueued, int *nadmitting) 
     if (!done.empty()) {
         hipStream_t rs = s->ret->stream;
         for (auto &r : done) {
-            r->U.resize((size_t)nv * r->nelem);
+            r->nU = (size_t)nv * r->nelem;
+            TBS_TRY(s->pinned.get(r->nU * 8, &r->hU, &r->hU_cls));
             r->hist.resize(r->nsweeps + 1);
             TBS_TRY(tbs::plan_scatter_U(r->plan, rs, r->d_U));
-            TBS_HIP(hipMemcpyAsync(r->U.data(), r->d_U, r->U.size() * 8, hipMemcpyDeviceToHost, rs));
+            TBS_HIP(hipMemcpyAsync(r->hU, r->d_U, r->nU * 8, hipMemcpyDeviceToHost, rs));
             TBS_HIP(hipMemcpyAsync(r->hist.data(), tbs::plan_hist(r->plan), r->hist.size() * 8,
                                    hipMemcpyDeviceToHost, rs));
         }
@@ -458,8 +494,8 @@ int tbs_server_result(tbs_server *s, long id, double *U, size_t ucap, double *hi
     TBS_REQUIRE(it != s->finished.end(), TBS_ERR_ARG, "request %ld is not finished", id);
     Request *r = it->second.get();
     if (U) {
-        TBS_REQUIRE(ucap >= r->U.size(), TBS_ERR_ARG, "U capacity %zu < %zu", ucap, r->U.size());
-        std::memcpy(U, r->U.data(), r->U.size() * 8);
+        TBS_REQUIRE(ucap >= r->nU, TBS_ERR_ARG, "U capacity %zu < %zu", ucap, r->nU);
+        if (r->nU) std::memcpy(U, r->hU, r->nU * 8);
     }
     if (hist && hcap > 0) std::memcpy(hist, r->hist.data(), std::min<size_t>(hcap, r->hist.size()) * 8);
     if (nsweeps) *nsweeps = r->nsweeps;
@@ -474,7 +510,11 @@ int tbs_server_result(tbs_server *s, long id, double *U, size_t ucap, double *hi
 
 int tbs_server_release(tbs_server *s, long id) {
     TBS_REQUIRE(s, TBS_ERR_ARG, "null server");
-    s->finished.erase(id);
+    auto it = s->finished.find(id);
+    if (it != s->finished.end()) {
+        s->pinned.put(it->second->hU, it->second->hU_cls);
+        s->finished.erase(it);
+    }
     return TBS_OK;
 }
 

@@ -30,6 +30,16 @@ struct tbs_ctx {
     // call's later work on the main stream (the factorisation's block packing).
     hipStream_t aux = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // Pinned host staging for table uploads of small plans: a ring of
+    // buffers (grown on demand, reused; stage_ev[i] marks the end of the last
+    // upload read from buffer i), so building one plan's tables does not wait
+    // for the previous plan's upload, which is queued behind earlier work.
+    static constexpr int kStages = 3;
+    char *stage[kStages] = {};
+    size_t stage_cap[kStages] = {};
+    hipEvent_t stage_ev[kStages] = {};
+    bool stage_busy[kStages] = {};
+    int stage_cur = 0;
 };
 
 namespace tbs {
@@ -38,6 +48,12 @@ void set_error(const char *fmt, ...);
 
 // The context's second stream and its fork/join events, created on first use.
 int aux_stream(tbs_ctx *ctx, hipStream_t *out);
+
+// The context's pinned staging buffer, at least `bytes` long, once the last
+// upload from it has completed; the caller queues its upload on the
+// context's stream and then calls stage_release.
+int stage_acquire(tbs_ctx *ctx, size_t bytes, char **out);
+int stage_release(tbs_ctx *ctx);
 
 // Record a HIP failure and return its positive status.
 int hip_fail(hipError_t e, const char *what, const char *file, int line);
@@ -73,6 +89,11 @@ int dev_alloc(tbs_ctx *ctx, size_t bytes, void **ptr, bool *from_region);
 constexpr size_t kRegionAlign = 256;
 
 // Internal access to a plan for the serving engine (same library).
+// tbs_plan_create, with the mesh consistency checks skipped when !validate
+// (for meshes the library built itself).
+int plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_plan **out, bool validate);
+// tbs_plan_footprint, with the same choice.
+int plan_footprint(const tbs_mesh_host *mesh, int precision, size_t *bytes, bool validate);
 // Scatter the plan's solution into U (device, element order) on stream st.
 int plan_scatter_U(tbs_plan *p, hipStream_t st, double *U);
 // Device residual history [kHistCap] of a plan.
