@@ -26,6 +26,10 @@ def meshes():
     out.append(("ragged_nv9", ragged_lines([1, 2, 17, 64, 3, 100, 1, 65], 9)))
     out.append(("grid_4x6_nv3", system(4, 6, 3)[0]))
     out.append(("grid_3x20_nv16", system(3, 20, 16)[0]))
+    # block sizes past a 16-lane row: the paper's block-size study runs to ~30
+    out.append(("grid_3x9_nv17", system(3, 9, 17)[0]))
+    out.append(("ragged_nv22", ragged_lines([1, 7, 12, 3], 22)))
+    out.append(("grid_2x6_nv32", system(2, 6, 32)[0]))
     out.append(("grid_2x5_nv1", system(2, 5, 1)[0]))
     return out
 
