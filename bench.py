@@ -268,6 +268,69 @@ def bench_serving(tbs, ctx, args, nreq=160, seed=7):
     return out
 
 
+def bench_block_sizes(tbs, args, NL=600, NK=640, sweeps=4, nvars=(5, 9, 10, 16, 17, 22, 32)):
+    """Sweeps/s and GB/s against the block size on one 384,000-element grid (the
+    paper's block-size study, Fig. 8): the reference loop through the drop-in
+    kernels (solveDU, addAtoB, copyAtoB, lineRes: src/main.cxx:207-229) for
+    every size, the fused sweep where it is built (2..16)."""
+    import numpy as np
+    out = {"workload": f"grid_{NL}x{NK}_shift{{2nv+2}}", "nelem": NL * NK, "sweeps": sweeps, "by_nvar": {}}
+    plan_nv = set()
+    import ctypes
+    buf = (ctypes.c_int * 64)()
+    for i in range(tbs.load().tbs_plan_supported_nvar(buf, 64)):
+        plan_nv.add(buf[i])
+    for nv in nvars:
+        ctx = tbs.Context(0, nv)
+        m = tbs.structured_mesh(NL, NK, nv)
+        n, nif = m.nelem, m.nintface
+        z = lambda k: ctx.array(size=k, zero=True)
+        D, O1, O2, rhs = z(nv * nv * n), z(nv * nv * nif), z(nv * nv * nif), z(nv * n)
+        ctx.generate_device(11, n, nif, 2.0 * nv + 2, D, O1, O2, rhs)
+        d = m.to_device(ctx)
+        DLU, DinvC, Am, U, dU, R = (z(nv * nv * n), z(nv * nv * n), z(nv * nv * n), z(nv * n), z(nv * n), z(nv * n))
+        ctx.assembleTriBlocks(d, O1, O2, Am)
+        ctx.copyAtoBjac(n, D, DLU)
+        ctx.lineLU(n, nif, d, DLU, O1, O2, DinvC)
+        ctx.copyAtoB(n, rhs, R)
+
+        def sweep():
+            ctx.solveDU(n, nif, d, DLU, DinvC, Am, dU, R)
+            ctx.addAtoB(n, dU, U)
+            ctx.copyAtoB(n, rhs, R)
+            ctx.lineRes(n, nif, d, D, O1, O2, U, R)
+        sweep()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(sweeps):
+            sweep()
+        ctx.sync()
+        t_ref = (time.perf_counter() - t0) / sweeps
+        ref_bytes = tbs.reference_model_bytes(m)
+        row = {"dropin_sweeps_per_s": 1.0 / t_ref, "dropin_gbps_reference_model": ref_bytes / t_ref / 1e9}
+        if nv in plan_nv:
+            plan = tbs.Plan(ctx, m, tbs.TBS_FP64)
+            plan.factor(D, O1, O2)
+            plan.begin(rhs, None)
+            plan.launch_thomas()
+            plan.launch_residual()
+            ctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(sweeps):
+                plan.launch_thomas()
+                plan.launch_residual()
+            ctx.sync()
+            t_f = (time.perf_counter() - t0) / sweeps
+            row["fused_sweeps_per_s"] = 1.0 / t_f
+            row["fused_gbps_model"] = plan.info()["alg_bytes_per_sweep"] / t_f / 1e9
+            plan.close()
+        out["by_nvar"][str(nv)] = row
+        for x in (D, O1, O2, rhs, DLU, DinvC, Am, U, dU, R, *d.values()):
+            x.close()
+        ctx.close()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -412,6 +475,7 @@ def main():
         out["gmres"] = bench_gmres(tbs, ctx, m, args)
         out["batched"] = bench_batched(tbs, ctx, args)
         out["serving"] = bench_serving(tbs, ctx, args)
+        out["block_sizes"] = bench_block_sizes(tbs, args)
     if not args.no_cpu:
         cb = cpu_baseline(tbs, args, n)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
