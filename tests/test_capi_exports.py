@@ -49,3 +49,10 @@ def test_plan_footprint_is_host_side(tbs):
     fp64 = tbs.Plan.footprint(m, tbs.TBS_FP64)
     fp32 = tbs.Plan.footprint(m, tbs.TBS_FP32)
     assert 8e9 < fp64 < 20e9 and fp32 < fp64
+
+
+def test_variant_names(tbs):
+    """Thomas variant codes (100 * prefetch windows + ring cycles) name the
+    compiled template instance the autotuner reports."""
+    assert tbs.variant_name(302) == "pf3_k2"
+    assert tbs.variant_name(1001) == "pf10_k1"
