@@ -121,10 +121,9 @@ def bench_batched(tbs, ctx, args, seed=3):
     ctx.sync()
     for prec, name in ((tbs.TBS_FP64, "fp64"), (tbs.TBS_FP32, "fp32_factor")):
         plans = [tbs.Plan(ctx, m, prec) for m in meshes]
-        t0 = time.perf_counter()
-        for p, (D, O1, O2, _) in zip(plans, bufs):
-            p.factor(D, O1, O2)
         ctx.sync()
+        t0 = time.perf_counter()
+        tbs.factor_plans(ctx, plans, *[[b[k] for b in bufs] for k in range(3)])   # one launch for all
         t_factor = time.perf_counter() - t0
         b = tbs.Batch(ctx, plans)
         chosen, us = b.autotune(reps=3)

@@ -174,6 +174,14 @@ int tbs_plan_info_get(tbs_plan *plan, tbs_plan_info *info);
  * them, so work queued after the call sees the whole factorisation.) */
 int tbs_plan_factor(tbs_plan *plan, const double *jacD, const double *jacO1,
                     const double *jacO2);
+/* Factorise n plans of one context (same block size and precision) in one
+ * launch of the factor kernel over all their lines, longest first, plus one
+ * packing launch per plan: jacD[i], jacO1[i], jacO2[i] are plan i's device
+ * arrays.  The same factors, bitwise, as n calls of tbs_plan_factor, in the
+ * time of the longest line rather than the sum over plans.  No counterpart
+ * in the reference (one system per run); for batches of systems. */
+int tbs_plans_factor(tbs_ctx *ctx, tbs_plan *const *plans, int n, const double *const *jacD,
+                     const double *const *jacO1, const double *const *jacO2);
 /* Solve J U = -rhs by LIJ sweeps from the initial U (device, reference
  * layout, overwritten with the result).  Stops after max_sweeps or when
  * ||R_k|| <= rtol * ||R_0||.  hist (host, >= max_sweeps+1) gets ||R_k||_2,

@@ -27,17 +27,22 @@ def _solve_alone(tbs, ctx, m, j, prec, iters, rtol):
     return hist, ns, host(U)
 
 
+@pytest.mark.parametrize("one_launch", [False, True], ids=["factor_each", "factor_plans"])
 @pytest.mark.parametrize("prec", [0, 1])
-def test_uneven_batch_autotuned_graph(tbs, ctx, oracle, prec):
-    print(f"case uneven batch prec {prec}", flush=True)
+def test_uneven_batch_autotuned_graph(tbs, ctx, oracle, prec, one_launch):
+    """The systems of a batch, factored one by one or all in one launch
+    (tbs_plans_factor), solve bitwise as each does alone (factored alone)."""
+    print(f"case uneven batch prec {prec} one_launch {one_launch}", flush=True)
     shapes = tbs.batch_shapes(5, nsys=8, total=30000, lmin=96, lmax=1280, nlmin=3, nlmax=9)
     meshes = [tbs.ragged_mesh(s) for s in shapes]
     jacs = [tbs.generate_host(100 + i, 9, m.nelem, m.nintface, 20.0) for i, m in enumerate(meshes)]
-    plans = []
-    for m, j in zip(meshes, jacs):
-        p = tbs.Plan(ctx, m, prec)
-        p.factor(dev(j["jacD"]), dev(j["jacO1"]), dev(j["jacO2"]))
-        plans.append(p)
+    plans = [tbs.Plan(ctx, m, prec) for m in meshes]
+    devj = [(dev(j["jacD"]), dev(j["jacO1"]), dev(j["jacO2"])) for j in jacs]
+    if one_launch:
+        tbs.factor_plans(ctx, plans, *zip(*devj))
+    else:
+        for p, (D, O1, O2) in zip(plans, devj):
+            p.factor(D, O1, O2)
     b = tbs.Batch(ctx, plans)
     chosen, us = b.autotune(reps=2)
     Us = [zeros(9 * m.nelem) for m in meshes]
@@ -199,3 +204,15 @@ def test_server_rejects_bad_requests(tbs, ctx):
         srv.result(12345)
     assert ei.value.code == -1
     srv.close()
+
+
+def test_plans_factor_rejects_mixed_plans(tbs, ctx):
+    """One launch needs one block size and one precision."""
+    m = tbs.ragged_mesh([5, 9])
+    j = tbs.generate_host(3, 9, m.nelem, m.nintface, 20.0)
+    a, b = tbs.Plan(ctx, m, 0), tbs.Plan(ctx, m, 1)
+    D, O1, O2 = dev(j["jacD"]), dev(j["jacO1"]), dev(j["jacO2"])
+    with pytest.raises(tbs.TbsError):
+        tbs.factor_plans(ctx, [a, b], [D, D], [O1, O1], [O2, O2])
+    a.close()
+    b.close()

@@ -10,8 +10,10 @@ falling back to anything.
 from ._capi import (TBS_FP32, TBS_FP64, TbsError, load, LIB_PATH)  # noqa: F401
 from .solver import (Context, DeviceBuffer, StructuredMesh, HostMesh, Plan, Batch, Server, Gmres,  # noqa: F401
                      generate_host, structured_mesh, ragged_mesh, batch_shapes, read_mesh, write_mesh,
-                     read_jacobian, write_jacobian, model_bytes, reference_model_bytes, variant_name)
+                     read_jacobian, write_jacobian, model_bytes, reference_model_bytes, variant_name,
+                     factor_plans)
 
 __all__ = ["Context", "DeviceBuffer", "StructuredMesh", "HostMesh", "Plan", "Batch", "Server", "Gmres", "TBS_FP32", "TBS_FP64",
            "TbsError", "load", "generate_host", "structured_mesh", "ragged_mesh", "batch_shapes", "read_mesh", "write_mesh",
-           "read_jacobian", "write_jacobian", "model_bytes", "reference_model_bytes", "variant_name"]
+           "read_jacobian", "write_jacobian", "model_bytes", "reference_model_bytes", "variant_name",
+           "factor_plans"]

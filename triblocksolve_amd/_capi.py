@@ -87,6 +87,7 @@ SIGNATURES = {
     "tbs_plan_destroy": [P],
     "tbs_plan_info_get": [P, C.POINTER(PlanInfo)],
     "tbs_plan_factor": [P, P, P, P],
+    "tbs_plans_factor": [P, P, C.c_int, P, P, P],
     "tbs_plan_solve": [P, P, P, I, D, P, C.POINTER(C.c_int), P],
     "tbs_plan_autotune": [P, I, C.POINTER(C.c_int), P, I],
     "tbs_plan_set_variant": [P, I],

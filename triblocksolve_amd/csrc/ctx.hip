@@ -82,6 +82,20 @@ int stage_acquire(tbs_ctx *ctx, size_t bytes, char **out) {
     return TBS_OK;
 }
 
+int dev_stage(tbs_ctx *ctx, size_t bytes) {
+    if (bytes <= ctx->dstage_cap) return TBS_OK;
+    if (ctx->dstage) {
+        TBS_HIP(hipStreamSynchronize(ctx->stream));   // earlier users of the old buffer
+        TBS_HIP(hipFree(ctx->dstage));
+    }
+    ctx->dstage = nullptr;
+    ctx->dstage_cap = 0;
+    const size_t cap = std::max<size_t>(bytes, 1 << 16);
+    TBS_HIP(hipMalloc((void **)&ctx->dstage, cap));
+    ctx->dstage_cap = cap;
+    return TBS_OK;
+}
+
 int stage_release(tbs_ctx *ctx) {
     const int i = ctx->stage_cur;
     TBS_HIP(hipEventRecord(ctx->stage_ev[i], ctx->stream));
@@ -140,6 +154,7 @@ int tbs_ctx_destroy(tbs_ctx *ctx) {
         if (ctx->stage_ev[i]) hipEventDestroy(ctx->stage_ev[i]);
         if (ctx->stage[i]) hipHostFree(ctx->stage[i]);
     }
+    if (ctx->dstage) hipFree(ctx->dstage);
     if (ctx->red_buf) hipFree(ctx->red_buf);
     if (ctx->red_host) hipHostFree(ctx->red_host);
     if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);

@@ -651,17 +651,20 @@ __device__ __forceinline__ double recip(double x) {
 // their sources come from the per-element table fidx = {e, A source, C
 // source of the face before} (source = 2 * face + 1 for Of2, 2 * face for
 // Of1, -1 none: the zero block).
+// What the factor kernel needs of one plan: its line tables, its Jacobian
+// and its factor arrays.  One plan's record is a kernel argument; many plans
+// factored in one launch read theirs from device memory through a line table
+// {record, line}.
+struct FactorRec {
+    const int *linepoint, *linesize, *woff;
+    const int4 *fidx;
+    const double *D, *O1, *O2, *zero;   // zero: a block of zeros (the rhs padding)
+    void *W, *EF, *GB;
+};
+
 template <int NV, typename TF>
-__global__ __launch_bounds__(64) void ffactor_kernel(int nline, const int *__restrict__ linepoint,
-                                                     const int *__restrict__ linesize,
-                                                     const int *__restrict__ woff,
-                                                     const int4 *__restrict__ fidx,
-                                                     const double *__restrict__ jacD,
-                                                     const double *__restrict__ O1,
-                                                     const double *__restrict__ O2,
-                                                     const double *__restrict__ zero_blk,
-                                                     TF *__restrict__ W, TF *__restrict__ EF,
-                                                     TF *__restrict__ GB) {
+__global__ __launch_bounds__(64) void ffactor_kernel(FactorRec one, int nline, const FactorRec *__restrict__ recs,
+                                                     const int2 *__restrict__ ltab) {
     constexpr int BS = NV * NV;
     constexpr int CH = 4;                       // steps per chunk
     constexpr int NB = 3 * CH;                  // blocks per chunk: D_k, A_k, C_{k+1} of each step
@@ -676,8 +679,25 @@ __global__ __launch_bounds__(64) void ffactor_kernel(int nline, const int *__res
     __shared__ double sI[BS];                   // identity: the W lanes' starting column
     __shared__ const double *sptr[2][NB];       // sources of a chunk's blocks
 
-    const int l = blockIdx.x;
-    if (l >= nline) return;
+    if ((int)blockIdx.x >= nline) return;
+    int l = blockIdx.x;
+    FactorRec fr = one;
+    if (ltab) {
+        const int2 e = ltab[blockIdx.x];
+        fr = recs[__builtin_amdgcn_readfirstlane(e.x)];
+        l = __builtin_amdgcn_readfirstlane(e.y);
+    }
+    const int *__restrict__ linepoint = fr.linepoint;
+    const int *__restrict__ linesize = fr.linesize;
+    const int *__restrict__ woff = fr.woff;
+    const int4 *__restrict__ fidx = fr.fidx;
+    const double *__restrict__ jacD = fr.D;
+    const double *__restrict__ O1 = fr.O1;
+    const double *__restrict__ O2 = fr.O2;
+    const double *__restrict__ zero_blk = fr.zero;
+    TF *__restrict__ W = (TF *)fr.W;
+    TF *__restrict__ EF = (TF *)fr.EF;
+    TF *__restrict__ GB = (TF *)fr.GB;
     const int lane = threadIdx.x;
     // wave-uniform by construction: the chunk and step loops must be scalar
     // loops (the pivot broadcasts read lanes that every step writes)
@@ -1300,6 +1320,62 @@ int launch_final(hipStream_t st, const void *sys, int nsys, int max_sweeps, doub
 // Dispatch on (nvar, precision) for the fused kernels.
 #define TBS_PLAN_NVARS(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
 
+FactorRec factor_rec(const tbs_plan *p, const double *D, const double *O1, const double *O2) {
+    FactorRec r{};
+    r.linepoint = p->d_linepoint;
+    r.linesize = p->d_linesize;
+    r.woff = p->d_woff;
+    r.fidx = p->d_fidx;
+    r.D = D;
+    r.O1 = O1;
+    r.O2 = O2;
+    r.zero = p->a_rhs;   // the rhs allocation starts with kPad * nv >= nv * nv zeros
+    r.W = p->d_W;
+    r.EF = p->d_EF;
+    r.GB = p->d_GB;
+    return r;
+}
+
+int launch_factor(int nv, int prec, hipStream_t st, const FactorRec &one, int nline, const FactorRec *recs,
+                  const int2 *ltab) {
+    if (nline == 0) return TBS_OK;
+    switch (nv) {
+#define X(v) case v: \
+        if (prec == TBS_FP32) hipLaunchKernelGGL((ffactor_kernel<v, float>), dim3(nline), dim3(64), 0, st, one, \
+                                                 nline, recs, ltab); \
+        else hipLaunchKernelGGL((ffactor_kernel<v, double>), dim3(nline), dim3(64), 0, st, one, nline, recs, ltab); \
+        break;
+        TBS_PLAN_NVARS(X)
+#undef X
+    }
+    TBS_LAUNCHED("ffactor_kernel");
+    return TBS_OK;
+}
+
+int launch_pack(const tbs_plan *p, hipStream_t pst, const double *jacD, const double *jacO1, const double *jacO2) {
+    const int nv = p->nv, BS = nv * nv;
+    const long nws = (long)p->ngroups * p->S;
+    if (2 * nws < INT_MAX) {
+        switch (nv) {
+#define X(v) case v: hipLaunchKernelGGL(pack_lds_kernel<v>, dim3((unsigned)(2 * nws)), dim3(128), 0, pst, \
+                   p->S, p->d_src, jacD, jacO1, jacO2, p->d_blk); break;
+            TBS_PLAN_NVARS(X)
+#undef X
+        }
+        TBS_LAUNCHED("pack_lds_kernel");
+    } else {
+        const long total = (long)p->ngroups * kGroup * p->S * BS;
+        switch (nv) {
+#define X(v) case v: hipLaunchKernelGGL(pack_blocks_kernel<v>, dim3(gs_grid(total)), dim3(256), 0, pst, \
+                   total, p->S, p->d_src, jacD, jacO1, jacO2, p->d_blk); break;
+            TBS_PLAN_NVARS(X)
+#undef X
+        }
+        TBS_LAUNCHED("pack_blocks_kernel");
+    }
+    return TBS_OK;
+}
+
 int k_residual(int nv, int prec, hipStream_t st, const void *sys, const int2 *gtab, int ng,
                int want_R, int force) {
     switch (nv) {
@@ -1625,62 +1701,73 @@ int tbs_plan_info_get(tbs_plan *p, tbs_plan_info *info) {
 
 int tbs_plan_factor(tbs_plan *p, const double *jacD, const double *jacO1, const double *jacO2) {
     TBS_REQUIRE(p, TBS_ERR_ARG, "null plan");
-    TBS_TRY(tbs::use_ctx(p->ctx));
-    TBS_REQUIRE(jacD && (p->nintface == 0 || (jacO1 && jacO2)), TBS_ERR_ARG, "null Jacobian array");
-    tbs_ctx *ctx = p->ctx;
+    return tbs_plans_factor(p->ctx, &p, 1, &jacD, &jacO1, &jacO2);
+}
+
+int tbs_plans_factor(tbs_ctx *ctx, tbs_plan *const *plans, int n, const double *const *jacD,
+                     const double *const *jacO1, const double *const *jacO2) {
+    TBS_REQUIRE(ctx && plans && n >= 1 && jacD && jacO1 && jacO2, TBS_ERR_ARG, "bad arguments");
+    TBS_TRY(tbs::use_ctx(ctx));
+    const int nv = plans[0] ? plans[0]->nv : 0, prec = plans[0] ? plans[0]->prec : 0;
+    long nl = 0;
+    for (int i = 0; i < n; ++i) {
+        const tbs_plan *p = plans[i];
+        TBS_REQUIRE(p && p->ctx == ctx, TBS_ERR_ARG, "plan %d is null or on another context", i);
+        TBS_REQUIRE(p->nv == nv && p->prec == prec, TBS_ERR_ARG, "plan %d: block size or precision differs", i);
+        TBS_REQUIRE(jacD[i] && (p->nintface == 0 || (jacO1[i] && jacO2[i])), TBS_ERR_ARG,
+                    "plan %d: null Jacobian array", i);
+        nl += p->nline;
+    }
+    TBS_REQUIRE(nl < INT_MAX, TBS_ERR_ARG, "too many lines");
     hipStream_t st = ctx->stream;
-    const int nv = p->nv, BS = nv * nv;
-    const size_t G = (size_t)p->ngroups * kGroup;
-    const size_t tf = tf_size(p->prec);
     // sweep-form factors W, E, Gamma straight from the blocks; padding stays zero
-    TBS_HIP(hipMemsetAsync(p->d_EF, 0, (size_t)p->nwin * win_stride(nv, p->prec) * tf, st));
-    TBS_HIP(hipMemsetAsync(p->d_GB, 0, (size_t)p->nwin * win_stride(nv, p->prec) * tf, st));
-    TBS_HIP(hipMemsetAsync(p->d_W, 0, G * BS * tf, st));
+    for (int i = 0; i < n; ++i) {
+        const tbs_plan *p = plans[i];
+        const size_t tf = tf_size(prec), wb = (size_t)p->nwin * win_stride(nv, prec) * tf;
+        TBS_HIP(hipMemsetAsync(p->d_EF, 0, wb, st));
+        TBS_HIP(hipMemsetAsync(p->d_GB, 0, wb, st));
+        TBS_HIP(hipMemsetAsync(p->d_W, 0, (size_t)p->ngroups * kGroup * nv * nv * tf, st));
+    }
     hipStream_t pst;
     TBS_TRY(tbs::aux_stream(ctx, &pst));
     TBS_HIP(hipEventRecord(ctx->ev_fork, st));
     TBS_HIP(hipStreamWaitEvent(pst, ctx->ev_fork, 0));
-    switch (nv) {
-#define X(v) case v: \
-        if (p->prec == TBS_FP32) hipLaunchKernelGGL((ffactor_kernel<v, float>), dim3(p->nline), dim3(64), 0, st, \
-            p->nline, p->d_linepoint, p->d_linesize, p->d_woff, p->d_fidx, jacD, jacO1, jacO2, p->a_rhs, \
-            (float *)p->d_W, (float *)p->d_EF, (float *)p->d_GB); \
-        else hipLaunchKernelGGL((ffactor_kernel<v, double>), dim3(p->nline), dim3(64), 0, st, \
-            p->nline, p->d_linepoint, p->d_linesize, p->d_woff, p->d_fidx, jacD, jacO1, jacO2, p->a_rhs, \
-            (double *)p->d_W, (double *)p->d_EF, (double *)p->d_GB); break;
-        TBS_PLAN_NVARS(X)
-#undef X
+    if (n == 1) {
+        TBS_TRY(launch_factor(nv, prec, st, factor_rec(plans[0], jacD[0], jacO1[0], jacO2[0]), plans[0]->nline,
+                              nullptr, nullptr));
+    } else {
+        // one launch over every plan's lines, longest first; the records and
+        // the line table go up through the staging ring into a device buffer
+        // owned by the context (stream-ordered reuse)
+        std::vector<std::pair<int, int2>> order;
+        order.reserve(nl);
+        for (int i = 0; i < n; ++i)
+            for (int l = 0; l < plans[i]->nline; ++l) order.push_back({plans[i]->h_linesize[l], make_int2(i, l)});
+        std::stable_sort(order.begin(), order.end(),
+                         [](const std::pair<int, int2> &a, const std::pair<int, int2> &b) { return a.first > b.first; });
+        const size_t rec_bytes = (size_t)n * sizeof(FactorRec), lt_off = (rec_bytes + 15) / 16 * 16;
+        const size_t bytes = lt_off + (size_t)nl * sizeof(int2);
+        char *h = nullptr;
+        TBS_TRY(tbs::stage_acquire(ctx, bytes, &h));
+        FactorRec *rec = (FactorRec *)h;
+        for (int i = 0; i < n; ++i) rec[i] = factor_rec(plans[i], jacD[i], jacO1[i], jacO2[i]);
+        int2 *lt = (int2 *)(h + lt_off);
+        for (long b = 0; b < nl; ++b) lt[b] = order[b].second;
+        TBS_TRY(tbs::dev_stage(ctx, bytes));
+        TBS_HIP(hipMemcpyAsync(ctx->dstage, h, bytes, hipMemcpyHostToDevice, st));
+        TBS_TRY(tbs::stage_release(ctx));
+        TBS_TRY(launch_factor(nv, prec, st, FactorRec{}, (int)nl, (const FactorRec *)ctx->dstage,
+                              (const int2 *)(ctx->dstage + lt_off)));
     }
-    TBS_LAUNCHED("ffactor_kernel");
     // residual blocks, packed on the context's second stream while the
     // factor kernel (latency-bound, one wave per line) runs on the main one:
     // queued after the factor kernel (its waves take their CUs first) and
     // forked after the memsets; the main stream waits for the packing before
     // anything after the call
-    {
-        const long nws = (long)p->ngroups * p->S;
-        if (2 * nws < INT_MAX) {
-            switch (nv) {
-#define X(v) case v: hipLaunchKernelGGL(pack_lds_kernel<v>, dim3((unsigned)(2 * nws)), dim3(128), 0, pst, \
-                       p->S, p->d_src, jacD, jacO1, jacO2, p->d_blk); break;
-                TBS_PLAN_NVARS(X)
-#undef X
-            }
-            TBS_LAUNCHED("pack_lds_kernel");
-        } else {
-            const long total = (long)G * p->S * BS;
-            switch (nv) {
-#define X(v) case v: hipLaunchKernelGGL(pack_blocks_kernel<v>, dim3(gs_grid(total)), dim3(256), 0, pst, \
-                       total, p->S, p->d_src, jacD, jacO1, jacO2, p->d_blk); break;
-                TBS_PLAN_NVARS(X)
-#undef X
-            }
-            TBS_LAUNCHED("pack_blocks_kernel");
-        }
-    }
+    for (int i = 0; i < n; ++i) TBS_TRY(launch_pack(plans[i], pst, jacD[i], jacO1[i], jacO2[i]));
     TBS_HIP(hipEventRecord(ctx->ev_join, pst));
     TBS_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
-    p->factored = true;
+    for (int i = 0; i < n; ++i) plans[i]->factored = true;
     return TBS_OK;
 }
 

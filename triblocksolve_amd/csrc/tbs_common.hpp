@@ -40,6 +40,10 @@ struct tbs_ctx {
     hipEvent_t stage_ev[kStages] = {};
     bool stage_busy[kStages] = {};
     int stage_cur = 0;
+    // Device buffer for small per-call tables (grown on demand; reused in
+    // stream order on the context's stream).
+    char *dstage = nullptr;
+    size_t dstage_cap = 0;
 };
 
 namespace tbs {
@@ -54,6 +58,9 @@ int aux_stream(tbs_ctx *ctx, hipStream_t *out);
 // context's stream and then calls stage_release.
 int stage_acquire(tbs_ctx *ctx, size_t bytes, char **out);
 int stage_release(tbs_ctx *ctx);
+
+// The context's device table buffer, at least `bytes` long.
+int dev_stage(tbs_ctx *ctx, size_t bytes);
 
 // Record a HIP failure and return its positive status.
 int hip_fail(hipError_t e, const char *what, const char *file, int line);

@@ -15,6 +15,18 @@ import numpy as np
 from . import _capi as A
 
 
+def factor_plans(ctx, plans, jacD, jacO1, jacO2):
+    """Factorise several plans of one context in one launch over all their lines
+    (tbs_plans_factor); jacD[i], jacO1[i], jacO2[i] belong to plans[i].  Returns
+    after the device has finished reading the Jacobians."""
+    n = len(plans)
+    P = C.c_void_p * n
+    hs = P(*[p.handle for p in plans])
+    d, o1, o2 = (P(*[_ptr(x).value for x in arrs]) for arrs in (jacD, jacO1, jacO2))
+    A.check(ctx._lib.tbs_plans_factor(ctx.handle, hs, n, d, o1, o2))
+    ctx.sync()
+
+
 def variant_name(code):
     """Name of a Thomas variant from its code (100 * prefetch windows + ring cycles)."""
     code = int(code)
