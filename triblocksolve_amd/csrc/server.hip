@@ -88,6 +88,7 @@ struct Request {
     std::vector<int> epoint, ef, fc, linesize, linepoint, lines, lineface;
     tbs_plan *plan = nullptr;
     double *d_rhs = nullptr, *d_U = nullptr;   // element order, in the slab
+    double *tD = nullptr, *tO1 = nullptr, *tO2 = nullptr;   // generated Jacobian (slab temporaries)
     hipEvent_t ready = nullptr;
     double *hU = nullptr;           // result U (pinned, from the server's pool)
     size_t nU = 0, hU_cls = 0;
@@ -222,8 +223,11 @@ void drop_device(tbs_server *s, Request *r) {
     r->d_rhs = r->d_U = nullptr;
 }
 
-// Enqueue a request's set-up on the admission stream, inside its slab.
-int admit(tbs_server *s, Request *r) {
+// Enqueue a request's set-up on the admission stream, inside its slab:
+// plan tables, device generation of its Jacobian and rhs, U = 0.  Its
+// factorisation and first residual follow in admit_finish, together with the
+// other requests admitted in the same step.
+int admit_setup(tbs_server *s, Request *r) {
     const tbs_mesh_host mh = mesh_view(*r, s->ctx->nvar);
     const int nv = mh.nvar;
     const size_t bs = (size_t)nv * nv;
@@ -231,7 +235,6 @@ int admit(tbs_server *s, Request *r) {
     a->region = s->addr(r->slab_off);
     a->region_size = r->slab_len;
     a->region_used = 0;
-    double *D = nullptr, *O1 = nullptr, *O2 = nullptr;
     int rc = TBS_OK;
     do {
         const double tp0 = s->now();
@@ -242,25 +245,44 @@ int admit(tbs_server *s, Request *r) {
         if ((rc = tbs::dev_alloc(a, (size_t)nv * r->nelem * 8, (void **)&r->d_U, nullptr))) break;
         if (a->region_used > r->persist) { rc = TBS_ERR_STATE; tbs::set_error("slab layout overflow"); break; }
         a->region_used = r->persist;   // temporaries start at the persistent boundary
-        if ((rc = tbs::dev_alloc(a, bs * r->nelem * 8, (void **)&D, nullptr))) break;
-        if ((rc = tbs::dev_alloc(a, bs * std::max(r->nintface, 1) * 8, (void **)&O1, nullptr))) break;
-        if ((rc = tbs::dev_alloc(a, bs * std::max(r->nintface, 1) * 8, (void **)&O2, nullptr))) break;
-        if ((rc = tbs_generate_device(a, r->spec.seed, nv, r->nelem, r->nintface, r->spec.shift, D, O1, O2,
-                                      r->d_rhs)))
+        if ((rc = tbs::dev_alloc(a, bs * r->nelem * 8, (void **)&r->tD, nullptr))) break;
+        if ((rc = tbs::dev_alloc(a, bs * std::max(r->nintface, 1) * 8, (void **)&r->tO1, nullptr))) break;
+        if ((rc = tbs::dev_alloc(a, bs * std::max(r->nintface, 1) * 8, (void **)&r->tO2, nullptr))) break;
+        if ((rc = tbs_generate_device(a, r->spec.seed, nv, r->nelem, r->nintface, r->spec.shift, r->tD, r->tO1,
+                                      r->tO2, r->d_rhs)))
             break;
         if ((rc = tbs_memset(a, r->d_U, 0, (size_t)nv * r->nelem * 8))) break;
-        if ((rc = tbs_plan_factor(r->plan, D, O1, O2))) break;   // workspace from the region: no wait
-        if ((rc = tbs_plan_begin(r->plan, r->d_rhs, r->d_U))) break;
-        if (hipEventCreateWithFlags(&r->ready, hipEventDisableTiming) ||
-            hipEventRecord(r->ready, a->stream)) {
-            rc = TBS_ERR_STATE;
-            tbs::set_error("admission event");
-            break;
-        }
     } while (0);
     a->region = nullptr;
     a->region_size = a->region_used = 0;
     if (rc) hipStreamSynchronize(a->stream);   // nothing of the slab may still be in use
+    return rc;
+}
+
+// Factorise the requests set up in this step in one launch over all their
+// lines (a small request's factorisation is a latency-bound chain per line
+// on a few CUs; one launch runs them side by side), then their first
+// residuals and set-up events.
+int admit_finish(tbs_server *s, const std::vector<Request *> &rs) {
+    tbs_ctx *a = s->adm;
+    std::vector<tbs_plan *> plans;
+    std::vector<const double *> D, O1, O2;
+    for (Request *r : rs) {
+        plans.push_back(r->plan);
+        D.push_back(r->tD);
+        O1.push_back(r->tO1);
+        O2.push_back(r->tO2);
+    }
+    int rc = tbs_plans_factor(a, plans.data(), (int)plans.size(), D.data(), O1.data(), O2.data());
+    for (size_t i = 0; i < rs.size() && rc == TBS_OK; ++i) {
+        Request *r = rs[i];
+        if ((rc = tbs_plan_begin(r->plan, r->d_rhs, r->d_U))) break;
+        if (hipEventCreateWithFlags(&r->ready, hipEventDisableTiming) || hipEventRecord(r->ready, a->stream)) {
+            rc = TBS_ERR_STATE;
+            tbs::set_error("admission event");
+        }
+    }
+    if (rc) hipStreamSynchronize(a->stream);   // nothing of the slabs may still be in use
     return rc;
 }
 
@@ -437,6 +459,7 @@ int tbs_server_step(tbs_server *s, int *nactive, int *nqueued, int *nadmitting) 
 
     // 5. admission: FIFO; later requests may pass a head that does not fit
     //    until the head has been passed over kMaxSkips times
+    std::vector<Request *> fresh;
     for (auto it = s->queue.begin(); it != s->queue.end();) {
         Request *r = it->get();
         const bool is_head = it == s->queue.begin();
@@ -453,7 +476,7 @@ int tbs_server_step(tbs_server *s, int *nactive, int *nqueued, int *nadmitting) 
         r->slab_len = r->footprint;
         s->used += r->footprint;
         r->t_admit = s->now();
-        const int rc = admit(s, r);
+        const int rc = admit_setup(s, r);
         std::unique_ptr<Request> own = std::move(*it);
         it = s->queue.erase(it);
         if (rc) {
@@ -464,7 +487,21 @@ int tbs_server_step(tbs_server *s, int *nactive, int *nqueued, int *nadmitting) 
             continue;
         }
         s->count[C_ADMITTED]++;
+        fresh.push_back(own.get());
         s->admitting.push_back(std::move(own));
+    }
+    if (!fresh.empty() && admit_finish(s, fresh) != TBS_OK) {
+        // none of this step's admissions is usable: fail them all
+        for (auto it = s->admitting.begin(); it != s->admitting.end();) {
+            if (std::find(fresh.begin(), fresh.end(), it->get()) == fresh.end()) { ++it; continue; }
+            std::unique_ptr<Request> own = std::move(*it);
+            it = s->admitting.erase(it);
+            s->used -= own->slab_len;
+            drop_device(s, own.get());
+            own->status = -1;
+            s->count[C_ADMITTED]--;
+            finish(s, std::move(own));
+        }
     }
     lap(T_ADMIT);
 
