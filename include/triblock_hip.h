@@ -72,7 +72,11 @@ int tbs_memcpy_h2d(tbs_ctx *ctx, void *dst, const void *src, size_t bytes);
 int tbs_memcpy_d2h(tbs_ctx *ctx, void *dst, const void *src, size_t bytes);
 int tbs_memset(tbs_ctx *ctx, void *dst, int value, size_t bytes);
 /* Timing events on the context's stream (hipEvent_t behind an opaque handle;
- * the reference times with OCCA stream tags, src/main.cxx:181-228). */
+ * the reference times with OCCA stream tags, src/main.cxx:181-228).  They are
+ * created without the system-scope fence a default event record performs (a
+ * cache writeback and invalidate that would slow the kernels being timed), so
+ * they order and time work but do not publish device writes to the host:
+ * synchronise the context for that. */
 int tbs_event_create(tbs_ctx *ctx, void **ev);
 int tbs_event_destroy(tbs_ctx *ctx, void *ev);
 int tbs_event_record(tbs_ctx *ctx, void *ev);

@@ -246,7 +246,7 @@ int tbs_event_create(tbs_ctx *ctx, void **ev) {
     if (rc) return rc;
     TBS_REQUIRE(ev != nullptr, TBS_ERR_ARG, "null output pointer");
     hipEvent_t e;
-    TBS_HIP(hipEventCreate(&e));
+    TBS_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     *ev = (void *)e;
     return TBS_OK;
 }

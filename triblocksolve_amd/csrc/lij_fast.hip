@@ -1896,8 +1896,8 @@ int autotune_thomas(tbs_ctx *ctx, int nv, int prec, const void *sys, int nsys, c
     TBS_TRY(k_reset(prec, st, sys, nsys));
     const int nvar = thomas_variants(nv, prec);
     hipEvent_t e0, e1;
-    TBS_HIP(hipEventCreate(&e0));
-    TBS_HIP(hipEventCreate(&e1));
+    TBS_HIP(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));   // timing only
+    TBS_HIP(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
     int best = 0;
     double best_t = 1e300;
     int rc = TBS_OK;
