@@ -141,6 +141,22 @@ def test_fp32_factors_reach_fp64_accuracy(tbs, ctxs, oracle):
     assert relerr(o32["U"], ref["U"]) < 1e-9
 
 
+def test_second_solve_equals_first(tbs, ctxs):
+    """A plan's first solve launches its sweeps eagerly, later ones replay
+    captured graphs: the same history and U, bitwise."""
+    m, jac = system(30, 40, 9, seed=5)
+    plan = tbs.Plan(ctxs(9), m, 0)
+    plan.factor(dev(jac["jacD"]), dev(jac["jacO1"]), dev(jac["jacO2"]))
+    out = []
+    for _ in range(3):
+        U = zeros(9 * m.nelem)
+        hist, ns = plan.solve(dev(jac["rhs"]), U, 30, 1e-12)
+        out.append((hist, ns, host(U)))
+    plan.close()
+    for hist, ns, U in out[1:]:
+        assert ns == out[0][1] and np.array_equal(hist, out[0][0]) and np.array_equal(U, out[0][2])
+
+
 def test_graph_replay_equals_eager(tbs, ctxs):
     """Several replays of a captured K-sweep graph, with eager work between them,
     give exactly the eager launches' history and solution."""

@@ -1011,6 +1011,7 @@ struct tbs_plan {
     std::map<int, hipGraphExec_t> graphs;
     int max_sweeps = kHistCap - 1;
     double rtol = 0.0;
+    int solves = 0;           // tbs_plan_solve calls so far
     int variant = 0;          // Thomas prefetch variant (autotuned)
     size_t bytes = 0;
     int *h_state = nullptr;      // pinned unless region-backed
@@ -1984,9 +1985,13 @@ int tbs_plan_solve(tbs_plan *p, const double *rhs, double *U, int max_sweeps, do
     hipStream_t st = p->ctx->stream;
     int done = 0;
     const int chunk = 4;
+    // a plan's first solve launches its sweeps one kernel at a time (a graph
+    // capture costs more than the launches it saves over a few sweeps);
+    // later solves replay captured graphs
+    const int use_graph = p->solves++ > 0 ? 1 : 0;
     while (done < max_sweeps) {
         const int k = std::min(chunk, max_sweeps - done);
-        TBS_TRY(tbs_plan_launch_sweeps(p, k, 1));
+        TBS_TRY(tbs_plan_launch_sweeps(p, k, use_graph));
         done += k;
         TBS_HIP(hipMemcpyAsync(p->h_state, p->d_state, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
         TBS_HIP(hipStreamSynchronize(st));
