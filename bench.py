@@ -175,9 +175,9 @@ def bench_gmres(tbs, ctx, m, args, restart=30):
         D, O1, O2, rhs = z(81 * n), z(81 * nif), z(81 * nif), z(9 * n)
         ctx.generate_device(args.seed, n, nif, shift, D, O1, O2, rhs)
         U = ctx.array(size=9 * n)
+        p = tbs.Plan(ctx, m, tbs.TBS_FP64)
         ctx.sync()
         t0 = time.perf_counter()
-        p = tbs.Plan(ctx, m, tbs.TBS_FP64)
         p.factor(D, O1, O2)
         ctx.sync()
         t_factor = time.perf_counter() - t0
