@@ -832,7 +832,10 @@ __global__ __launch_bounds__(64) void ffactor_kernel(FactorRec one, int nline, c
         const int w = k / SPW, st = k - w * SPW;
         const int mm = m0 + k;
         const bool isW = grp == 3;
-        const bool act = (grp == xg && k <= n - 2) || (grp == 2 && k >= 1) || isW;
+        // every window entry of a real step is written: E_0 and Gamma_{n-1}
+        // (no neighbour) as zeros, so the factor arrays need no zero fill
+        const bool act = grp == xg || grp == 2 || isW;
+        const bool zero = (grp == xg && k == n - 1) || (grp == 2 && k == 0);
         const uintptr_t wdst = aW + (uintptr_t)((size_t)(mm / kGroup) * BS * kGroup * sizeof(TF));
         const uintptr_t fdst = aEF + (uintptr_t)(grp == xg) * dGB +
                                (uintptr_t)((wbase + (long)w * WL::STRIDE) * (long)sizeof(TF));
@@ -841,7 +844,14 @@ __global__ __launch_bounds__(64) void ffactor_kernel(FactorRec one, int nline, c
         const int sc = isW ? mm % kGroup : st;
         if (act) {
 #pragma unroll
-            for (int i = 0; i < NV; ++i) dst[u[i] + sc * v[i]] = (TF)col[i];
+            for (int i = 0; i < NV; ++i) dst[u[i] + sc * v[i]] = zero ? TF(0) : (TF)col[i];
+        }
+        // the window's alignment padding, once per window (never read as a
+        // factor; written so the arrays hold no stale memory)
+        if (WL::STRIDE > WL::ELEMS && st == 0 && grp == 2 && j < WL::STRIDE - WL::ELEMS) {
+            const long at = wbase + (long)w * WL::STRIDE + WL::ELEMS + j;
+            ((gdst)(TF *)aEF)[at] = TF(0);
+            ((gdst)(TF *)(aEF + dGB))[at] = TF(0);
         }
     };
 
@@ -860,6 +870,21 @@ __global__ __launch_bounds__(64) void ffactor_kernel(FactorRec one, int nline, c
         }
         step(std::integral_constant<int, 0>(), c & 1, s, k);
         if (k + 1 < n) step(std::integral_constant<int, 1>(), c & 1, s + 1, k + 1);
+    }
+    // steps past the line end in its last window: E and Gamma zero (the
+    // Thomas sweep runs whole windows and needs finite factors there)
+    for (int k = n; k < (n + SPW - 1) / SPW * SPW; ++k) {
+        if (grp == 2) {
+            const int w = k / SPW, st = k - w * SPW;
+            using gdst = __attribute__((address_space(1))) TF *;
+            const uintptr_t base = (uintptr_t)((wbase + (long)w * WL::STRIDE) * (long)sizeof(TF));
+            const gdst e = (gdst)(TF *)(aEF + base), g = (gdst)(TF *)(aEF + dGB + base);
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                e[u[i] + st * v[i]] = TF(0);
+                g[u[i] + st * v[i]] = TF(0);
+            }
+        }
     }
 }
 
@@ -1721,13 +1746,13 @@ int tbs_plans_factor(tbs_ctx *ctx, tbs_plan *const *plans, int n, const double *
     }
     TBS_REQUIRE(nl < INT_MAX, TBS_ERR_ARG, "too many lines");
     hipStream_t st = ctx->stream;
-    // sweep-form factors W, E, Gamma straight from the blocks; padding stays zero
+    // sweep-form factors W, E, Gamma straight from the blocks: the factor
+    // kernel writes every window entry the sweep reads; of W only the last
+    // group's slot has lanes past the last element, zeroed here
     for (int i = 0; i < n; ++i) {
         const tbs_plan *p = plans[i];
-        const size_t tf = tf_size(prec), wb = (size_t)p->nwin * win_stride(nv, prec) * tf;
-        TBS_HIP(hipMemsetAsync(p->d_EF, 0, wb, st));
-        TBS_HIP(hipMemsetAsync(p->d_GB, 0, wb, st));
-        TBS_HIP(hipMemsetAsync(p->d_W, 0, (size_t)p->ngroups * kGroup * nv * nv * tf, st));
+        const size_t slot = (size_t)kGroup * nv * nv * tf_size(prec);
+        TBS_HIP(hipMemsetAsync((char *)p->d_W + (size_t)(p->ngroups - 1) * slot, 0, slot, st));
     }
     hipStream_t pst;
     TBS_TRY(tbs::aux_stream(ctx, &pst));
