@@ -207,6 +207,15 @@ int tbs_plan_variants(tbs_plan *plan, int *pf, int cap);
 int tbs_plan_begin(tbs_plan *plan, const double *rhs, const double *U0);
 int tbs_plan_launch_thomas(tbs_plan *plan);
 int tbs_plan_launch_residual(tbs_plan *plan);   /* residual + norm of sweep */
+/* One kernel of a sweep, for timing it alone: TBS_STEP_THOMAS (Thomas pass
+ * and U update), TBS_STEP_RESIDUAL (R, per-group partial norms, next Thomas
+ * right-hand side) or TBS_STEP_NORM (norm, history, convergence over the
+ * partials).  A sweep is the three in this order.  With two events
+ * (tbs_event_create, both or neither), the kernel is launched with
+ * hipExtLaunchKernelGGL, which records them at the kernel's own start and
+ * end, so tbs_event_elapsed_ms gives its execution time alone. */
+enum { TBS_STEP_THOMAS = 0, TBS_STEP_RESIDUAL = 1, TBS_STEP_NORM = 2 };
+int tbs_plan_launch_kernel(tbs_plan *plan, int step, void *ev_start, void *ev_stop);
 int tbs_plan_launch_sweeps(tbs_plan *plan, int nsweeps, int use_graph);
 int tbs_plan_end(tbs_plan *plan, double *U, double *R_final);
 int tbs_plan_history(tbs_plan *plan, double *hist, int cap, int *count);

@@ -92,3 +92,25 @@ def test_cfg4_properties(tbs):
     assert ok, worst
     assert relerr(host(Ud), U64) < 1e-12
     ctx.close()
+
+
+def test_norm_past_one_pass(tbs):
+    """2,201,600 elements = 34,400 residual groups, more partials than the norm
+    kernel sums in one pass of its threads (32 x 1024): ||R_0|| = ||rhs|| (U = 0)
+    and the last history entry = ||R|| of the returned residual, in numpy."""
+    print("case norm 1720x1280", flush=True)
+    ctx = tbs.Context(0, 9)
+    m = tbs.structured_mesh(1720, 1280)
+    n, nif = m.nelem, m.nintface
+    assert (n + 63) // 64 > 32 * 1024
+    D, O1, O2, rhs = zeros(81 * n), zeros(81 * nif), zeros(81 * nif), zeros(9 * n)
+    ctx.generate_device(8, n, nif, 12.0, D, O1, O2, rhs)
+    plan = tbs.Plan(ctx, m, 0)
+    plan.factor(D, O1, O2)
+    U, R = zeros(9 * n), zeros(9 * n)
+    hist, ns = plan.solve(rhs, U, 2, 0.0, R_out=R)
+    plan.close()
+    assert ns == 2
+    assert abs(hist[0] - np.linalg.norm(host(rhs))) <= 1e-13 * hist[0]
+    assert abs(hist[-1] - np.linalg.norm(host(R))) <= 1e-13 * hist[-1]
+    ctx.close()

@@ -163,13 +163,24 @@ def test_graph_replay_equals_eager(tbs, ctxs):
     m, jac = system(30, 40, 9, seed=2)
     ctx = ctxs(9)
     res = {}
-    for mode in ("eager", "graph"):
+    steps = (tbs.TBS_STEP_THOMAS, tbs.TBS_STEP_RESIDUAL, tbs.TBS_STEP_NORM)
+    for mode in ("eager", "graph", "kernels"):
         plan = tbs.Plan(ctx, m)
         plan.factor(dev(jac["jacD"]), dev(jac["jacO1"]), dev(jac["jacO2"]))
         rhs = dev(jac["rhs"])
         plan.begin(rhs, None)
         for _ in range(3):
-            plan.launch_sweeps(2, use_graph=(mode == "graph"))
+            if mode == "kernels":   # one kernel at a time, each timed by its own events
+                for _ in range(2):
+                    for step in steps:
+                        e0, e1 = ctx.event(), ctx.event()
+                        plan.launch_kernel(step, e0, e1)
+                        ctx.sync()
+                        assert ctx.elapsed_ms(e0, e1) > 0.0
+                        ctx.destroy_event(e0)
+                        ctx.destroy_event(e1)
+            else:
+                plan.launch_sweeps(2, use_graph=(mode == "graph"))
             ctx.sync()
             zeros(1000).numpy().sum()   # unrelated work and a device-wide sync
         U = zeros(9 * m.nelem)
@@ -177,8 +188,9 @@ def test_graph_replay_equals_eager(tbs, ctxs):
         ctx.sync()
         res[mode] = (plan.history(), host(U))
         plan.close()
-    assert np.array_equal(res["eager"][0], res["graph"][0])
-    assert np.array_equal(res["eager"][1], res["graph"][1])
+    for mode in ("graph", "kernels"):
+        assert np.array_equal(res["eager"][0], res[mode][0]), mode
+        assert np.array_equal(res["eager"][1], res[mode][1]), mode
 
 
 def test_deterministic(tbs, ctxs):
@@ -225,6 +237,12 @@ def test_bad_inputs_raise(tbs, ctxs):
     with pytest.raises(tbs.TbsError) as ei:
         p.solve(dev(jac["rhs"]), zeros(9 * m.nelem), 3)
     assert ei.value.code == -2
+    e = ctxs(9).event()
+    for step, ev in ((7, (None, None)), (tbs.TBS_STEP_NORM, (e, None))):   # rejected before any launch
+        with pytest.raises(tbs.TbsError) as ei:
+            p.launch_kernel(step, *ev)
+        assert ei.value.code == -1
+    ctxs(9).destroy_event(e)
     p.close()
     with pytest.raises(tbs.TbsError):
         tbs.Context(0, 33)

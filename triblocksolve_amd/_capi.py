@@ -18,6 +18,9 @@ TBS_ERR_IO = -5
 TBS_ERR_NUMERIC = -6
 TBS_FP64 = 0
 TBS_FP32 = 1
+TBS_STEP_THOMAS = 0
+TBS_STEP_RESIDUAL = 1
+TBS_STEP_NORM = 2
 
 
 class TbsError(RuntimeError):
@@ -95,6 +98,7 @@ SIGNATURES = {
     "tbs_plan_begin": [P, P, P],
     "tbs_plan_launch_thomas": [P],
     "tbs_plan_launch_residual": [P],
+    "tbs_plan_launch_kernel": [P, I, P, P],
     "tbs_plan_launch_sweeps": [P, I, I],
     "tbs_plan_end": [P, P, P],
     "tbs_plan_history": [P, P, I, C.POINTER(C.c_int)],

@@ -33,6 +33,8 @@
 #include "tbs_common.hpp"
 #include "lane_ops.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -459,8 +461,11 @@ __global__ __launch_bounds__(64) void fthomas_kernel(const SysDev<TF> *__restric
 
 // ------------------------------------------------ norm / convergence --
 // One block per system: fixed-order sum of the group partials (thread t
-// adds partials t, t + 1024, ... in order, then a fixed LDS tree).
+// adds partials t, t + 1024, ... in order, then a fixed LDS tree).  Each
+// thread issues the loads of up to kFinalPer partials before it adds any, so
+// the pass costs one memory round trip per kFinalPer * 1024 partials.
 constexpr int kFinalThreads = 1024;
+constexpr int kFinalPer = 32;
 template <typename TF>
 __global__ __launch_bounds__(kFinalThreads) void ffinal_kernel(const SysDev<TF> *__restrict__ sys, int nsys,
                                                                int max_sweeps, double rtol, int force) {
@@ -471,16 +476,17 @@ __global__ __launch_bounds__(kFinalThreads) void ffinal_kernel(const SysDev<TF> 
     if (!force && !S.state[1]) return;
     const int ng = S.ngroups;
     double acc = 0.0;
-    int i = threadIdx.x;
-    for (; i + 3 * kFinalThreads < ng; i += 4 * kFinalThreads) {
-        const double a0 = S.gpart[i], a1 = S.gpart[i + kFinalThreads];
-        const double a2 = S.gpart[i + 2 * kFinalThreads], a3 = S.gpart[i + 3 * kFinalThreads];
-        acc += a0;
-        acc += a1;
-        acc += a2;
-        acc += a3;
+    for (int base = 0; base < ng; base += kFinalPer * kFinalThreads) {
+        double v[kFinalPer];
+#pragma unroll
+        for (int q = 0; q < kFinalPer; ++q) {
+            const int i = base + q * kFinalThreads + (int)threadIdx.x;
+            v[q] = i < ng ? S.gpart[i] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < kFinalPer; ++q)
+            if (base + q * kFinalThreads + (int)threadIdx.x < ng) acc += v[q];
     }
-    for (; i < ng; i += kFinalThreads) acc += S.gpart[i];
     s[threadIdx.x] = acc;
     __syncthreads();
     for (int w = kFinalThreads / 2; w >= 1; w >>= 1) {
@@ -1273,12 +1279,22 @@ void free_plan(tbs_plan *p) {
     delete p;
 }
 
+// Launch on st; with a start event, through hipExtLaunchKernelGGL, which
+// records the two events at the kernel's own start and end (timing of exactly
+// this kernel, without the dispatch or the completion signal around it).
+template <typename K, typename... Args>
+void launch_k(K kernel, dim3 grid, dim3 block, hipStream_t st, hipEvent_t e0, hipEvent_t e1, Args... args) {
+    if (e0) hipExtLaunchKernelGGL(kernel, grid, block, 0, st, e0, e1, 0, args...);
+    else hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
+}
+
 template <int NV, typename TF>
-int launch_residual(hipStream_t st, const void *sys, const int2 *gtab, int ng, int want_R, int force) {
+int launch_residual(hipStream_t st, const void *sys, const int2 *gtab, int ng, int want_R, int force,
+                    hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     if (ng == 0) return TBS_OK;
     const int nblk = (ng + 3) / 4;
-    hipLaunchKernelGGL((fres_kernel<NV, TF>), dim3(8 * ((nblk + 7) / 8)), dim3(256), 0, st,
-                       (const SysDev<TF> *)sys, gtab, ng, want_R, force);
+    launch_k(fres_kernel<NV, TF>, dim3(8 * ((nblk + 7) / 8)), dim3(256), st, e0, e1,
+             (const SysDev<TF> *)sys, gtab, ng, want_R, force);
     TBS_LAUNCHED("fres_kernel");
     return TBS_OK;
 }
@@ -1298,15 +1314,15 @@ template <> struct ThomasV<9, float> {
 };
 
 template <int NV, typename TF, int PF, int K>
-int launch_thomas_pf(hipStream_t st, const void *sys, const int4 *ltab, int nl) {
-    hipLaunchKernelGGL((fthomas_kernel<NV, TF, PF, K>), dim3(nl), dim3(64), 0, st,
-                       (const SysDev<TF> *)sys, ltab, nl);
+int launch_thomas_pf(hipStream_t st, const void *sys, const int4 *ltab, int nl, hipEvent_t e0, hipEvent_t e1) {
+    launch_k(fthomas_kernel<NV, TF, PF, K>, dim3(nl), dim3(64), st, e0, e1, (const SysDev<TF> *)sys, ltab, nl);
     TBS_LAUNCHED("fthomas_kernel");
     return TBS_OK;
 }
 
 template <int NV, typename TF>
-int launch_thomas(hipStream_t st, const void *sys, const int4 *ltab, int nl, int variant) {
+int launch_thomas(hipStream_t st, const void *sys, const int4 *ltab, int nl, int variant, hipEvent_t e0 = nullptr,
+                  hipEvent_t e1 = nullptr) {
     if (nl == 0) return TBS_OK;
     using P = ThomasV<NV, TF>;
     if (variant < 0 || variant >= P::n) variant = 0;
@@ -1315,10 +1331,10 @@ int launch_thomas(hipStream_t st, const void *sys, const int4 *ltab, int nl, int
     constexpr TV c = P::v[P::n > 2 ? 2 : 0];
     constexpr TV d = P::v[P::n > 3 ? 3 : 0];
     switch (variant) {
-        case 1: return launch_thomas_pf<NV, TF, b.pf, b.k>(st, sys, ltab, nl);
-        case 2: return launch_thomas_pf<NV, TF, c.pf, c.k>(st, sys, ltab, nl);
-        case 3: return launch_thomas_pf<NV, TF, d.pf, d.k>(st, sys, ltab, nl);
-        default: return launch_thomas_pf<NV, TF, a.pf, a.k>(st, sys, ltab, nl);
+        case 1: return launch_thomas_pf<NV, TF, b.pf, b.k>(st, sys, ltab, nl, e0, e1);
+        case 2: return launch_thomas_pf<NV, TF, c.pf, c.k>(st, sys, ltab, nl, e0, e1);
+        case 3: return launch_thomas_pf<NV, TF, d.pf, d.k>(st, sys, ltab, nl, e0, e1);
+        default: return launch_thomas_pf<NV, TF, a.pf, a.k>(st, sys, ltab, nl, e0, e1);
     }
 }
 
@@ -1336,9 +1352,10 @@ int thomas_pf_of(int nv, int prec, int variant) {
 }
 
 template <typename TF>
-int launch_final(hipStream_t st, const void *sys, int nsys, int max_sweeps, double rtol, int force) {
-    hipLaunchKernelGGL((ffinal_kernel<TF>), dim3(nsys), dim3(kFinalThreads), 0, st,
-                       (const SysDev<TF> *)sys, nsys, max_sweeps, rtol, force);
+int launch_final(hipStream_t st, const void *sys, int nsys, int max_sweeps, double rtol, int force,
+                 hipEvent_t e0, hipEvent_t e1) {
+    launch_k(ffinal_kernel<TF>, dim3(nsys), dim3(kFinalThreads), st, e0, e1, (const SysDev<TF> *)sys, nsys,
+             max_sweeps, rtol, force);
     TBS_LAUNCHED("ffinal_kernel");
     return TBS_OK;
 }
@@ -1403,29 +1420,31 @@ int launch_pack(const tbs_plan *p, hipStream_t pst, const double *jacD, const do
 }
 
 int k_residual(int nv, int prec, hipStream_t st, const void *sys, const int2 *gtab, int ng,
-               int want_R, int force) {
+               int want_R, int force, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     switch (nv) {
-#define X(v) case v: return prec == TBS_FP32 ? launch_residual<v, float>(st, sys, gtab, ng, want_R, force) \
-                                         : launch_residual<v, double>(st, sys, gtab, ng, want_R, force);
+#define X(v) case v: return prec == TBS_FP32 ? launch_residual<v, float>(st, sys, gtab, ng, want_R, force, e0, e1) \
+                                         : launch_residual<v, double>(st, sys, gtab, ng, want_R, force, e0, e1);
         TBS_PLAN_NVARS(X)
 #undef X
     }
     tbs::set_error("nvar %d not built for the fused path", nv);
     return TBS_ERR_UNSUPPORTED;
 }
-int k_thomas(int nv, int prec, hipStream_t st, const void *sys, const int4 *ltab, int nl, int variant) {
+int k_thomas(int nv, int prec, hipStream_t st, const void *sys, const int4 *ltab, int nl, int variant,
+             hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     switch (nv) {
-#define X(v) case v: return prec == TBS_FP32 ? launch_thomas<v, float>(st, sys, ltab, nl, variant) \
-                                         : launch_thomas<v, double>(st, sys, ltab, nl, variant);
+#define X(v) case v: return prec == TBS_FP32 ? launch_thomas<v, float>(st, sys, ltab, nl, variant, e0, e1) \
+                                         : launch_thomas<v, double>(st, sys, ltab, nl, variant, e0, e1);
         TBS_PLAN_NVARS(X)
 #undef X
     }
     tbs::set_error("nvar %d not built for the fused path", nv);
     return TBS_ERR_UNSUPPORTED;
 }
-int k_final(int prec, hipStream_t st, const void *sys, int nsys, int max_sweeps, double rtol, int force) {
-    return prec == TBS_FP32 ? launch_final<float>(st, sys, nsys, max_sweeps, rtol, force)
-                            : launch_final<double>(st, sys, nsys, max_sweeps, rtol, force);
+int k_final(int prec, hipStream_t st, const void *sys, int nsys, int max_sweeps, double rtol, int force,
+            hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+    return prec == TBS_FP32 ? launch_final<float>(st, sys, nsys, max_sweeps, rtol, force, e0, e1)
+                            : launch_final<double>(st, sys, nsys, max_sweeps, rtol, force, e0, e1);
 }
 int k_reset(int prec, hipStream_t st, const void *sys, int nsys) {
     if (prec == TBS_FP32)
@@ -1825,10 +1844,23 @@ int tbs_plan_launch_thomas(tbs_plan *p) {
 }
 
 int tbs_plan_launch_residual(tbs_plan *p) {
+    TBS_TRY(tbs_plan_launch_kernel(p, TBS_STEP_RESIDUAL, nullptr, nullptr));
+    return tbs_plan_launch_kernel(p, TBS_STEP_NORM, nullptr, nullptr);
+}
+
+int tbs_plan_launch_kernel(tbs_plan *p, int step, void *ev_start, void *ev_stop) {
     TBS_REQUIRE(p, TBS_ERR_ARG, "null plan");
+    TBS_REQUIRE((ev_start == nullptr) == (ev_stop == nullptr), TBS_ERR_ARG, "give both events or neither");
     TBS_TRY(tbs::use_ctx(p->ctx));
-    TBS_TRY(k_residual(p->nv, p->prec, p->ctx->stream, p->d_sys, p->d_gtab, p->ngroups, 0, 0));
-    return k_final(p->prec, p->ctx->stream, p->d_sys, 1, p->max_sweeps, p->rtol, 0);
+    hipStream_t st = p->ctx->stream;
+    hipEvent_t e0 = (hipEvent_t)ev_start, e1 = (hipEvent_t)ev_stop;
+    switch (step) {
+        case TBS_STEP_THOMAS: return k_thomas(p->nv, p->prec, st, p->d_sys, p->d_ltab, p->nline, p->variant, e0, e1);
+        case TBS_STEP_RESIDUAL: return k_residual(p->nv, p->prec, st, p->d_sys, p->d_gtab, p->ngroups, 0, 0, e0, e1);
+        case TBS_STEP_NORM: return k_final(p->prec, st, p->d_sys, 1, p->max_sweeps, p->rtol, 0, e0, e1);
+    }
+    tbs::set_error("unknown sweep step %d", step);
+    return TBS_ERR_ARG;
 }
 
 int tbs_plan_launch_sweeps(tbs_plan *p, int nsweeps, int use_graph) {

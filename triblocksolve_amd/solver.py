@@ -456,6 +456,11 @@ class Plan:
     def launch_residual(self):
         A.check(self.ctx._lib.tbs_plan_launch_residual(self.handle))
 
+    def launch_kernel(self, step, ev_start=None, ev_stop=None):
+        """One kernel of a sweep (STEP_THOMAS, STEP_RESIDUAL, STEP_NORM); with
+        two events from Context.event(), they time that kernel alone."""
+        A.check(self.ctx._lib.tbs_plan_launch_kernel(self.handle, step, ev_start, ev_stop))
+
     def launch_sweeps(self, n, use_graph=True):
         A.check(self.ctx._lib.tbs_plan_launch_sweeps(self.handle, n, 1 if use_graph else 0))
 
