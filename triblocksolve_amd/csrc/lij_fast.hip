@@ -9,10 +9,15 @@
 //                 block loads 512 B coalesced), the per-group sum of R^2, and
 //                 the Thomas right-hand side g = D'^-1 (-R) (W = D'^-1 is a
 //                 factor).  R itself is only written when asked for.
-//   fthomas_kernel one wave per line.  Forward: d_k = g_k - E_k d_{k-1} with
-//                 E_k = D'_k^-1 A_k; backward: d_k = d'_k - Gamma_k d_{k+1}
-//                 with Gamma_k = D'_k^-1 C_k (the reference's DinvC), and
-//                 U_k += d_k.  A window of SPW = floor(64/NV) consecutive steps
+//   fthomas_kernel two waves per line, one from each end ("twisted" block
+//                 Thomas: the factorisation runs top-down to step m-1 and
+//                 bottom-up to step m+1 and closes at step m).  Forward:
+//                 d_k = g_k - E_k d_{k-1} with E_k = D'_k^-1 A_k (bottom half:
+//                 d_{k+1}, E_k = D'_k^-1 C_{k+1}); step m takes both
+//                 neighbours; backward: d_k = d'_k - Gamma_k d_{k+1} with
+//                 Gamma_k = D'_k^-1 C_{k+1} (the reference's DinvC; bottom
+//                 half: d_{k-1}, D'_k^-1 A_k), and U_k += d_k.  Each wave's
+//                 chain is half a line long.  A window of SPW = floor(64/NV) consecutive steps
 //                 is spread over the wave, lane = step*NV + row; each step's
 //                 9x9 matvec uses the previous step's vector as uniform SGPR
 //                 operands (v_readlane), so the per-step dependency chain is one
@@ -54,6 +59,28 @@ constexpr int kHistCap = 4096;  // sweeps of history kept per system
 constexpr int kPad = 64;                  // zero elements before and after U, rhs, R
 constexpr int kSlotNone = INT_MIN;        // no lane of the group has this face
 constexpr int kSlotGather = INT_MIN + 1;  // irregular neighbours: per-lane gather
+
+// Two-ended ("twisted") split of a line of n elements between the two waves
+// of its workgroup: one runs steps 0..m-1 top-down (q whole windows of spw
+// steps), the other steps n-1..m+1 bottom-up (nb windows, the first pad
+// steps of its first window empty), and step m closes the line from both
+// sides.  A line's factor windows: the q top windows, one window holding
+// step m (at its first step), then the nb bottom windows in bottom-up order.
+struct Split { int q, m, B, nb, pad; };
+__host__ __device__ inline Split split_line(int n, int spw) {
+    Split s;
+    const int wall = (n - 1 + spw - 1) / spw;   // windows of the n - 1 steps off the middle
+    s.q = wall / 2;
+    s.m = s.q * spw;
+    s.B = n - 1 - s.m;
+    s.nb = (s.B + spw - 1) / spw;
+    s.pad = s.nb * spw - s.B;
+    return s;
+}
+__host__ __device__ inline int line_windows(int n, int spw) {
+    const Split s = split_line(n, spw);
+    return s.q + 1 + s.nb;
+}
 
 template <typename TF>
 struct SysDev {
@@ -298,11 +325,16 @@ __device__ __forceinline__ void bulk_store(tbs::rsrc_t r, uint32_t base, int lan
     }
 }
 
-// One wave per line; windows of SPW steps; PF factor windows prefetched in a
-// register ring; vectors staged per group of G = K*(PF+1) windows in LDS.
+// Two waves per line (split_line): wave 0 eliminates steps 0..m-1 top-down,
+// wave 1 steps n-1..m+1 bottom-up, each over windows of SPW steps with PF
+// factor windows prefetched in a register ring and its vectors staged per
+// group of G = K*(PF+1) windows in LDS; wave 0 then closes step m from both
+// sides and both waves substitute back outwards from it.  The bottom wave
+// runs the same loops over its own windows; only the place of a step's
+// values in memory differs (element m0+n-1-s for its step s, less pad).
 template <int NV, typename TF, int PF, int K>
-__global__ __launch_bounds__(64) void fthomas_kernel(const SysDev<TF> *__restrict__ sys,
-                                                     const int4 *__restrict__ ltab, int nl) {
+__global__ __launch_bounds__(128) void fthomas_kernel(const SysDev<TF> *__restrict__ sys,
+                                                      const int4 *__restrict__ ltab, int nl) {
     constexpr int SPW = 64 / NV;      // steps per window
     constexpr int LW = SPW * NV;      // lanes carrying data
     constexpr int NB = PF + 1;        // register ring depth
@@ -310,9 +342,10 @@ __global__ __launch_bounds__(64) void fthomas_kernel(const SysDev<TF> *__restric
     constexpr int VG = G * LW;        // vector values per group
     using IOT = GroupIO<VG, TF>;
     using IOD = GroupIO<VG, double>;
-    __shared__ TF s_vin[2][IOT::LDS];       // forward: g; backward: d'
-    __shared__ TF s_out[IOT::LDS];          // forward: d' staging
-    __shared__ double s_u[2][IOD::LDS];     // backward: U (updated in place)
+    __shared__ TF s_vin[2][2][IOT::LDS];    // [wave] forward: g; backward: d'
+    __shared__ TF s_out[2][IOT::LDS];       // [wave] forward: d' staging
+    __shared__ double s_u[2][2][IOD::LDS];  // [wave] backward: U (updated in place)
+    __shared__ TF s_x[3][NV];               // d' at step m-1, d' at step m+1, d at step m
     const int l = blockIdx.x;
     if (l >= nl) return;
     // {system, m0, n, window offset}, wave-uniform by construction: the
@@ -324,22 +357,43 @@ __global__ __launch_bounds__(64) void fthomas_kernel(const SysDev<TF> *__restric
     lt.z = __builtin_amdgcn_readfirstlane(lt.z);
     lt.w = __builtin_amdgcn_readfirstlane(lt.w);
     const SysDev<TF> &S = sys[lt.x];
-    if (!S.state[1]) return;
-    const int lane = threadIdx.x;
+    if (!S.state[1]) return;                                            // the whole workgroup
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const bool bot = wave == 1;
+    const int lane = threadIdx.x & 63;
     const int st_of_lane = lane / NV;
+    const int row = lane - st_of_lane * NV;
     const int m0 = lt.y, n = lt.z;
-    const int nw = (n + SPW - 1) / SPW;
+    const Split sp = split_line(n, SPW);
+    const int nw = bot ? sp.nb : sp.q;                                  // the wave's windows
     const int ngrp = (nw + G - 1) / G;
+    // the bottom wave runs voff empty windows before its first one, so that
+    // its groups of windows start at whole groups of elements from element
+    // m+1 (every group's values start at an offset >= 0 in the wave's range)
+    const int voff = bot ? ngrp * G - nw : 0;
     constexpr int WSTRIDE = tbs::WinLayout<NV, LW, TF>::STRIDE;
     const uint32_t wbytes = (uint32_t)WSTRIDE * sizeof(TF);           // one window of factors
-    const tbs::rsrc_t rEF = tbs::make_rsrc(S.EF + (size_t)lt.w * WSTRIDE, (uint32_t)nw * wbytes);
-    const tbs::rsrc_t rGB = tbs::make_rsrc(S.GB + (size_t)lt.w * WSTRIDE, (uint32_t)nw * wbytes);
-    const uint32_t vb = (uint32_t)n * NV;                              // vector values of the line
-    const tbs::rsrc_t rg = tbs::make_rsrc(S.g + (size_t)NV * m0, vb * sizeof(TF));
-    const tbs::rsrc_t rd = tbs::make_rsrc(S.dUp + (size_t)NV * m0, vb * sizeof(TF));
-    const tbs::rsrc_t rU = tbs::make_rsrc(S.U + (size_t)NV * m0, vb * 8u);
-    // group j's vector block starts at value j*VG (windows j*G ...)
-    auto gbase = [&](int j, int bytes_per) -> uint32_t { return (uint32_t)(j * VG) * (uint32_t)bytes_per; };
+    const long wfirst = (long)lt.w + (bot ? sp.q + 1 : 0);
+    const tbs::rsrc_t rEF = tbs::make_rsrc(S.EF + wfirst * WSTRIDE, (uint32_t)nw * wbytes);
+    const tbs::rsrc_t rGB = tbs::make_rsrc(S.GB + wfirst * WSTRIDE, (uint32_t)nw * wbytes);
+    // the wave's elements: [m0, m0+m) top-down, [m0+m+1, m0+n) bottom-up
+    const int e0 = bot ? m0 + sp.m + 1 : m0;
+    const uint32_t vb = (uint32_t)(bot ? sp.B : sp.m) * NV;           // vector values of the wave
+    const tbs::rsrc_t rg = tbs::make_rsrc(S.g + (size_t)NV * e0, vb * sizeof(TF));
+    const tbs::rsrc_t rd = tbs::make_rsrc(S.dUp + (size_t)NV * e0, vb * sizeof(TF));
+    const tbs::rsrc_t rU = tbs::make_rsrc(S.U + (size_t)NV * e0, vb * 8u);
+    // Where the values of a group start in memory (first element, relative
+    // to the wave's first element; always >= 0) and where a step of window bb
+    // (lane = st*NV + row) sits in the group's LDS copy, which has the memory
+    // layout.  Groups are numbered in the direction of the forward pass: the
+    // forward pass visits groups 0, 1, ..., the backward pass the reverse;
+    // group jg holds the wave's windows jg*G ... jg*G+G-1 (bottom wave:
+    // counted with its voff empty windows).
+    const int lofs = lane < LW ? (bot ? row - st_of_lane * NV : lane) : 0;
+    auto gbase = [&](int jg) -> int { return (bot ? ngrp - 1 - jg : jg) * G * SPW; };
+    auto fwd_slot = [&](int bb) -> int { return lane < LW ? (bot ? VG - NV - bb * LW + lofs : bb * LW + lofs) : 0; };
+    auto bwd_slot = [&](int bb) -> int { return lane < LW ? (bot ? (bb + 1) * LW - NV + lofs : (G - 1 - bb) * LW + lofs) : 0; };
+    auto vbytes = [&](int elem, int bytes_per) -> uint32_t { return (uint32_t)(elem * NV * bytes_per); };
 
     using WL = tbs::WinLayout<NV, LW, TF>;
     auto wbase = [&](int w) -> uint32_t { return w >= 0 ? (uint32_t)w * wbytes : tbs::kOOB; };
@@ -356,29 +410,32 @@ __global__ __launch_bounds__(64) void fthomas_kernel(const SysDev<TF> *__restric
             e[WL::NC * WL::VW + k] = tbs::bload_nt<TF>(r, b + (uint32_t)((WL::NC * WL::VW * LW + k * LW + lane) * sizeof(TF)), 0);
     };
 
+    if (lane < NV) s_x[wave][lane] = TF(0);                             // an empty side contributes 0
     TF d[NV];
     TF eb[NB][NV];
-    // ---- forward: d_k = g_k - E_k d_{k-1}
+    // ---- forward: d_k = g_k - E_k d_{k-1} (bottom wave: d_{k+1})
 #pragma unroll
     for (int j = 0; j < NV; ++j) d[j] = TF(0);
 #pragma unroll
-    for (int p = 0; p < PF; ++p) load_row(rEF, p, eb[p]);
+    for (int p = 0; p < PF; ++p) load_row(rEF, p - voff, eb[p]);
     {
         tbs::u32x4 vr[IOT::NCH];
-        bulk_load<VG, TF, true>(rg, gbase(0, sizeof(TF)), lane, vr);
-        bulk_commit<VG, TF>(vr, lane, s_vin[0]);
+        bulk_load<VG, TF, true>(rg, vbytes(gbase(0), sizeof(TF)), lane, vr);
+        bulk_commit<VG, TF>(vr, lane, s_vin[wave][0]);
     }
     for (int jg = 0; jg < ngrp; ++jg) {
         const int cur = jg & 1;
         tbs::u32x4 vr[IOT::NCH];
-        bulk_load<VG, TF, true>(rg, gbase(jg + 1, sizeof(TF)), lane, vr);   // next group's g
+        if (jg + 1 < ngrp)                                              // next group's g
+            bulk_load<VG, TF, true>(rg, vbytes(gbase(jg + 1), sizeof(TF)), lane, vr);
 #pragma unroll
         for (int bb = 0; bb < G; ++bb) {
-            const int w = jg * G + bb;
+            const int w = jg * G + bb - voff;                           // the wave's window
             const int b = bb % NB;
             const int bn = (b + PF) % NB;
             load_row(rEF, w + PF, eb[bn]);
-            const TF gv = s_vin[cur][bb * LW + lane];
+            const int slot = fwd_slot(bb);
+            const TF gv = s_vin[wave][cur][slot];
             TF out = TF(0);
 #pragma unroll
             for (int st = 0; st < SPW; ++st) {
@@ -394,37 +451,60 @@ __global__ __launch_bounds__(64) void fthomas_kernel(const SysDev<TF> *__restric
                 for (int j = 0; j < NV; ++j) d[j] = rdlane(v, st * NV + j);
                 out = (st_of_lane == st) ? v : out;
             }
-            if (lane < LW) s_out[bb * LW + lane] = out;
+            if (lane < LW) s_out[wave][slot] = out;
+            // the wave's last step (step m-1, or m+1 for the bottom wave)
+            if (w == nw - 1 && st_of_lane == SPW - 1 && lane < LW) s_x[wave][row] = out;
         }
-        bulk_store<VG, TF>(rd, gbase(jg, sizeof(TF)), lane, s_out);
-        bulk_commit<VG, TF>(vr, lane, s_vin[cur ^ 1]);
+        bulk_store<VG, TF>(rd, vbytes(gbase(jg), sizeof(TF)), lane, s_out[wave]);
+        if (jg + 1 < ngrp) bulk_commit<VG, TF>(vr, lane, s_vin[wave][cur ^ 1]);
     }
-    __threadfence_block();
-
-    // ---- backward: d_k = d'_k - Gamma_k d_{k+1};  U_k += d_k
-    // Same group partition as the forward pass, visited last to first;
-    // windows past the line end read 0 and keep d = 0.
-    const int wtop = ngrp * G;        // one past the last (padded) window
+    __syncthreads();
+    // ---- step m: d_m = g_m - E_m d'_{m-1} - F_m d'_{m+1}, U_m += d_m
+    if (!bot && lane < NV) {
+        const tbs::rsrc_t rME = tbs::make_rsrc(S.EF + ((long)lt.w + sp.q) * WSTRIDE, wbytes);
+        const tbs::rsrc_t rMF = tbs::make_rsrc(S.GB + ((long)lt.w + sp.q) * WSTRIDE, wbytes);
+        TF em[NV], fm[NV];
+        load_row(rME, 0, em);
+        load_row(rMF, 0, fm);
+        const size_t at = (size_t)NV * (m0 + sp.m) + lane;
+        TF x0 = TF(0), x1 = TF(0);
 #pragma unroll
-    for (int j = 0; j < NV; ++j) d[j] = TF(0);
+        for (int j = 0; j < NV; ++j) {
+            x0 += em[j] * s_x[0][j];
+            x1 += fm[j] * s_x[1][j];
+        }
+        const TF v = S.g[at] - (x0 + x1);
+        s_x[2][lane] = v;
+        S.U[at] += (double)v;
+    }
+    __syncthreads();
+
+    // ---- backward: d_k = d'_k - Gamma_k d_{k+1} (bottom wave: d_{k-1}); U_k += d_k
+    // The groups in reverse, each from its last window down.  The top wave's
+    // last group may end past its last window: those windows are skipped
+    // (d must reach window nw-1 unchanged); the bottom wave's empty windows
+    // come last and change nothing that is stored.
+#pragma unroll
+    for (int j = 0; j < NV; ++j) d[j] = s_x[2][j];
+    const int wtop = ngrp * G - voff;                                   // one past the last window visited
 #pragma unroll
     for (int p = 0; p < PF; ++p) load_row(rGB, wtop - 1 - p, eb[p]);
     {
         tbs::u32x4 vr[IOT::NCH];
         tbs::u32x4 ur[IOD::NCH];
-        bulk_load<VG, TF, true>(rd, gbase(ngrp - 1, sizeof(TF)), lane, vr);
-        bulk_load<VG, double>(rU, gbase(ngrp - 1, 8), lane, ur);
-        bulk_commit<VG, TF>(vr, lane, s_vin[0]);
-        bulk_commit<VG, double>(ur, lane, s_u[0]);
+        bulk_load<VG, TF, true>(rd, vbytes(gbase(ngrp - 1), sizeof(TF)), lane, vr);
+        bulk_load<VG, double>(rU, vbytes(gbase(ngrp - 1), 8), lane, ur);
+        bulk_commit<VG, TF>(vr, lane, s_vin[wave][0]);
+        bulk_commit<VG, double>(ur, lane, s_u[wave][0]);
     }
     for (int t = 0; t < ngrp; ++t) {
         const int jg = ngrp - 1 - t;
         const int cur = t & 1;
         tbs::u32x4 vr[IOT::NCH];
         tbs::u32x4 ur[IOD::NCH];
-        if (jg > 0) {                                                   // previous group's d', U
-            bulk_load<VG, TF, true>(rd, gbase(jg - 1, sizeof(TF)), lane, vr);
-            bulk_load<VG, double>(rU, gbase(jg - 1, 8), lane, ur);
+        if (jg > 0) {                                                   // next group's d', U
+            bulk_load<VG, TF, true>(rd, vbytes(gbase(jg - 1), sizeof(TF)), lane, vr);
+            bulk_load<VG, double>(rU, vbytes(gbase(jg - 1), 8), lane, ur);
         }
 #pragma unroll
         for (int bb = 0; bb < G; ++bb) {
@@ -432,8 +512,9 @@ __global__ __launch_bounds__(64) void fthomas_kernel(const SysDev<TF> *__restric
             const int b = bb % NB;
             const int bn = (b + PF) % NB;
             load_row(rGB, w - PF, eb[bn]);
-            const int slot = (G - 1 - bb) * LW + lane;                  // window w inside group jg
-            const TF pv = s_vin[cur][slot];
+            if (w >= nw) continue;                                      // past the top wave's last window
+            const int slot = bwd_slot(bb);
+            const TF pv = s_vin[wave][cur][slot];
             TF out = TF(0);
 #pragma unroll
             for (int st = SPW - 1; st >= 0; --st) {
@@ -449,12 +530,12 @@ __global__ __launch_bounds__(64) void fthomas_kernel(const SysDev<TF> *__restric
                 for (int j = 0; j < NV; ++j) d[j] = rdlane(v, st * NV + j);
                 out = (st_of_lane == st) ? v : out;
             }
-            if (lane < LW) s_u[cur][slot] += (double)out;
+            if (lane < LW) s_u[wave][cur][slot] += (double)out;
         }
-        bulk_store<VG, double>(rU, gbase(jg, 8), lane, s_u[cur]);
+        bulk_store<VG, double>(rU, vbytes(gbase(jg), 8), lane, s_u[wave][cur]);
         if (jg > 0) {
-            bulk_commit<VG, TF>(vr, lane, s_vin[cur ^ 1]);
-            bulk_commit<VG, double>(ur, lane, s_u[cur ^ 1]);
+            bulk_commit<VG, TF>(vr, lane, s_vin[wave][cur ^ 1]);
+            bulk_commit<VG, double>(ur, lane, s_u[wave][cur ^ 1]);
         }
     }
 }
@@ -641,7 +722,14 @@ __device__ __forceinline__ double recip(double x) {
 // pivot either).  Gamma_k is the reference's DinvC, W_k = D'_k^{-1} and
 // E_k = D'_k^{-1} A_k are what the sweep kernels read.
 //
-// One wave per line, one lane per column: lanes P..P+NV-1 hold S_k (the
+// Two waves per line (split_line): wave 0 runs steps 0..m-1 as above, wave 1
+// runs steps n-1..m+1 of the same recurrence bottom-up (S_k = D_k -
+// C_{k+1} Gamma''_{k+1}, [Gamma''_k | E''_k | W_k] = S_k^{-1} [A_k | C_{k+1} |
+// I]: the same step with the roles of A and C exchanged), and wave 0 then
+// closes step m with both neighbours, S_m = D_m - A_m Gamma_{m-1} -
+// C_{m+1} Gamma''_{m+1}, [F_m | E_m | W_m] = S_m^{-1} [C_{m+1} | A_m | I]
+// (F_m is stored in Gamma's place).
+// In each wave, one lane per column: lanes P..P+NV-1 hold S_k (the
 // pivot columns), X..X+NV-1 hold C_{k+1} and end with Gamma_k, 2NV.. hold A_k
 // then E_k, 3NV.. hold I then W_k.  P and X swap every step, so the lanes
 // that end a step holding Gamma_k form S_{k+1} in place; each pivot step
@@ -669,10 +757,10 @@ struct FactorRec {
 };
 
 template <int NV, typename TF>
-__global__ __launch_bounds__(64) void ffactor_kernel(FactorRec one, int nline, const FactorRec *__restrict__ recs,
-                                                     const int2 *__restrict__ ltab) {
+__global__ __launch_bounds__(128) void ffactor_kernel(FactorRec one, int nline, const FactorRec *__restrict__ recs,
+                                                      const int2 *__restrict__ ltab) {
     constexpr int BS = NV * NV;
-    constexpr int CH = 4;                       // steps per chunk
+    constexpr int CH = NV <= 10 ? 4 : 2;        // steps per chunk (LDS: two waves, two buffers)
     constexpr int NB = 3 * CH;                  // blocks per chunk: D_k, A_k, C_{k+1} of each step
     constexpr int SB = 3 * BS;                  // doubles per step
     constexpr int NLD = (CH * SB + 63) / 64;    // loads per lane per chunk
@@ -681,9 +769,9 @@ __global__ __launch_bounds__(64) void ffactor_kernel(FactorRec one, int nline, c
     using WS = tbs::Soa64<BS, TF>;
     static_assert(4 * NV <= 64, "lane roles need 4*NV lanes");
     static_assert(BS <= 256, "entry index kept in 8 bits");
-    __shared__ double sblk[2][CH * SB];
+    __shared__ double sblk_all[2][2][CH * SB];  // [wave][buffer]
     __shared__ double sI[BS];                   // identity: the W lanes' starting column
-    __shared__ const double *sptr[2][NB];       // sources of a chunk's blocks
+    __shared__ const double *sptr_all[2][2][NB];  // [wave][buffer]: sources of a chunk's blocks
 
     if ((int)blockIdx.x >= nline) return;
     int l = blockIdx.x;
@@ -704,17 +792,31 @@ __global__ __launch_bounds__(64) void ffactor_kernel(FactorRec one, int nline, c
     TF *__restrict__ W = (TF *)fr.W;
     TF *__restrict__ EF = (TF *)fr.EF;
     TF *__restrict__ GB = (TF *)fr.GB;
-    const int lane = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const bool bot = wave == 1;
+    const int lane = threadIdx.x & 63;
+    auto sblk = sblk_all[wave];
+    auto sptr = sptr_all[wave];
+    // Gamma''_{m+1} for step m, from the bottom wave's last step, in its
+    // block buffer (no longer read by then)
+    double *sG = &sblk_all[1][0][0];
     // wave-uniform by construction: the chunk and step loops must be scalar
     // loops (the pivot broadcasts read lanes that every step writes)
     const int m0 = __builtin_amdgcn_readfirstlane(linepoint[l]);
     const int n = __builtin_amdgcn_readfirstlane(linesize[l]);
-    const long wbase = (long)__builtin_amdgcn_readfirstlane(woff[l]) * WL::STRIDE;
+    const Split sp = split_line(n, SPW);
+    const int nsteps = bot ? sp.B : sp.m + 1;   // the top wave's last step is step m
+    // the wave's windows: top q + 1 (the last holds step m), bottom nb
+    const long wbase = ((long)__builtin_amdgcn_readfirstlane(woff[l]) + (bot ? sp.q + 1 : 0)) * WL::STRIDE;
+    // a block of 128 threads advances its two waves' LDS use apart: within a
+    // wave, LDS written by some lanes is read by others in program order
+    auto wsync = []() { __builtin_amdgcn_wave_barrier(); };
 
     // ---- per-lane constants
     const int grp = lane / NV;                  // role group 0..3 (4+: idle lanes)
     const int j = lane - grp * NV;              // the lane's column
-    for (int t = lane; t < BS; t += 64) sI[t] = (t % NV == t / NV) ? 1.0 : 0.0;
+    if (!bot)
+        for (int t = lane; t < BS; t += 64) sI[t] = (t % NV == t / NV) ? 1.0 : 0.0;
     // chunk entry t = lane + 64 r: block (step * 3 + which) << 8 | entry
     int code[NLD];
 #pragma unroll
@@ -740,14 +842,19 @@ __global__ __launch_bounds__(64) void ffactor_kernel(FactorRec one, int nline, c
     }
     const double fP[2] = {grp == 0 ? 1.0 : 0.0, grp == 1 ? 1.0 : 0.0};   // pivot lanes at parity 0 / 1
 
-    // lane t < NB: block t of a chunk is (step t / 3, which = t % 3) and its
-    // source is component `which` of fidx[k] ({element, A source, C source}),
-    // with k the step for D and A, the next step for C_{k+1}; -1: none
+    // lane t < NB: block t of a chunk is (wave step t / 3, slot t % 3) and
+    // its source is a component of fidx ({element, A source, C source of the
+    // face before}): top wave, line step k = s: slots D_k, A_k, C_{k+1}
+    // (fidx[k].x, fidx[k].y, fidx[k+1].z); bottom wave, k = n-1-s: slots D_k,
+    // C_{k+1}, A_k (fidx[k].x, fidx[k+1].z, fidx[k].y); -1: none
     const int lst = lane / 3, lwhich = lane - 3 * lst;
+    const int nextslot = bot ? 1 : 2;
     auto load_idx = [&](int c) -> int {
-        const int k = c * CH + lst + (lwhich == 2 ? 1 : 0);
+        const int s = c * CH + lst;
+        const int k = (bot ? n - 1 - s : s) + (lwhich == nextslot ? 1 : 0);
+        const int comp = lwhich == 0 ? 0 : (lwhich == nextslot ? 2 : 1);
         int code_src = -1;
-        if (lane < NB && k < n) code_src = reinterpret_cast<const int *>(fidx + m0 + k)[lwhich];
+        if (lane < NB && s < nsteps && k < n) code_src = reinterpret_cast<const int *>(fidx + m0 + k)[comp];
         return code_src;
     };
     // (pointers are chosen by arithmetic on their values: a select between
@@ -786,11 +893,11 @@ __global__ __launch_bounds__(64) void ffactor_kernel(FactorRec one, int nline, c
     double R[NLD];
     // prologue: chunk 0 in LDS, chunk 1 in flight, indices of chunk 2 in flight
     set_ptrs(0, load_idx(0));
-    __syncthreads();
+    __syncthreads();                            // sI, and wave-local sptr
     load_blocks(0, R);
     store_blocks(0, R);
     set_ptrs(1, load_idx(1));
-    __syncthreads();
+    wsync();
     load_blocks(1, R);
     int xnext = load_idx(2);
 
@@ -799,9 +906,10 @@ __global__ __launch_bounds__(64) void ffactor_kernel(FactorRec one, int nline, c
     for (int i = 0; i < NV; ++i) col[i] = 0.0;
 
     const uintptr_t aW = (uintptr_t)W, aEF = (uintptr_t)EF, dGB = (uintptr_t)GB - (uintptr_t)EF;
-    auto step = [&](auto par, int b, int s, int k) {
+    auto step = [&](auto par, int b, int s, int sw) {
         constexpr int PAR = decltype(par)::value;
         constexpr int P = PAR ? NV : 0;
+        const int k = bot ? n - 1 - sw : sw;    // line step of wave step sw
         const double *Dk = &sblk[b][s * SB];
         const double *Ak = Dk + BS;
         // the lane's new column: pivot lanes D_k, X lanes C_{k+1}, A lanes
@@ -834,14 +942,18 @@ __global__ __launch_bounds__(64) void ffactor_kernel(FactorRec one, int nline, c
                 if (i != p) col[i] -= cpv[i] * x;
             col[p] = x;
         }
-        // outputs: Gamma_k (X lanes, k <= n-2), E_k (A lanes, k >= 1), W_k
-        const int w = k / SPW, st = k - w * SPW;
+        // outputs: Gamma_k (X lanes), E_k (A lanes), W_k, at the wave's
+        // window position of the step (the bottom wave's windows start with
+        // pad empty steps)
+        const int pos = bot ? sw + sp.pad : sw;
+        const int w = pos / SPW, st = pos - w * SPW;
         const int mm = m0 + k;
         const bool isW = grp == 3;
-        // every window entry of a real step is written: E_0 and Gamma_{n-1}
-        // (no neighbour) as zeros, so the factor arrays need no zero fill
+        // every window entry of a real step is written: E_0, E''_{n-1} and
+        // Gamma_{n-1} (no neighbour) as zeros, so the factor arrays need no
+        // zero fill
         const bool act = grp == xg || grp == 2 || isW;
-        const bool zero = (grp == xg && k == n - 1) || (grp == 2 && k == 0);
+        const bool zero = bot ? (grp == 2 && sw == 0) : ((grp == xg && k == n - 1) || (grp == 2 && k == 0));
         const uintptr_t wdst = aW + (uintptr_t)((size_t)(mm / kGroup) * BS * kGroup * sizeof(TF));
         const uintptr_t fdst = aEF + (uintptr_t)(grp == xg) * dGB +
                                (uintptr_t)((wbase + (long)w * WL::STRIDE) * (long)sizeof(TF));
@@ -852,44 +964,83 @@ __global__ __launch_bounds__(64) void ffactor_kernel(FactorRec one, int nline, c
 #pragma unroll
             for (int i = 0; i < NV; ++i) dst[u[i] + sc * v[i]] = zero ? TF(0) : (TF)col[i];
         }
-        // the window's alignment padding, once per window (never read as a
-        // factor; written so the arrays hold no stale memory)
-        if (WL::STRIDE > WL::ELEMS && st == 0 && grp == 2 && j < WL::STRIDE - WL::ELEMS) {
-            const long at = wbase + (long)w * WL::STRIDE + WL::ELEMS + j;
-            ((gdst)(TF *)aEF)[at] = TF(0);
-            ((gdst)(TF *)(aEF + dGB))[at] = TF(0);
-        }
     };
 
     // one uniform loop over step pairs (CH is even); at the first step of
     // chunk c > 0: chunk c into LDS, chunk c+1's blocks and chunk c+2's
     // indices into flight
-    for (int k = 0; k < n; k += 2) {
+    // step m (top wave): wait for the bottom wave's last Gamma'' (the one
+    // block-wide barrier of the two waves after the start), then D_m -=
+    // C_{m+1} Gamma''_{m+1} in the chunk's copy of D_m, so that the plain
+    // step gives S_m = D_m - C_{m+1} Gamma''_{m+1} - A_m Gamma_{m-1}
+    auto middle = [&](int b, int s) {
+        __syncthreads();
+        double *Dm = &sblk[b][s * SB];
+        const double *Cm = Dm + 2 * BS;
+        for (int t = lane; t < BS; t += 64) {
+            const int i = t % NV, jj = t / NV;
+            double tot = 0.0;
+#pragma unroll
+            for (int mm = 0; mm < NV; ++mm) tot += Cm[i + NV * mm] * sG[NV * jj + mm];
+            Dm[t] -= tot;
+        }
+        wsync();
+    };
+    for (int k = 0; k < nsteps; k += 2) {
         const int c = k / CH, s = k - c * CH;
         if (s == 0 && c > 0) {
-            __syncthreads();                            // LDS of chunk c-1 read
+            wsync();                                    // LDS of chunk c-1 read
             store_blocks(c & 1, R);                     // chunk c
             set_ptrs((c + 1) & 1, xnext);               // sources of chunk c+1
-            __syncthreads();
+            wsync();
             load_blocks((c + 1) & 1, R);
             xnext = load_idx(c + 2);
         }
+        if (!bot && k == sp.m) middle(c & 1, s);
         step(std::integral_constant<int, 0>(), c & 1, s, k);
-        if (k + 1 < n) step(std::integral_constant<int, 1>(), c & 1, s + 1, k + 1);
+        if (k + 1 < nsteps) {
+            if (!bot && k + 1 == sp.m) middle(c & 1, s + 1);
+            step(std::integral_constant<int, 1>(), c & 1, s + 1, k + 1);
+        }
     }
-    // steps past the line end in its last window: E and Gamma zero (the
-    // Thomas sweep runs whole windows and needs finite factors there)
-    for (int k = n; k < (n + SPW - 1) / SPW * SPW; ++k) {
+    if (bot) {
+        // Gamma''_{m+1}: the X lanes of the last step (group 1 after an even
+        // step, 0 after an odd one); zeros when the bottom side is empty
+        wsync();
+        const int xl = sp.B == 0 ? -1 : (((sp.B - 1) & 1) ? 0 : 1);
+        if (xl < 0) {
+            for (int t = lane; t < BS; t += 64) sG[t] = 0.0;
+        } else if (grp == xl) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) sG[NV * j + i] = col[i];
+        }
+        __syncthreads();
+    }
+    // empty steps of the wave's windows (top: window q after step m; bottom:
+    // the first pad steps of window 0): E and Gamma zero (the Thomas sweep
+    // runs whole windows and needs finite factors there); and each window's
+    // alignment padding (never read as a factor; written so the arrays hold
+    // no stale memory)
+    using gdst = __attribute__((address_space(1))) TF *;
+    const int e_lo = bot ? 0 : 1, e_hi = bot ? sp.pad : SPW;
+    const int wlast = bot ? 0 : sp.q;
+    for (int st = e_lo; st < e_hi; ++st) {
         if (grp == 2) {
-            const int w = k / SPW, st = k - w * SPW;
-            using gdst = __attribute__((address_space(1))) TF *;
-            const uintptr_t base = (uintptr_t)((wbase + (long)w * WL::STRIDE) * (long)sizeof(TF));
+            const uintptr_t base = (uintptr_t)((wbase + (long)wlast * WL::STRIDE) * (long)sizeof(TF));
             const gdst e = (gdst)(TF *)(aEF + base), g = (gdst)(TF *)(aEF + dGB + base);
 #pragma unroll
             for (int i = 0; i < NV; ++i) {
                 e[u[i] + st * v[i]] = TF(0);
                 g[u[i] + st * v[i]] = TF(0);
             }
+        }
+    }
+    if (WL::STRIDE > WL::ELEMS && grp == 2 && j < WL::STRIDE - WL::ELEMS) {
+        const int nwin = bot ? sp.nb : sp.q + 1;
+        for (int w = 0; w < nwin; ++w) {
+            const long at = wbase + (long)w * WL::STRIDE + WL::ELEMS + j;
+            ((gdst)(TF *)aEF)[at] = TF(0);
+            ((gdst)(TF *)(aEF + dGB))[at] = TF(0);
         }
     }
 }
@@ -1160,7 +1311,7 @@ Layout layout_of(const tbs_mesh_host *m) {
     int w = 0;
     for (int l = 0; l < m->nline; ++l) {
         L.woff[l] = w;
-        w += (m->linesize[l] + spw - 1) / spw;
+        w += line_windows(m->linesize[l], spw);
     }
     L.woff[m->nline] = w;
     L.nwin = w;
@@ -1302,20 +1453,20 @@ int launch_residual(hipStream_t st, const void *sys, const int2 *gtab, int ng, i
 // Thomas variants the autotuner picks from: {prefetch windows PF, ring
 // cycles K per loop iteration}; index 0 is the default.
 struct TV { int pf, k; };
-template <int NV, typename TF> struct ThomasV { static constexpr int n = 1; static constexpr TV v[1] = {{3, 2}}; };
-// (LDS per line stays <= 31 KiB so five lines fit a CU.)
+template <int NV, typename TF> struct ThomasV { static constexpr int n = 1; static constexpr TV v[1] = {{2, 2}}; };
+// (LDS per two-wave workgroup stays <= 32 KiB so five lines fit a CU.)
 template <> struct ThomasV<9, double> {
     static constexpr int n = 4;
-    static constexpr TV v[4] = {{3, 2}, {2, 3}, {5, 2}, {2, 2}};
+    static constexpr TV v[4] = {{2, 2}, {3, 1}, {1, 3}, {1, 2}};
 };
 template <> struct ThomasV<9, float> {
     static constexpr int n = 4;
-    static constexpr TV v[4] = {{3, 3}, {5, 2}, {7, 2}, {3, 2}};
+    static constexpr TV v[4] = {{3, 2}, {2, 2}, {5, 1}, {3, 1}};
 };
 
 template <int NV, typename TF, int PF, int K>
 int launch_thomas_pf(hipStream_t st, const void *sys, const int4 *ltab, int nl, hipEvent_t e0, hipEvent_t e1) {
-    launch_k(fthomas_kernel<NV, TF, PF, K>, dim3(nl), dim3(64), st, e0, e1, (const SysDev<TF> *)sys, ltab, nl);
+    launch_k(fthomas_kernel<NV, TF, PF, K>, dim3(nl), dim3(128), st, e0, e1, (const SysDev<TF> *)sys, ltab, nl);
     TBS_LAUNCHED("fthomas_kernel");
     return TBS_OK;
 }
@@ -1345,7 +1496,7 @@ int thomas_variants(int nv, int prec) {
 
 // encoded as 100 * PF + K
 int thomas_pf_of(int nv, int prec, int variant) {
-    TV t = {3, 2};
+    TV t = {2, 2};
     if (nv == 9) t = prec == TBS_FP32 ? ThomasV<9, float>::v[variant % ThomasV<9, float>::n]
                                       : ThomasV<9, double>::v[variant % ThomasV<9, double>::n];
     return 100 * t.pf + t.k;
@@ -1384,9 +1535,9 @@ int launch_factor(int nv, int prec, hipStream_t st, const FactorRec &one, int nl
     if (nline == 0) return TBS_OK;
     switch (nv) {
 #define X(v) case v: \
-        if (prec == TBS_FP32) hipLaunchKernelGGL((ffactor_kernel<v, float>), dim3(nline), dim3(64), 0, st, one, \
+        if (prec == TBS_FP32) hipLaunchKernelGGL((ffactor_kernel<v, float>), dim3(nline), dim3(128), 0, st, one, \
                                                  nline, recs, ltab); \
-        else hipLaunchKernelGGL((ffactor_kernel<v, double>), dim3(nline), dim3(64), 0, st, one, nline, recs, ltab); \
+        else hipLaunchKernelGGL((ffactor_kernel<v, double>), dim3(nline), dim3(128), 0, st, one, nline, recs, ltab); \
         break;
         TBS_PLAN_NVARS(X)
 #undef X
