@@ -141,6 +141,29 @@ def test_fp32_factors_reach_fp64_accuracy(tbs, ctxs, oracle):
     assert relerr(o32["U"], ref["U"]) < 1e-9
 
 
+@pytest.mark.parametrize("nv", [2, 3, 5, 9, 16])
+def test_one_sweep_is_the_line_solve(tbs, ctxs, nv):
+    """From U = 0 one sweep is U = -M^-1 rhs exactly (M: the lines' block
+    tridiagonal, assembled and solved by scipy), whatever the line lengths:
+    pins the two-ended factorisation and Thomas pass, whose split point,
+    window padding and empty halves all depend on the length (window of
+    64 // nv steps).  Lengths around 1, 2, 3 windows and their halves, both
+    line orientations; FP64 factors 1e-13, FP32 factors 1e-5."""
+    import scipy.sparse.linalg as spla
+    spw = 64 // nv
+    lens = sorted({1, 2, 3, spw - 1, spw, spw + 1, 2 * spw, 2 * spw + 1, 3 * spw - 1, 3 * spw + 2,
+                   4 * spw, 5 * spw + 3, 7 * spw + 1, 13 * spw, 97, 130} - {0})
+    print(f"case one sweep nv {nv} lengths {lens}", flush=True)
+    base = ragged_lines(lens, nv)
+    for m in (base, reversed_lines(base)):
+        jac = synth_np.generate(7, nv, m.nelem, m.nintface, 2.0 * nv + 2.0)
+        ref = -spla.splu(cf.assemble_M(m, jac).tocsc()).solve(jac["rhs"])
+        for prec, tol in ((0, 1e-13), (1, 1e-5)):
+            out = solve_plan(tbs, ctxs(nv), m, jac, 1, prec=prec)
+            assert out["ns"] == 1
+            assert relerr(out["U"], ref) < tol, (prec, relerr(out["U"], ref))
+
+
 def test_second_solve_equals_first(tbs, ctxs):
     """A plan's first solve launches its sweeps eagerly, later ones replay
     captured graphs: the same history and U, bitwise."""
