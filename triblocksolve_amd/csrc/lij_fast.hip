@@ -741,7 +741,8 @@ __device__ __forceinline__ double recip(double x) {
 // S_k update with its old column scaled by 1 (pivot lanes) or 0 (the rest),
 // and the outputs go out through per-lane destination offsets
 // u_i + s * v_i precomputed for the lane's role.  The D, A, C blocks of 4
-// steps at a time are prefetched into double-buffered LDS one chunk ahead;
+// steps at a time (2 for blocks larger than 10x10) are prefetched into
+// double-buffered LDS one chunk ahead;
 // their sources come from the per-element table fidx = {e, A source, C
 // source of the face before} (source = 2 * face + 1 for Of2, 2 * face for
 // Of1, -1 none: the zero block).
