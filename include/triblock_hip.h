@@ -297,6 +297,12 @@ int tbs_server_destroy(tbs_server *srv);
 int tbs_server_footprint(tbs_server *srv, const tbs_server_request *req, size_t *bytes);
 /* TBS_ERR_NOMEM if the request alone exceeds the budget. */
 int tbs_server_submit(tbs_server *srv, const tbs_server_request *req, long *id);
+/* One scheduling step: collect the running batch's states, let finished
+ * systems leave and set-up-complete admissions join, launch the next
+ * sweeps, start copying finished results out (a request is reported done
+ * by a later step, once its copies have completed; at once when nothing
+ * else is in flight), admit from the queue.  nactive counts running and
+ * retiring requests. */
 int tbs_server_step(tbs_server *srv, int *nactive, int *nqueued, int *nadmitting);
 /* ids of requests finished since the last poll */
 int tbs_server_poll(tbs_server *srv, long *ids, int cap, int *n);

@@ -202,6 +202,7 @@ struct tbs_server {
     }
     std::deque<std::unique_ptr<Request>> queue;
     std::vector<std::unique_ptr<Request>> admitting, active;
+    std::vector<std::unique_ptr<Request>> retiring;   // results being copied out on the retirement stream
     std::map<long, std::unique_ptr<Request>> finished;
     std::vector<long> completed_unpolled;
     tbs_batch *batch = nullptr;
@@ -315,6 +316,32 @@ void finish(tbs_server *s, std::unique_ptr<Request> r) {
     s->finished[id] = std::move(r);
 }
 
+// Finish the retiring requests whose copies have completed (all of them,
+// waiting, when wait is set); their slabs go back to the arena only then.
+int reap(tbs_server *s, bool wait) {
+    std::vector<std::unique_ptr<Request>> keep;
+    int rc = TBS_OK;
+    for (auto &r : s->retiring) {
+        if (wait) {
+            const hipError_t e = hipEventSynchronize(r->ready);
+            if (e != hipSuccess && rc == TBS_OK) {
+                tbs::set_error("retirement copy: %s", hipGetErrorString(e));
+                rc = TBS_ERR_STATE;
+            }
+        } else if (hipEventQuery(r->ready) != hipSuccess) {
+            keep.push_back(std::move(r));
+            continue;
+        }
+        s->used -= r->slab_len;
+        drop_device(s, r.get());
+        s->count[C_RETIRED]++;
+        finish(s, std::move(r));
+    }
+    hipGetLastError();   // hipEventQuery reports "not ready" as an error code
+    s->retiring.swap(keep);
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -351,6 +378,7 @@ int tbs_server_destroy(tbs_server *s) {
     if (s->batch) tbs_batch_destroy(s->batch);
     for (auto &r : s->active) drop_device(s, r.get());
     for (auto &r : s->admitting) drop_device(s, r.get());
+    for (auto &r : s->retiring) drop_device(s, r.get());
     for (char *b : s->seg_base) hipFree(b);
     s->finished.clear();
     s->pinned.clear();
@@ -435,7 +463,9 @@ int tbs_server_step(tbs_server *s, int *nactive, int *nqueued, int *nadmitting) 
     if (!s->active.empty()) TBS_TRY(tbs_batch_launch_sweeps(s->batch, s->k, s->max_sweeps, s->rtol));
     lap(T_LAUNCH);
 
-    // 4. retire on the retirement stream; no queued kernel touches these systems
+    // 4. retire on the retirement stream (no queued kernel touches these
+    //    systems), without waiting: a request finishes at the first step that
+    //    finds its copies complete, or at once when nothing else is in flight
     if (!done.empty()) {
         hipStream_t rs = s->ret->stream;
         for (auto &r : done) {
@@ -446,15 +476,12 @@ int tbs_server_step(tbs_server *s, int *nactive, int *nqueued, int *nadmitting) 
             TBS_HIP(hipMemcpyAsync(r->hU, r->d_U, r->nU * 8, hipMemcpyDeviceToHost, rs));
             TBS_HIP(hipMemcpyAsync(r->hist.data(), tbs::plan_hist(r->plan), r->hist.size() * 8,
                                    hipMemcpyDeviceToHost, rs));
-        }
-        TBS_HIP(hipStreamSynchronize(rs));
-        for (auto &r : done) {
-            s->used -= r->slab_len;
-            drop_device(s, r.get());
-            s->count[C_RETIRED]++;
-            finish(s, std::move(r));
+            TBS_HIP(hipEventRecord(r->ready, rs));
+            s->retiring.push_back(std::move(r));
         }
     }
+    const bool idle = s->active.empty() && s->admitting.empty() && s->queue.empty();
+    TBS_TRY(reap(s, idle));
     lap(T_RETIRE);
 
     // 5. admission: FIFO; later requests may pass a head that does not fit
@@ -505,7 +532,7 @@ int tbs_server_step(tbs_server *s, int *nactive, int *nqueued, int *nadmitting) 
     }
     lap(T_ADMIT);
 
-    if (nactive) *nactive = (int)s->active.size();
+    if (nactive) *nactive = (int)(s->active.size() + s->retiring.size());
     if (nqueued) *nqueued = (int)s->queue.size();
     if (nadmitting) *nadmitting = (int)s->admitting.size();
     return TBS_OK;
