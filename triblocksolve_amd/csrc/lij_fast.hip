@@ -1211,6 +1211,10 @@ struct tbs_batch {
     int4 *d_ltab = nullptr;
     int2 *d_gtab = nullptr;
     int nl = 0, ng = 0;
+    // eager FP64 sweeps run in two halves on the context's two streams: systems
+    // [0, nsA) with lines [0, nlA) and groups [0, ngA) of the tables, the
+    // rest after them (nsA = 0: one stream)
+    int nsA = 0, nlA = 0, ngA = 0;
     std::map<std::pair<int, std::pair<int, double>>, hipGraphExec_t> graphs;
     int *h_states = nullptr;    // pinned [cap_sys][4]
     int *d_states = nullptr;    // [cap_sys][4] gathered system states
@@ -1619,11 +1623,15 @@ bool plan_nvar_ok(int nv) {
 }
 
 // One sweep: Thomas (+U update), residual, norm/convergence.
-int enqueue_sweep(int nv, int prec, hipStream_t st, const void *sys, int nsys, const int4 *ltab,
-                  int nl, const int2 *gtab, int ng, int max_sweeps, double rtol, int variant) {
+int batch_sweeps(tbs_batch *b, hipStream_t st, int nsweeps, int max_sweeps, double rtol, bool split);
+
+// One sweep of the systems the tables ltab[0, nl) and gtab[0, ng) cover
+// (they index sys by system); the norm step runs over sys_final[0, nsys).
+int enqueue_sweep(int nv, int prec, hipStream_t st, const void *sys, const void *sys_final, int nsys,
+                  const int4 *ltab, int nl, const int2 *gtab, int ng, int max_sweeps, double rtol, int variant) {
     TBS_TRY(k_thomas(nv, prec, st, sys, ltab, nl, variant));
     TBS_TRY(k_residual(nv, prec, st, sys, gtab, ng, 0, 0));
-    TBS_TRY(k_final(prec, st, sys, nsys, max_sweeps, rtol, 0));
+    TBS_TRY(k_final(prec, st, sys_final, nsys, max_sweeps, rtol, 0));
     return TBS_OK;
 }
 
@@ -2020,7 +2028,7 @@ int tbs_plan_launch_sweeps(tbs_plan *p, int nsweeps, int use_graph) {
     TBS_TRY(tbs::use_ctx(p->ctx));
     hipStream_t st = p->ctx->stream;
     auto one = [p, st]() {
-        return enqueue_sweep(p->nv, p->prec, st, p->d_sys, 1, p->d_ltab, p->nline, p->d_gtab,
+        return enqueue_sweep(p->nv, p->prec, st, p->d_sys, p->d_sys, 1, p->d_ltab, p->nline, p->d_gtab,
                              p->ngroups, p->max_sweeps, p->rtol, p->variant);
     };
     if (!use_graph) {
@@ -2131,6 +2139,41 @@ int autotune_thomas(tbs_ctx *ctx, int nv, int prec, const void *sys, int nsys, c
     TBS_HIP(hipStreamSynchronize(st));
     if (chosen) *chosen = best;
     return TBS_OK;
+}
+
+// nsweeps sweeps of a batch, enqueued on st.  With split, a batch in two
+// halves forks the second half onto the context's second stream once and
+// joins it back after the last sweep, so the halves drift apart within the
+// run: each half's kernels follow each other, and every system's arithmetic
+// is the same as on one stream.  (Captured into a graph, the two branches
+// made a solve slower, so graphs are captured on one stream.)
+int batch_sweeps(tbs_batch *b, hipStream_t st, int nsweeps, int max_sweeps, double rtol, bool split) {
+    const int ns = (int)b->plans.size();
+    if (ns == 0 || nsweeps == 0) return TBS_OK;
+    if (!split || b->nsA <= 0 || b->nsA >= ns) {
+        for (int i = 0; i < nsweeps; ++i)
+            TBS_TRY(enqueue_sweep(b->nv, b->prec, st, b->d_sys, b->d_sys, ns, b->d_ltab, b->nl, b->d_gtab, b->ng,
+                                  max_sweeps, rtol, b->variant));
+        return TBS_OK;
+    }
+    hipStream_t aux;
+    TBS_TRY(tbs::aux_stream(b->ctx, &aux));
+    const size_t ssz = b->prec == TBS_FP32 ? sizeof(SysDev<float>) : sizeof(SysDev<double>);
+    const void *sysB = (const char *)b->d_sys + ssz * (size_t)b->nsA;
+    TBS_HIP(hipEventRecord(b->ctx->ev_fork, st));
+    TBS_HIP(hipStreamWaitEvent(aux, b->ctx->ev_fork, 0));
+    int rc = TBS_OK;
+    for (int i = 0; i < nsweeps && rc == TBS_OK; ++i) {
+        rc = enqueue_sweep(b->nv, b->prec, st, b->d_sys, b->d_sys, b->nsA, b->d_ltab, b->nlA, b->d_gtab, b->ngA,
+                           max_sweeps, rtol, b->variant);
+        if (rc == TBS_OK)
+            rc = enqueue_sweep(b->nv, b->prec, aux, b->d_sys, sysB, ns - b->nsA, b->d_ltab + b->nlA, b->nl - b->nlA,
+                               b->d_gtab + b->ngA, b->ng - b->ngA, max_sweeps, rtol, b->variant);
+    }
+    // join even after a failed launch, so that a capture on st can end
+    TBS_HIP(hipEventRecord(b->ctx->ev_join, aux));
+    TBS_HIP(hipStreamWaitEvent(st, b->ctx->ev_join, 0));
+    return rc;
 }
 }  // namespace
 
@@ -2248,17 +2291,34 @@ int tbs_batch_set_plans(tbs_batch *b, tbs_plan *const *plans, int nplans) {
         for (auto &kv : b->graphs) hipGraphExecDestroy(kv.second);
         b->graphs.clear();
     }
-    // tables: lines longest-first so the longest chains start first
+    // FP64 factors: the systems in two halves of about equal element count,
+    // run on two streams (each half's Thomas pass overlaps the other's
+    // residual; with FP32 factors one stream is as fast)
+    int nsA = 0;
+    if (b->prec == TBS_FP64 && nplans >= 2) {
+        long tot = 0, acc = 0;
+        for (int s2 = 0; s2 < nplans; ++s2) tot += plans[s2]->nle;
+        while (nsA < nplans - 1 && 2 * (acc + plans[nsA]->nle) <= tot) acc += plans[nsA++]->nle;
+        nsA = std::max(nsA, 1);
+    }
+    // tables: each half's lines longest-first so the longest chains start
+    // first; groups in system order
     std::vector<std::pair<int, int4>> lines;
     std::vector<int2> gtab;
+    size_t nlA = 0, ngA = 0;
+    auto longest_first = [](const std::pair<int, int4> &x, const std::pair<int, int4> &y) { return x.first > y.first; };
     for (int s2 = 0; s2 < nplans; ++s2) {
+        if (s2 == nsA && nsA > 0) {
+            std::stable_sort(lines.begin(), lines.end(), longest_first);
+            nlA = lines.size();
+            ngA = gtab.size();
+        }
         const tbs_plan *p = plans[s2];
         for (int l = 0; l < p->nline; ++l)
             lines.push_back({p->h_linesize[l], make_int4(s2, p->h_linepoint[l], p->h_linesize[l], p->h_woff[l])});
         for (int g = 0; g < p->ngroups; ++g) gtab.push_back(make_int2(s2, g));
     }
-    std::stable_sort(lines.begin(), lines.end(),
-                     [](const std::pair<int, int4> &x, const std::pair<int, int4> &y) { return x.first > y.first; });
+    std::stable_sort(lines.begin() + nlA, lines.end(), longest_first);
     const size_t ssz = b->prec == TBS_FP32 ? sizeof(SysDev<float>) : sizeof(SysDev<double>);
     const size_t nl = lines.size(), ng = gtab.size(), ns = (size_t)nplans;
     // grow the device tables (after the work that reads them has finished);
@@ -2316,6 +2376,9 @@ int tbs_batch_set_plans(tbs_batch *b, tbs_plan *const *plans, int nplans) {
     b->plans.assign(plans, plans + nplans);
     b->nl = (int)nl;
     b->ng = (int)ng;
+    b->nsA = nsA;
+    b->nlA = (int)nlA;
+    b->ngA = (int)ngA;
     return TBS_OK;
 }
 
@@ -2347,11 +2410,7 @@ int tbs_batch_launch_sweeps(tbs_batch *b, int nsweeps, int max_sweeps, double rt
     TBS_REQUIRE(b && nsweeps >= 0 && max_sweeps >= 0 && max_sweeps < kHistCap, TBS_ERR_ARG, "bad arguments");
     TBS_TRY(tbs::use_ctx(b->ctx));
     if (b->plans.empty()) return TBS_OK;
-    hipStream_t st = b->ctx->stream;
-    for (int i = 0; i < nsweeps; ++i)
-        TBS_TRY(enqueue_sweep(b->nv, b->prec, st, b->d_sys, (int)b->plans.size(), b->d_ltab, b->nl, b->d_gtab,
-                              b->ng, max_sweeps, rtol, b->variant));
-    return TBS_OK;
+    return batch_sweeps(b, b->ctx->stream, nsweeps, max_sweeps, rtol, true);
 }
 
 int tbs_batch_states(tbs_batch *b, int *states) {
@@ -2415,11 +2474,10 @@ int tbs_batch_solve(tbs_batch *b, const double *const *rhs, double *const *U, in
     auto it = b->graphs.find(key);
     if (it == b->graphs.end()) {
         hipGraphExec_t ge = nullptr;
-        auto one = [b, st, ns, max_sweeps, rtol]() {
-            return enqueue_sweep(b->nv, b->prec, st, b->d_sys, ns, b->d_ltab, b->nl, b->d_gtab, b->ng,
-                                 max_sweeps, rtol, b->variant);
+        auto all = [b, st, sweeps_per_check, max_sweeps, rtol]() {
+            return batch_sweeps(b, st, sweeps_per_check, max_sweeps, rtol, false);
         };
-        TBS_TRY(capture_sweeps(st, sweeps_per_check, &ge, one));
+        TBS_TRY(capture_sweeps(st, 1, &ge, all));
         it = b->graphs.emplace(key, ge).first;
     }
     int done = 0;
