@@ -1965,7 +1965,7 @@ int tbs_plans_factor(tbs_ctx *ctx, tbs_plan *const *plans, int n, const double *
                               (const int2 *)(ctx->dstage + lt_off)));
     }
     // residual blocks, packed on the context's second stream while the
-    // factor kernel (latency-bound, one wave per line) runs on the main one:
+    // factor kernel (latency-bound, two waves per line) runs on the main one:
     // queued after the factor kernel (its waves take their CUs first) and
     // forked after the memsets; the main stream waits for the packing before
     // anything after the call
