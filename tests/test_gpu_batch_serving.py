@@ -130,16 +130,18 @@ def test_plan_footprint_is_exact(tbs, ctx):
         p.close()
 
 
-def test_batch_membership_changes_in_place(tbs, ctx):
+@pytest.mark.parametrize("prec", [1, 0], ids=["fp32", "fp64"])
+def test_batch_membership_changes_in_place(tbs, ctx, prec):
     """Systems leave and join a running batch between sweeps; every system's
-    history and U stay bitwise those of its solo solve."""
-    print("case batch set_plans", flush=True)
+    history and U stay bitwise those of its solo solve (FP64 batches run their
+    eager sweeps as two halves on two streams)."""
+    print(f"case batch set_plans prec {prec}", flush=True)
     specs = [(5, 200, 41), (12, 96, 42), (2, 700, 43), (30, 64, 44)]
     systems = []
     for NL, NK, seed in specs:
         m = tbs.structured_mesh(NL, NK)
         j = tbs.generate_host(seed, 9, m.nelem, m.nintface, 12.0)
-        p = tbs.Plan(ctx, m, tbs.TBS_FP32)
+        p = tbs.Plan(ctx, m, prec)
         p.factor(dev(j["jacD"]), dev(j["jacO1"]), dev(j["jacO2"]))
         rhs = dev(j["rhs"])
         systems.append((m, j, p, rhs))
@@ -163,7 +165,7 @@ def test_batch_membership_changes_in_place(tbs, ctx):
         U = zeros(9 * m.nelem)
         p.end(U)
         h = p.history()
-        h0, n0, U0 = _solve_alone(tbs, ctx, m, j, tbs.TBS_FP32, 60, 1e-10)
+        h0, n0, U0 = _solve_alone(tbs, ctx, m, j, prec, 60, 1e-10)
         assert np.array_equal(h, h0) and np.array_equal(host(U), U0)
     b.close()
     for s_ in systems:
