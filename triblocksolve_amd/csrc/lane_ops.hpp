@@ -43,6 +43,10 @@ __device__ __forceinline__ rsrc_t make_rsrc(const void *base, uint32_t bytes) {
 
 // Offset used by lanes that must neither load nor store.
 constexpr uint32_t kOOB = 0xFFFFFF00u;
+// Base offset of a whole wave access that must read zeros: it stays out of
+// range after a wave's lane offsets (a few KiB) are added to it, where kOOB
+// would wrap around to small offsets.
+constexpr uint32_t kOOBBase = 0x80000000u;
 
 template <typename T>
 __device__ __forceinline__ T bload(rsrc_t r, uint32_t voff, uint32_t soff);

@@ -396,7 +396,7 @@ __global__ __launch_bounds__(128) void fthomas_kernel(const SysDev<TF> *__restri
     auto vbytes = [&](int elem, int bytes_per) -> uint32_t { return (uint32_t)(elem * NV * bytes_per); };
 
     using WL = tbs::WinLayout<NV, LW, TF>;
-    auto wbase = [&](int w) -> uint32_t { return w >= 0 ? (uint32_t)w * wbytes : tbs::kOOB; };
+    auto wbase = [&](int w) -> uint32_t { return w >= 0 ? (uint32_t)w * wbytes : tbs::kOOBBase; };
     auto load_row = [&](tbs::rsrc_t r, int w, TF (&e)[NV]) {
         const uint32_t b = wbase(w);
 #pragma unroll
