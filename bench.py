@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, published
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nl", type=int, default=1200)
     ap.add_argument("--nk", type=int, default=1280)
