@@ -81,6 +81,14 @@ int tbs_event_create(tbs_ctx *ctx, void **ev);
 int tbs_event_destroy(tbs_ctx *ctx, void *ev);
 int tbs_event_record(tbs_ctx *ctx, void *ev);
 int tbs_event_elapsed_ms(void *ev_start, void *ev_stop, float *ms);
+/* Achievable HBM bandwidth of the context's device: a read-only stream and a
+ * copy (16-byte non-temporal loads and stores, 8 workgroups per CU) over
+ * `bytes` bytes of device memory the call allocates and frees, `reps` timed
+ * launches each after one warm-up.  gbps[0] = read GB/s, gbps[1] = copy GB/s
+ * (bytes read + bytes written).  Synchronous.  No counterpart in the
+ * reference: it is the ceiling the sweep kernels' GB/s (the reference prints
+ * GB/s at src/main.cxx:243-252) are compared with. */
+int tbs_stream_ceiling(tbs_ctx *ctx, size_t bytes, int reps, double *gbps);
 
 /* ---------------------------------------------------------------------- */
 /* Drop-in kernels: one entry per active reference OKL kernel, same       */
@@ -128,6 +136,12 @@ int tbs_triblock_lineRes(tbs_ctx *ctx, int nelem, int nintfaces, int eftot,
                          const int *linepoint, const int *lines,
                          const double *Dia, const double *Of1,
                          const double *Of2, const double *U, double *R);
+
+/* b(:,e) = L_e U_e x(:,e) for every element e < nelem, with L_e \ U_e the
+ * packed LU factors of block e (unit-lower L strictly below the diagonal, U
+ * on and above it, as tbs_lineLU leaves Dia).  The reference's dgemvLU
+ * helper, src/okl/linesmoothLU_TriBlock_v3.okl:48-69, applied per block. */
+int tbs_dgemvLU(tbs_ctx *ctx, int nelem, const double *LU, const double *x, double *b);
 
 /* ||v||_2 of n doubles (device), deterministic fixed-order reduction; the
  * reference has no norm (build-defined convergence measure).              */

@@ -92,6 +92,34 @@ void orc_solveLU(int nv, const double *A, const double *b, double *x) {
     }
 }
 
+/* b = L U x with the packed factors of orc_LU (unit-lower L below the
+ * diagonal, U on and above it), one block per element: U first (row sums
+ * from the diagonal, ascending j), then L starting from b_i = y_i and adding
+ * L(i,j) y_j for ascending j < i.  src/okl/linesmoothLU_TriBlock_v3.okl:48-69 */
+void orc_dgemvLU(int nv, int nelem, const double *LU, const double *x, double *b, int nthreads) {
+    set_threads(nthreads);
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < nelem; ++e) {
+        const double *A = LU + (size_t)nv * nv * e;
+        const double *xe = x + (size_t)nv * e;
+        double *be = b + (size_t)nv * e;
+        double y[64];
+        for (int i = 0; i < nv; ++i) {
+            double tot = 0.0;
+#if ORACLE_MUTATION == 5
+            for (int j = i + 1; j < nv; ++j) tot += MX(A, i, j) * xe[j]; /* planted: U diagonal dropped */
+#else
+            for (int j = i; j < nv; ++j) tot += MX(A, i, j) * xe[j];
+#endif
+            y[i] = tot;
+        }
+        for (int i = 0; i < nv; ++i) {
+            be[i] = y[i];
+            for (int j = 0; j < i; ++j) be[i] += MX(A, i, j) * y[j];
+        }
+    }
+}
+
 /* B[n] = A[n] over n < nelem*nv.  src/okl/linesmoothLU_v5.okl:75-83 */
 void orc_copyAtoB(int nv, int nelem, const double *A, double *B, int nthreads) {
     set_threads(nthreads);

@@ -94,7 +94,7 @@ def thomas_scalar(a, b, c, d):
     return x
 
 
-def history_close(h_test, h_ref, rel=1e-12, floor=1e-3):
+def history_close(h_test, h_ref, rel=1e-12, floor=1e-4):
     """Normalised-history parity: |h_k/h_0 - h'_k/h'_0| <= rel * max(h'_k/h'_0, floor)."""
     a = np.asarray(h_test) / h_test[0]
     b = np.asarray(h_ref) / h_ref[0]

@@ -38,6 +38,15 @@ class Oracle:
         self.lib.orc_solveLU(nv, _p(LUf), _p(b), _p(x))
         return x
 
+    def dgemvLU(self, nv, LU, x, nthreads=1):
+        """b = L U x per block (LU: packed factors, blocks by element)."""
+        LU = np.ascontiguousarray(LU, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        n = x.size // nv
+        b = np.zeros(nv * n)
+        self.lib.orc_dgemvLU(nv, n, _p(LU), _p(x), _p(b), nthreads)
+        return b
+
     def assembleTriBlocks(self, m, jacO1, jacO2, nthreads=1):
         Aout = np.zeros(m.nvar * m.nvar * m.nelem)
         self.lib.orc_assembleTriBlocks(m.nvar, m.nline, _p(m.linesize), _p(m.linepoint), _p(m.lines),

@@ -85,6 +85,19 @@ def test_kernels_match_oracle(ctxs, oracle, name, m):
     ctx.lineLU(n, m.nintface, d, Dg, O1, O2, Cg)
     assert relerr(host(Dg), Dia_ref) < RTOL
     assert relerr(host(Cg), DinvC_ref) < RTOL
+    # dgemvLU on the oracle's factors: the oracle's b = L U x, and for the
+    # first element of every line (D'_0 = LU(D_0)) the raw block product D x
+    xg = np.random.default_rng(3).uniform(-1, 1, nv * n)
+    bg = dev(np.zeros(nv * n))
+    ctx.dgemvLU(n, dev(Dia_ref), dev(xg), bg)
+    b_ref = oracle.dgemvLU(nv, Dia_ref, xg)
+    assert relerr(host(bg), b_ref) < RTOL
+    bh = host(bg)
+    for l in range(min(m.nline, 8)):
+        e = m.lines[m.linepoint[l]]
+        Dblk = jac["jacD"][nv * nv * e: nv * nv * (e + 1)].reshape(nv, nv, order="F")
+        want = Dblk @ xg[nv * e: nv * (e + 1)]
+        assert relerr(bh[nv * e: nv * (e + 1)], want) < 1e-13
     # solveDU on the oracle's factors
     R = np.random.default_rng(2).uniform(-1, 1, nv * n)
     dU_ref = oracle.solveDU(m, Dia_ref, DinvC_ref, A_ref, R)
@@ -127,7 +140,7 @@ def run_dropin_loop(ctx, m, jac, iters):
 @pytest.mark.parametrize("NL,NK,shift", [(60, 64, 20.0), (60, 64, 12.0)])
 def test_reference_loop_history(ctxs, oracle, NL, NK, shift):
     """cfg1: 10 sweeps of the reference loop; residual history within 1e-12 relative
-    (normalised history, floor 1e-3) and U within 1e-12 of the oracle."""
+    (normalised history, floor 1e-4) and U within 1e-12 of the oracle."""
     m, jac = system(NL, NK, 9, seed=1, shift=shift)
     ref = oracle.lij_run(m, jac, 10)
     hist, U = run_dropin_loop(ctxs(9), m, jac, 10)

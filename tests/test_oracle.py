@@ -183,3 +183,27 @@ def test_openmp_equals_serial(oracle):
     a = oracle.lij_run(m, jac, 4, nthreads=1)
     b = oracle.lij_run(m, jac, 4, nthreads=4)
     assert np.array_equal(a["U"], b["U"]) and np.array_equal(a["hist"], b["hist"])
+
+
+@pytest.mark.parametrize("nv", [1, 2, 5, 9, 17, 32])
+def test_dgemvLU_is_the_factored_product(oracle, nv):
+    """TB3:48-69: b = L U x with L unit-lower (strictly below the diagonal of the
+    packed block) and U on/above it, against numpy's dense product; and with the
+    factors of A from the oracle's LU, b = A x."""
+    rng = np.random.default_rng(100 + nv)
+    n = 7
+    LU = rng.uniform(-1, 1, nv * nv * n)
+    x = rng.uniform(-1, 1, nv * n)
+    b = oracle.dgemvLU(nv, LU, x)
+    for e in range(n):
+        F = LU[nv * nv * e: nv * nv * (e + 1)].reshape(nv, nv, order="F")
+        ref = (np.tril(F, -1) + np.eye(nv)) @ (np.triu(F) @ x[nv * e: nv * (e + 1)])
+        assert np.allclose(b[nv * e: nv * (e + 1)], ref, rtol=1e-13, atol=1e-13)
+    A = _dd(rng, nv)
+    F = oracle.LU(nv, A)
+    xe = rng.uniform(-1, 1, nv)
+    bb = oracle.dgemvLU(nv, F.ravel(order="F"), xe)
+    assert np.allclose(bb, A @ xe, rtol=1e-13, atol=1e-13 * np.abs(A).max())
+    # a unit-diagonal identity factor (L = I, U = I) returns x unchanged
+    I = np.eye(nv).ravel(order="F")
+    assert np.array_equal(oracle.dgemvLU(nv, I, xe), xe)

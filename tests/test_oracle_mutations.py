@@ -35,11 +35,25 @@ def _res_ok(o):
     return np.allclose(R, jac["rhs"] + cf.assemble_J(m, jac) @ U, rtol=1e-13)
 
 
+def _gemv_ok(o):
+    nv = 5
+    rng = np.random.default_rng(4)
+    LU = rng.uniform(-1, 1, nv * nv * 3)
+    x = rng.uniform(-1, 1, nv * 3)
+    b = o.dgemvLU(nv, LU, x)
+    for e in range(3):
+        F = LU[nv * nv * e: nv * nv * (e + 1)].reshape(nv, nv, order="F")
+        ref = (np.tril(F, -1) + np.eye(nv)) @ (np.triu(F) @ x[nv * e: nv * (e + 1)])
+        if not np.allclose(b[nv * e: nv * (e + 1)], ref, rtol=1e-13, atol=1e-14):
+            return False
+    return True
+
+
 def test_real_oracle_passes(oracle):
-    assert _thomas_ok(oracle) and _res_ok(oracle)
+    assert _thomas_ok(oracle) and _res_ok(oracle) and _gemv_ok(oracle)
 
 
-@pytest.mark.parametrize("n,check", [(1, _thomas_ok), (2, _thomas_ok), (3, _thomas_ok), (4, _res_ok)])
+@pytest.mark.parametrize("n,check", [(1, _thomas_ok), (2, _thomas_ok), (3, _thomas_ok), (4, _res_ok), (5, _gemv_ok)])
 def test_mutant_fails(n, check):
     o = _mutant(n)
     assert o.lib.orc_mutation() == n

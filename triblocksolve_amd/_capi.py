@@ -77,6 +77,7 @@ SIGNATURES = {
     "tbs_event_destroy": [P, P],
     "tbs_event_record": [P, P],
     "tbs_event_elapsed_ms": [P, P, C.POINTER(C.c_float)],
+    "tbs_stream_ceiling": [P, C.c_size_t, I, P],
     "tbs_copyAtoBjac": [P, I, P, P],
     "tbs_copyAtoB": [P, I, P, P],
     "tbs_addAtoB": [P, I, P, P],
@@ -84,6 +85,7 @@ SIGNATURES = {
     "tbs_lineLU": [P, I, I, I, P, P, P, P, P, P, P, P, P],
     "tbs_triblock_solveDU": [P, I, I, I, I, I, P, P, P, P, P, P, P, P],
     "tbs_triblock_lineRes": [P, I, I, I, I, I, P, P, P, P, P, P, P, P, P, P, P],
+    "tbs_dgemvLU": [P, I, P, P, P],
     "tbs_norm2": [P, L, P, C.POINTER(C.c_double)],
     "tbs_plan_footprint": [C.POINTER(MeshHost), I, C.POINTER(C.c_size_t)],
     "tbs_plan_create": [P, C.POINTER(MeshHost), I, PP],

@@ -13,6 +13,8 @@
 #include "tbs_common.hpp"
 #include "lane_ops.hpp"
 
+#include <algorithm>
+
 namespace {
 
 using tbs::rdlane;
@@ -271,6 +273,46 @@ __global__ void lineRes_kernel(int nlines, const int *__restrict__ epoint,
     }
 }
 
+// -------------------------------------------------------------- dgemvLU --
+// src/okl/linesmoothLU_TriBlock_v3.okl:48-69 (an inline helper of the
+// reference, applied here to every element's block): b = L U x with the
+// packed factors of LU (unit-lower L strictly below the diagonal, U on and
+// above it).  One thread per (element, row); a workgroup holds whole
+// elements, so the U x products of an element's rows meet in LDS before the
+// L pass.  Both sums run in the reference's order: U from the diagonal
+// upward in j, then b_i = y_i plus L(i,j) y_j for ascending j < i.
+template <int NV>
+__global__ __launch_bounds__(256) void dgemvLU_kernel(long nelem, const double *__restrict__ LU,
+                                                      const double *__restrict__ x, double *__restrict__ b) {
+    constexpr int BS = NV * NV;
+    constexpr int EPB = 256 / NV;                 // elements per workgroup
+    __shared__ double sy[EPB * NV];
+    const int kl = threadIdx.x / NV, i = threadIdx.x - kl * NV;
+    const bool lane_ok = kl < EPB;
+    for (long e0 = (long)blockIdx.x * EPB; e0 < nelem; e0 += (long)gridDim.x * EPB) {
+        const long e = e0 + kl;
+        const bool ok = lane_ok && e < nelem;
+        const double *A = LU + (size_t)BS * (ok ? e : 0);
+        const double *xe = x + (size_t)NV * (ok ? e : 0);
+        double y = 0.0;
+        if (ok) {
+#pragma unroll
+            for (int j = 0; j < NV; ++j)
+                if (j >= i) y += A[i + NV * j] * xe[j];
+            sy[kl * NV + i] = y;
+        }
+        __syncthreads();
+        if (ok) {
+            double bi = y;
+#pragma unroll
+            for (int j = 0; j < NV; ++j)
+                if (j < i) bi += A[i + NV * j] * sy[kl * NV + j];
+            b[(size_t)NV * e + i] = bi;
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------ assembleTriBlocks --
 // src/Jacobian.cxx:76-117 as a device gather; grid.y = line.
 template <int NV>
@@ -487,6 +529,26 @@ int tbs_triblock_lineRes(tbs_ctx *ctx, int nelem, int nintfaces, int eftot, int 
         default: TBS_REQUIRE(false, TBS_ERR_UNSUPPORTED, "nvar %d", nv);
     }
     TBS_LAUNCHED("triblock_lineRes");
+    return TBS_OK;
+}
+
+int tbs_dgemvLU(tbs_ctx *ctx, int nelem, const double *LU, const double *x, double *b) {
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    TBS_REQUIRE(nelem >= 0, TBS_ERR_ARG, "nelem < 0");
+    if (nelem == 0) return TBS_OK;
+    TBS_REQUIRE(LU && x && b, TBS_ERR_ARG, "null array");
+    const int nv = ctx->nvar;
+    const long epb = 256 / nv;
+    const unsigned grid = (unsigned)std::min<long>((nelem + epb - 1) / epb, 65536L);
+    switch (nv) {
+#define X(v) case v: hipLaunchKernelGGL(dgemvLU_kernel<v>, dim3(grid), dim3(256), 0, ctx->stream, \
+                   (long)nelem, LU, x, b); break;
+        TBS_REF_NVARS(X)
+#undef X
+        default: TBS_REQUIRE(false, TBS_ERR_UNSUPPORTED, "nvar %d", nv);
+    }
+    TBS_LAUNCHED("dgemvLU");
     return TBS_OK;
 }
 

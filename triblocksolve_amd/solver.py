@@ -331,6 +331,12 @@ class Context:
     def destroy_event(self, ev):
         self._lib.tbs_event_destroy(self.handle, ev)
 
+    def stream_ceiling(self, nbytes=2 << 30, reps=10):
+        """Achievable HBM bandwidth (GB/s) of a read-only stream and of a copy."""
+        out = np.zeros(2)
+        A.check(self._lib.tbs_stream_ceiling(self.handle, int(nbytes), reps, _ptr(out)))
+        return {"read_gbps": float(out[0]), "copy_gbps": float(out[1])}
+
     # --- drop-in kernels (device tensors) ---
     def copyAtoBjac(self, nelem, a, b):
         A.check(self._lib.tbs_copyAtoBjac(self.handle, nelem, _ptr(a), _ptr(b)))
@@ -361,6 +367,9 @@ class Context:
                                                m["lines"].size, _ptr(m["epoint"]), _ptr(m["ef"]), _ptr(m["fc"]),
                                                _ptr(m["linesize"]), _ptr(m["linepoint"]), _ptr(m["lines"]),
                                                _ptr(Dia), _ptr(Of1), _ptr(Of2), _ptr(U), _ptr(R)))
+
+    def dgemvLU(self, nelem, LU, x, b):
+        A.check(self._lib.tbs_dgemvLU(self.handle, nelem, _ptr(LU), _ptr(x), _ptr(b)))
 
     def norm2(self, n, v):
         out = C.c_double()
