@@ -11,10 +11,12 @@ the library's stream gives the per-kernel durations of the roofline (each
 kernel launched with hipExtLaunchKernelGGL, whose start/stop events are taken at
 the kernel's own start and end).  Prints ONE JSON line.
 
-Also measured (outside the timed region, reported as extra fields): the same
-sweep with FP32 factors, hipGraph launch throughput, time to relative residual
-1e-10 including factorisation for FP64 and FP32 factors, and the oracle's CPU
-throughput on a bounded slice of the same workload.
+Also measured (outside the timed region, reported as extra fields): the
+device's achievable streaming bandwidth, the errors against the FP64 oracle,
+the same sweep with FP32 factors, time to relative residual 1e-10 including
+factorisation for FP64 and FP32 factors, the reference loop on the drop-in
+kernels, GMRES, the batched and served workloads, block sizes, and the
+oracle's CPU throughput on the same workload.
 """
 import argparse
 import json
@@ -39,10 +41,18 @@ def parse():
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="only the FP64 timed region")
+    ap.add_argument("--no-oracle", action="store_true", help="skip the errors against the FP64 oracle")
+    ap.add_argument("--cpu-baseline-only", action="store_true",
+                    help="(internal) run only the CPU baseline leg and print its JSON; no GPU use")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="before timing, copy jacD once (copy_kernel, 81*8*nelem bytes read and written): "
                          "the known-size launch that rocprofv3 FETCH_SIZE/WRITE_SIZE counts are scaled by")
     return ap.parse_args()
+
+
+def log(msg):
+    """Progress of the run on stderr (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def measured_traffic(workload, kernel):
@@ -76,45 +86,92 @@ def cpu_info():
     return model, flags
 
 
-def cpu_baseline(tbs, args, nelem_full, sweeps=10):
-    """The oracle (the reference's Serial/OpenMP path restated in C++, OpenMP
-    over lines) on a bounded slice of the workload: the first tenth of the
-    lines of the same grid and generator, `sweeps` timed sweeps after one
-    warm-up, with all host cores (up to 16) and with one thread."""
+def host_cores():
+    """Host cores this process may use, at most 16 (the GPU box's CPU share)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline_child(args):
+    """Runs in a child process started by cpu_baseline() with OMP_PROC_BIND=close
+    set before the OpenMP runtime loads: the oracle (the reference's
+    Serial/OpenMP path restated in C++, OpenMP over lines) on the whole
+    workload, 11 sweeps each with all cores and with one thread; the first
+    sweep is the warm-up, the median of the other 10 is the figure.  No norms
+    are taken inside the timed sweeps (the reference computes none)."""
+    import numpy as np
+    import triblocksolve_amd as tbs
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_py import Oracle
-    cores = min(os.cpu_count() or 1, 16)
-    nl = max(1, args.nl // 10)
-    m = tbs.structured_mesh(nl, args.nk)
+    cores = host_cores()
+    m = tbs.structured_mesh(args.nl, args.nk)
     jac = tbs.generate_host(args.seed, 9, m.nelem, m.nintface, args.shift)
     o = Oracle()
-    scale = m.nelem / nelem_full
     res = {}
     for threads in (cores, 1):
-        o.lij_run(m, jac, 1, nthreads=threads)                 # warm-up
-        out = o.lij_run(m, jac, sweeps, nthreads=threads)
-        res[threads] = (out["t_sweeps"] / sweeps, out["t_factor"])
+        out = o.lij_run(m, jac, 11, nthreads=threads, norms=False)
+        t = np.asarray(out["t_each"][1:11])
+        res[threads] = (float(np.median(t)), float(out["t_factor"]), [float(x) for x in t])
     model, flags = cpu_info()
-    t_all, f_all = res[cores]
-    t_one, f_one = res[1]
-    return {"value": scale / t_all, "unit": "sweeps/s", "cores": cores, "kind": "port",
-            "sample": f"{nl} x {args.nk} slice ({m.nelem} elements, {scale:.3f} of the workload), "
-                      f"{sweeps} sweeps after 1 warm-up, OpenMP over lines with {cores} threads; "
-                      f"sweeps/s scaled by the element ratio",
-            "serial_sweeps_per_s": scale / t_one, "t_sweep_sample_s": t_all, "t_sweep_sample_serial_s": t_one,
-            "t_factor_sample_s": f_all, "t_factor_sample_serial_s": f_one, "cpu_model": model,
-            "nproc": os.cpu_count(), "compiler_flags": flags}
+    t_all, f_all, each = res[cores]
+    t_one, f_one, _ = res[1]
+    print(json.dumps({
+        "value": 1.0 / t_all, "unit": "sweeps/s", "cores": cores, "kind": "port",
+        "sample": f"the whole {args.nl} x {args.nk} workload ({m.nelem} elements): median of 10 sweeps after "
+                  f"1 warm-up sweep, OpenMP over lines with {cores} threads, OMP_PROC_BIND=close",
+        "serial_sweeps_per_s": 1.0 / t_one, "t_sweep_median_s": t_all, "t_sweep_median_serial_s": t_one,
+        "t_sweeps_s": each, "t_factor_s": f_all, "t_factor_serial_s": f_one, "cpu_model": model,
+        "nproc": os.cpu_count(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+        "compiler_flags": flags}))
 
 
-def bench_batched(tbs, ctx, args, seed=3):
+def cpu_baseline(args):
+    """The CPU leg in a child process (no GPU use there), so that OMP_PROC_BIND
+    and OMP_PLACES reach the OpenMP runtime before it starts."""
+    import subprocess
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores", OMP_NUM_THREADS=str(host_cores()))
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-only", "--nl", str(args.nl),
+           "--nk", str(args.nk), "--shift", str(args.shift), "--seed", str(args.seed)]
+    out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=600)
+    if out.returncode != 0:
+        raise RuntimeError(f"CPU baseline failed ({out.returncode}): {out.stderr[-2000:]}")
+    return json.loads(out.stdout.strip().splitlines()[-1])
+
+
+def relerr(a, b):
+    import numpy as np
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def hist_dev(h, ref, floor=1e-4):
+    """Largest |h_k/h_0 - h'_k/h'_0| / max(h'_k/h'_0, floor): the parity measure of
+    the residual history against the oracle's (1e-12 is the bound the tests use)."""
+    import numpy as np
+    n = min(len(h), len(ref))
+    a = np.asarray(h[:n]) / h[0]
+    b = np.asarray(ref[:n]) / ref[0]
+    return float(np.max(np.abs(a - b) / np.maximum(b, floor)))
+
+
+def bench_batched(tbs, ctx, args, seed=3, oracle=None):
     """64 independent systems of uneven lines (96..1280 elements, 8..40 lines
     per system, 384,000 elements in all) advanced by one launch per kernel per
-    sweep, FP64 and FP32-factor; Thomas variant chosen by the autotuner."""
+    sweep, FP64 and FP32-factor; Thomas variant chosen by the autotuner.  With
+    the oracle: every system's U against the FP64 oracle (FP64 batch: the same
+    number of sweeps; FP32 factors: the oracle run to the round-off floor)."""
     import numpy as np
     shapes = tbs.batch_shapes(seed, nsys=64, total=384000, lmin=96, lmax=1280, nlmin=8, nlmax=40)
     meshes = [tbs.ragged_mesh(s) for s in shapes]
+    lens = np.concatenate(shapes)
     res = {"workload": f"batch64_384000_seed{seed}", "nsys": len(meshes),
-           "nelem": int(sum(m.nelem for m in meshes)), "nline": int(sum(m.nline for m in meshes))}
+           "nelem": int(sum(m.nelem for m in meshes)), "nline": int(sum(m.nline for m in meshes)),
+           "line_len_realised": [int(lens.min()), int(lens.max())],
+           "shapes_note": "64 systems, lines per system U{8..40}, line lengths U[96,1280] drawn then scaled to "
+                          "384,000 elements in all and clamped to [96,1280] (the independent draws total about "
+                          "1.06M elements); realised lengths are line_len_realised"}
     bufs = []
     for i, m in enumerate(meshes):
         z = lambda k: ctx.array(size=k, zero=False)
@@ -122,6 +179,12 @@ def bench_batched(tbs, ctx, args, seed=3):
         ctx.generate_device(1000 + i, m.nelem, m.nintface, args.shift, D, O1, O2, rhs)
         bufs.append((D, O1, O2, rhs))
     ctx.sync()
+    refs = {}
+    if oracle is not None:
+        cores = host_cores()
+        for i, m in enumerate(meshes):
+            j = tbs.generate_host(1000 + i, 9, m.nelem, m.nintface, args.shift)
+            refs[i] = (m, j, oracle.lij_run(m, j, 12, nthreads=cores))
     for prec, name in ((tbs.TBS_FP64, "fp64"), (tbs.TBS_FP32, "fp32_factor")):
         plans = [tbs.Plan(ctx, m, prec) for m in meshes]
         ctx.sync()
@@ -144,6 +207,9 @@ def bench_batched(tbs, ctx, args, seed=3):
         t = ctx.elapsed_ms(e0, e1) * 1e-3
         ctx.destroy_event(e0)
         ctx.destroy_event(e1)
+        st = b.states()
+        if not (st[:, 1] == 1).all() or not (st[:, 0] == args.warmup + steps + 1).all():
+            raise RuntimeError("a batched system stopped inside the timed sweeps")
         # time to 1e-10 from U = 0 (factorisation timed above, solve here)
         Us = [ctx.array(size=9 * m.nelem) for m in meshes]
         ctx.sync()
@@ -156,6 +222,18 @@ def bench_batched(tbs, ctx, args, seed=3):
                      "autotune_variant": int(chosen), "factor_s": t_factor, "solve_to_1e-10_s": t_solve,
                      "time_to_1e-10_s": t_factor + t_solve, "max_sweeps_to_1e-10": int(max(ns)),
                      "worst_final_rel_residual": float(max(h[-1] / h[0] for h in hists))}
+        if refs:
+            if prec == tbs.TBS_FP64:
+                # the same number of sweeps as the batch took: parity (1e-12)
+                errs, hdev = [], []
+                for i, (m, j, _) in refs.items():
+                    r = oracle.lij_run(m, j, int(ns[i]), nthreads=host_cores())
+                    errs.append(relerr(Us[i].numpy(), r["U"]))
+                    hdev.append(hist_dev(hists[i], r["hist"]))
+                res[name]["oracle_U_relerr_same_sweeps_worst"] = max(errs)
+                res[name]["oracle_history_dev_worst"] = max(hdev)
+            errs = [relerr(Us[i].numpy(), r["U"]) for i, (_, _, r) in refs.items()]
+            res[name]["oracle_U_relerr_vs_converged_worst"] = max(errs)
         b.close()
         for p in plans:
             p.close()
@@ -210,12 +288,14 @@ def bench_gmres(tbs, ctx, m, args, restart=30):
     return out
 
 
-def bench_serving(tbs, ctx, args, nreq=160, seed=7):
+def bench_serving(tbs, ctx, args, nreq=160, seed=7, oracle=None):
     """Continuous-batching server, FP32 factors / FP64 residual, tolerance
     1e-10.  Request sizes n log-uniform in [3840, 384000], line length L
     uniform in [64, 960], grid max(1, round(n/L)) x L.  Saturated throughput
     from a burst of nreq requests; then Poisson arrivals at 0.5, 0.8 and 0.95
-    of it, latency = scheduled arrival -> result on the host."""
+    of it, latency = scheduled arrival -> result on the host.  With the
+    oracle: every 20th request of the saturated run, its U against the FP64
+    oracle's solution at the round-off floor."""
     import numpy as np
     rng = np.random.default_rng(seed)
 
@@ -225,11 +305,11 @@ def bench_serving(tbs, ctx, args, nreq=160, seed=7):
         NL = np.maximum(1, np.rint(n / L)).astype(int)
         return [(int(a), int(b)) for a, b in zip(NL, L)]
 
-    def run(sizes, arrivals):
+    def run(sizes, arrivals, keep=()):
         srv = tbs.Server(ctx, precision=tbs.TBS_FP32, rtol=1e-10, max_sweeps=100, sweeps_per_step=2)
         t0 = srv.info()["now"]
-        pending = list(range(len(sizes)))
         rid_of, lat, nsw, status = {}, [], [], []
+        kept = {}
         k = 0
         while len(lat) < len(sizes):
             now = srv.info()["now"] - t0
@@ -239,7 +319,9 @@ def bench_serving(tbs, ctx, args, nreq=160, seed=7):
                 k += 1
             srv.step()
             for rid in srv.poll():
-                r = srv.result(rid, want_U=False)
+                r = srv.result(rid, want_U=rid_of[rid] in keep)
+                if rid_of[rid] in keep:
+                    kept[rid_of[rid]] = r["U"]
                 lat.append(r["t_done"] - t0 - arrivals[rid_of[rid]])
                 nsw.append(r["nsweeps"])
                 status.append(r["status"])
@@ -248,16 +330,28 @@ def bench_serving(tbs, ctx, args, nreq=160, seed=7):
         stats = srv.stats()
         srv.close()
         run.stats = stats
+        run.kept = kept
         return np.array(lat), wall, nsw, status
 
     sizes = draw(nreq)
     elems = sum(a * b for a, b in sizes)
-    lat, wall, nsw, st = run(sizes, np.zeros(nreq))
+    keep = set(range(0, nreq, 20)) if oracle is not None else set()
+    lat, wall, nsw, st = run(sizes, np.zeros(nreq), keep)
     sat = nreq / wall
     out = {"workload": f"serve_{nreq}req_seed{seed}", "mean_elements": elems / nreq,
            "saturated_req_per_s": sat, "saturated_elements_per_s": elems / wall,
            "all_converged": bool(all(s == 1 for s in st)), "max_sweeps": int(max(nsw)),
            "saturated_host_phase_s": run.stats, "loads": {}}
+    if keep:
+        errs = {}
+        for k, U in sorted(run.kept.items()):
+            NL, NK = sizes[k]
+            m = tbs.structured_mesh(NL, NK)
+            j = tbs.generate_host(5000 + k, 9, m.nelem, m.nintface, args.shift)
+            r = oracle.lij_run(m, j, 12, nthreads=host_cores())
+            errs[f"{NL}x{NK}"] = relerr(U, r["U"])
+        out["oracle_U_relerr_vs_converged"] = errs
+        out["oracle_U_relerr_vs_converged_worst"] = max(errs.values())
     for frac in (0.5, 0.8, 0.95):
         sizes = draw(nreq)
         arr = np.cumsum(rng.exponential(1.0 / (frac * sat), nreq))
@@ -267,6 +361,64 @@ def bench_serving(tbs, ctx, args, nreq=160, seed=7):
                                      "p99_ms": float(np.percentile(lat, 99) * 1e3),
                                      "all_converged": bool(all(s == 1 for s in st)),
                                      "host_phase_s": run.stats}
+    return out
+
+
+def bench_dropin(tbs, ctx, m, D, O1, O2, rhs, sweeps=5):
+    """The reference loop (src/main.cxx:180-229) on the drop-in kernels at the
+    headline shape: copyAtoBjac + lineLU once, then per sweep solveDU, addAtoB,
+    copyAtoB, lineRes, each kernel between its own pair of events.  GB/s per
+    kernel on its algorithmic bytes; the sweep on the same byte model as the
+    fused path (bytes_per_sweep_model)."""
+    import numpy as np
+    n, nif, nl = m.nelem, m.nintface, m.nline
+    z = lambda k: ctx.array(size=k, zero=True)
+    d = m.to_device(ctx)
+    DLU, DinvC, Am = z(81 * n), z(81 * n), z(81 * n)
+    U, dU, R = z(9 * n), z(9 * n), z(9 * n)
+    ev = [ctx.event() for _ in range(10)]
+    ctx.sync()
+    ctx.record(ev[0])
+    ctx.assembleTriBlocks(d, O1, O2, Am)
+    ctx.record(ev[1])
+    ctx.copyAtoBjac(n, D, DLU)
+    ctx.lineLU(n, nif, d, DLU, O1, O2, DinvC)
+    ctx.record(ev[2])
+    ctx.copyAtoB(n, rhs, R)
+    ctx.sync()
+    t_asm, t_lu = ctx.elapsed_ms(ev[0], ev[1]), ctx.elapsed_ms(ev[1], ev[2])
+    names = ("solveDU", "addAtoB", "copyAtoB", "lineRes")
+    t = {k: [] for k in names}
+    for sw in range(sweeps + 1):                    # the first sweep is a warm-up
+        ctx.record(ev[0])
+        ctx.solveDU(n, nif, d, DLU, DinvC, Am, dU, R)
+        ctx.record(ev[1])
+        ctx.addAtoB(n, dU, U)
+        ctx.record(ev[2])
+        ctx.copyAtoB(n, rhs, R)
+        ctx.record(ev[3])
+        ctx.lineRes(n, nif, d, D, O1, O2, U, R)
+        ctx.record(ev[4])
+        ctx.sync()
+        if sw:
+            for i, k in enumerate(names):
+                t[k].append(ctx.elapsed_ms(ev[i], ev[i + 1]) * 1e-3)
+    vec = 72.0 * n
+    alg = {"solveDU": 648.0 * (3 * n - 2 * nl) + 2 * vec, "addAtoB": 3 * vec, "copyAtoB": 2 * vec,
+           "lineRes": 648.0 * (n + 2 * nif) + 3 * vec}
+    kern = {k: {"avg_ms": float(np.mean(t[k])) * 1e3, "alg_bytes": alg[k],
+                "GBps": alg[k] / float(np.mean(t[k])) / 1e9} for k in names}
+    t_sweep = sum(float(np.mean(t[k])) for k in names)
+    model = tbs.model_bytes(m)
+    out = {"workload": f"lij_{m.nline}x{n // m.nline}_dropin_reference_loop", "sweeps": sweeps,
+           "sweeps_per_s": 1.0 / t_sweep, "gbps_model": model / t_sweep / 1e9,
+           "frac_hbm_peak": model / t_sweep / 1e9 / HBM_PEAK_GBS,
+           "gbps_reference_model": tbs.reference_model_bytes(m) / t_sweep / 1e9,
+           "kernels": kern, "assembleTriBlocks_ms": t_asm, "copyAtoBjac_lineLU_ms": t_lu}
+    for e in ev:
+        ctx.destroy_event(e)
+    for x in (DLU, DinvC, Am, U, dU, R, *d.values()):
+        x.close()
     return out
 
 
@@ -335,6 +487,9 @@ def bench_block_sizes(tbs, args, NL=600, NK=640, sweeps=4, nvars=(5, 9, 10, 16, 
 
 def main():
     args = parse()
+    if args.cpu_baseline_only:
+        cpu_baseline_child(args)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is designed "
                                    "(DESIGN.md) but not built", "n_gpus": args.gpus}))
@@ -386,17 +541,34 @@ def main():
                 for e in pair:
                     ctx.destroy_event(e)
         # the production path (tbs_plan_solve): the K sweeps captured in one
-        # hipGraph; the first launch captures and is untimed
+        # hipGraph; the first launch captures and is untimed.  The iterate is
+        # restarted before the timed replay, and every sweep of it must have
+        # run (a system that stops early would make its kernels return at once)
         ctx.sync()
         plan.launch_sweeps(steps, use_graph=True)
+        plan.begin(rhs, None)
         ctx.sync()
         t1 = time.perf_counter()
         plan.launch_sweeps(steps, use_graph=True)
         ctx.sync()
         t_graph = time.perf_counter() - t1
+        h = plan.history()
+        if len(h) != steps + 1 or not np.all(np.isfinite(h)):
+            raise RuntimeError(f"timed replay recorded {len(h)} of {steps + 1} residual norms")
         plan.close()
         return dict(wall=wall, thomas=th, residual=rs, norm=nm, info=info, t_factor=t_factor, t_graph=t_graph,
                     tuned=tuned)
+
+    def oracle_parity(ref, iters=8):
+        """FP64 plan, `iters` sweeps from zero, against the oracle's same sweeps."""
+        plan = tbs.Plan(ctx, m, tbs.TBS_FP64)
+        plan.factor(D, O1, O2)
+        U = ctx.array(size=9 * n)
+        hist, ns = plan.solve(rhs, U, iters, 0.0)
+        plan.close()
+        r = {"sweeps": int(ns), "U_relerr": relerr(U.numpy(), ref["U"]), "history_dev": hist_dev(hist, ref["hist"])}
+        U.close()
+        return r
 
     def time_to_tol(prec, rtol=1e-10):
         plan = tbs.Plan(ctx, m, prec)
@@ -413,6 +585,7 @@ def main():
         return dict(factor_s=t1 - t0, solve_s=t2 - t1, total_s=t2 - t0, sweeps=int(ns),
                     final_rel_residual=float(hist[-1] / hist[0]), U=U)
 
+    log("fp64 timed sweeps")
     r64 = timed_sweeps(tbs.TBS_FP64, args.steps, args.warmup)
     info = r64["info"]
     # value: the K sweeps as the library runs them (one graph replay between
@@ -465,7 +638,28 @@ def main():
         "factor_s": r64["t_factor"],
         "autotune": {"chosen_variant": r64["tuned"][0], "thomas_us_by_variant": r64["tuned"][1]},
     }
+    log("stream ceiling")
+    ceil = ctx.stream_ceiling()
+    out["achievable_gbps"] = ceil
+    roof["frac_of_copy_ceiling"] = kern[dom]["GBps"] / ceil["copy_gbps"]
+    out["frac_of_copy_ceiling"] = gbs / ceil["copy_gbps"]
+    oracle, ref = None, None
+    if not args.no_oracle:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_py import Oracle
+        oracle = Oracle()
+        log("oracle on the headline system")
+        jh = tbs.generate_host(args.seed, 9, n, nif, args.shift)
+        t0 = time.perf_counter()
+        ref = oracle.lij_run(m, jh, 8, nthreads=host_cores())
+        del jh
+        par = oracle_parity(ref, 8)
+        out["oracle"] = {"what": "FP64 oracle (the reference's CPU path restated, tests/oracle_py.py) on the same "
+                                 "seeded system, 8 sweeps from zero, outside every timed region",
+                         "oracle_s": time.perf_counter() - t0, "fp64_same_sweeps": par,
+                         "tolerance": "history |h_k/h_0 - h'_k/h'_0| <= 1e-12 max(h'_k/h'_0, 1e-4); U <= 1e-12"}
     if not args.no_extras:
+        log("fp32-factor timed sweeps")
         r32 = timed_sweeps(tbs.TBS_FP32, args.steps, args.warmup)
         i32 = r32["info"]
         out["fp32_factor"] = {
@@ -475,19 +669,30 @@ def main():
             "residual_ms": float(r32["residual"].mean()) * 1e3,
             "eager_sweeps_per_s": args.steps / r32["wall"],
         }
+        log("time to 1e-10")
         a = time_to_tol(tbs.TBS_FP64)
         b = time_to_tol(tbs.TBS_FP32)
         ua, ub = a["U"].numpy(), b["U"].numpy()
         err = float(np.linalg.norm(ub - ua) / np.linalg.norm(ua))
-        for d in (a, b):
-            d.pop("U")
+        for d_, u_ in ((a, ua), (b, ub)):
+            d_.pop("U").close()
+            if ref is not None:
+                # against the oracle's solution at the round-off floor (8 sweeps)
+                d_["oracle_U_relerr_vs_converged"] = relerr(u_, ref["U"])
         out["time_to_1e-10"] = {"fp64": a, "fp32_factor": b, "fp32_vs_fp64_U_relerr": err}
+        log("drop-in reference loop")
+        out["dropin"] = bench_dropin(tbs, ctx, m, D, O1, O2, rhs)
+        log("gmres")
         out["gmres"] = bench_gmres(tbs, ctx, m, args)
-        out["batched"] = bench_batched(tbs, ctx, args)
-        out["serving"] = bench_serving(tbs, ctx, args)
+        log("batched")
+        out["batched"] = bench_batched(tbs, ctx, args, oracle=oracle)
+        log("serving")
+        out["serving"] = bench_serving(tbs, ctx, args, oracle=oracle)
+        log("block sizes")
         out["block_sizes"] = bench_block_sizes(tbs, args)
     if not args.no_cpu:
-        cb = cpu_baseline(tbs, args, n)
+        log("cpu baseline (child process)")
+        cb = cpu_baseline(args)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
         out["cpu_baseline_detail"] = {k: v for k, v in cb.items() if k not in out["cpu_baseline"]}
     ctx.close()

@@ -379,6 +379,13 @@ int tbs_write_mesh(const char *path, const int sizes[6], const int *epoint,
                    const int *ef, const int *fc, const int *linesize,
                    const int *linepoint, const int *lines,
                    const int *lineface);
+/* Host-side check of every index the drop-in kernels follow (epoint monotone
+ * from 0 to eftot, ef below nintface, fc and lines inside [0, nelem),
+ * linepoint/linesize contiguous up to nlineelem <= nelem, lineface of every
+ * non-first line element an interior face joining it to its predecessor):
+ * TBS_ERR_IO naming the first inconsistency.  The reference trusts its files
+ * (src/Mesh.cxx:11-73); Mesh::fromFile calls this after reading. */
+int tbs_mesh_validate(const tbs_mesh_host *mesh);
 int tbs_read_jacobian_header(const char *path, int sizes[3]);
 int tbs_read_jacobian(const char *path, double *jacD, double *jacO1,
                       double *jacO2, double *rhs, double *U0);

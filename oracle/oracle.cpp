@@ -335,7 +335,8 @@ double orc_norm2(long n, const double *v) {
  * ||R||_2 before the first and after every sweep.  jacDLU, DinvC, A, dU and R
  * are caller-provided work/output arrays in the reference layout.
  * Returns wall seconds of the factorisation in t_factor and of the sweep
- * loop in t_sweeps (steady_clock). */
+ * loop in t_sweeps (steady_clock); t_each (may be NULL) gets each sweep's own
+ * seconds, without the norm (hist may be NULL: then no norm is taken). */
 void orc_lij_run(int nv, int nelem, int nintface, int eftot, int nline,
                  int nlineelem, const int *epoint, const int *ef,
                  const int *fc, const int *linesize, const int *linepoint,
@@ -343,7 +344,7 @@ void orc_lij_run(int nv, int nelem, int nintface, int eftot, int nline,
                  const double *jacO1, const double *jacO2, const double *rhs,
                  double *jacDLU, double *DinvC, double *A, double *U,
                  double *dU, double *R, int iters, double *hist,
-                 int nthreads, double *t_factor, double *t_sweeps) {
+                 int nthreads, double *t_factor, double *t_sweeps, double *t_each) {
     using clk = std::chrono::steady_clock;
     orc_assembleTriBlocks(nv, nline, linesize, linepoint, lines, lineface, fc, jacO1, jacO2, A);
     auto t0 = clk::now();
@@ -354,12 +355,14 @@ void orc_lij_run(int nv, int nelem, int nintface, int eftot, int nline,
     orc_copyAtoB(nv, nelem, rhs, R, nthreads);
     if (hist) hist[0] = orc_norm2((long)nv * nelem, R);
     for (int p = 0; p < iters; ++p) {
+        auto ts = clk::now();
         orc_solveDU(nv, nelem, nintface, eftot, nline, nlineelem, linesize, linepoint, lines,
                     jacDLU, DinvC, A, dU, R, nthreads);
         orc_addAtoB(nv, nelem, dU, U, nthreads);
         orc_copyAtoB(nv, nelem, rhs, R, nthreads);
         orc_lineRes(nv, nelem, nintface, eftot, nline, nlineelem, epoint, ef, fc, linesize,
                     linepoint, lines, jacD, jacO1, jacO2, U, R, nthreads);
+        if (t_each) t_each[p] = std::chrono::duration<double>(clk::now() - ts).count();
         if (hist) hist[p + 1] = orc_norm2((long)nv * nelem, R);
     }
     auto t2 = clk::now();

@@ -76,17 +76,20 @@ class Oracle:
     def norm2(self, v):
         return self.lib.orc_norm2(v.size, _p(v))
 
-    def lij_run(self, m, jac, iters, nthreads=1):
-        """Reference driver loop; returns dict(U, R, hist, dU, Dia, DinvC, A, t_factor, t_sweeps)."""
+    def lij_run(self, m, jac, iters, nthreads=1, norms=True):
+        """Reference driver loop; returns dict(U, R, hist, dU, Dia, DinvC, A, t_factor, t_sweeps,
+        t_each).  norms=False skips the history (t_each then times the sweeps alone)."""
         nv, n = m.nvar, m.nelem
         bs = nv * nv
         out = dict(jacDLU=np.zeros(bs * n), DinvC=np.zeros(bs * n), A=np.zeros(bs * n),
-                   U=np.zeros(nv * n), dU=np.zeros(nv * n), R=np.zeros(nv * n), hist=np.zeros(iters + 1))
+                   U=np.zeros(nv * n), dU=np.zeros(nv * n), R=np.zeros(nv * n), hist=np.zeros(iters + 1),
+                   t_each=np.zeros(max(iters, 1)))
         tf, ts = C.c_double(), C.c_double()
         self.lib.orc_lij_run(nv, n, m.nintface, m.eftot, m.nline, m.nlineelem, _p(m.epoint), _p(m.ef),
                              _p(m.fc), _p(m.linesize), _p(m.linepoint), _p(m.lines), _p(m.lineface),
                              _p(jac["jacD"]), _p(jac["jacO1"]), _p(jac["jacO2"]), _p(jac["rhs"]),
                              _p(out["jacDLU"]), _p(out["DinvC"]), _p(out["A"]), _p(out["U"]), _p(out["dU"]),
-                             _p(out["R"]), iters, _p(out["hist"]), nthreads, C.byref(tf), C.byref(ts))
+                             _p(out["R"]), iters, _p(out["hist"]) if norms else None, nthreads, C.byref(tf),
+                             C.byref(ts), _p(out["t_each"]))
         out["t_factor"], out["t_sweeps"] = tf.value, ts.value
         return out

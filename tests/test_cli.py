@@ -111,3 +111,58 @@ def test_resize_block_size(tbs, oracle, tmp_path, new):
     assert hist.size == 4 and np.allclose(hist, ref["hist"], rtol=1e-10, atol=0, equal_nan=True)
     if new < 9:
         assert np.isfinite(hist).all() and hist[-1] < 1e-3 * hist[0]
+
+
+def _write_system(tmp_path, mesh, jac_mesh, seed=3):
+    import triblocksolve_amd as tbs
+    tbs.write_mesh(os.path.join(tmp_path, "gpuline.mesh.data.bin"), mesh)
+    j = tbs.generate_host(seed, jac_mesh.nvar, jac_mesh.nelem, jac_mesh.nintface, 20.0)
+    tbs.write_jacobian(os.path.join(tmp_path, "gpuline.jacobian.data.bin"), jac_mesh.nvar, jac_mesh.nelem,
+                       jac_mesh.nintface, j)
+
+
+def _corrupt(mesh, **changes):
+    arrs = {f: getattr(mesh, f).copy() for f in ("epoint", "ef", "fc", "linesize", "linepoint", "lines", "lineface")}
+    for name, (idx, val) in changes.items():
+        arrs[name][idx] = val
+    return HostMesh(mesh.nvar, **arrs)
+
+
+@pytest.mark.parametrize("case", ["truncated", "fc_out_of_range", "ef_past_faces", "lines_out_of_range",
+                                  "lineface_not_joining", "epoint_decreasing", "jacobian_of_another_mesh"])
+def test_bad_input_files_are_refused(tmp_path, case):
+    """Corrupted mesh files and a Jacobian of another system are refused with an
+    error exit before any device work (every index is checked on the host; the
+    kernels would otherwise follow them out of bounds)."""
+    import triblocksolve_amd as tbs
+    m = tbs.structured_mesh(4, 5)
+    if case == "jacobian_of_another_mesh":
+        _write_system(tmp_path, m, tbs.structured_mesh(4, 6))
+    elif case == "truncated":
+        _write_system(tmp_path, m, m)
+        p = os.path.join(tmp_path, "gpuline.mesh.data.bin")
+        data = open(p, "rb").read()
+        open(p, "wb").write(data[:-8])
+    else:
+        bad = {"fc_out_of_range": dict(fc=(3, 10 ** 6)),
+               "ef_past_faces": dict(ef=(5, m.nintface + 3)),
+               "lines_out_of_range": dict(lines=(7, m.nelem)),
+               "lineface_not_joining": dict(lineface=(2, int(m.ef[4 * 10 + 1]))),
+               "epoint_decreasing": dict(epoint=(3, 2))}[case]
+        _write_system(tmp_path, _corrupt(m, **bad), m)
+    p, _ = run(["1", "0", "9"], tmp_path, expect_ok=False)
+    assert p.returncode == 1, (p.returncode, p.stdout, p.stderr)
+    assert "ERROR" in p.stderr, p.stderr
+    assert "GPU Memory Allocated" not in p.stdout
+
+
+def test_mesh_validate_accepts_reference_fixture():
+    """The reference's own ex05 mesh passes the same checks."""
+    import triblocksolve_amd as tbs
+    from triblocksolve_amd import _capi
+    m = tbs.read_mesh(os.path.join(ROOT, "tests", "golden", "ex05_gpuline.mesh.data.bin"))
+    ms = m.as_struct()
+    assert tbs.load().tbs_mesh_validate(ms) == 0
+    m.fc[0] = -1
+    ms = m.as_struct()
+    assert tbs.load().tbs_mesh_validate(ms) == _capi.TBS_ERR_IO

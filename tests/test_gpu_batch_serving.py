@@ -60,7 +60,7 @@ def test_uneven_batch_autotuned_graph(tbs, ctx, oracle, prec, one_launch):
             assert np.linalg.norm(U - ref["U"]) <= 1e-12 * np.linalg.norm(ref["U"])
 
 
-def test_server_results_equal_solo_solves(tbs, ctx):
+def test_server_results_equal_solo_solves(tbs, ctx, oracle):
     print("case server mix", flush=True)
     specs = [(3, 64, 11), (40, 96, 12), (7, 960, 13), (120, 320, 14), (1, 64, 15), (60, 64, 16), (25, 500, 17)]
     srv = tbs.Server(ctx, precision=tbs.TBS_FP32, rtol=1e-10, max_sweeps=60, sweeps_per_step=2)
@@ -87,6 +87,9 @@ def test_server_results_equal_solo_solves(tbs, ctx):
         h, n, U = _solve_alone(tbs, ctx, mesh, jac, tbs.TBS_FP32, 60, 1e-10)
         assert r["nsweeps"] == n and np.array_equal(r["hist"], h), (NL, NK)
         assert np.array_equal(r["U"], U), (NL, NK)
+        # and against the FP64 oracle's solution at the round-off floor
+        ref = oracle.lij_run(mesh, jac, 12)
+        assert np.linalg.norm(r["U"] - ref["U"]) <= 1e-9 * np.linalg.norm(ref["U"]), (NL, NK)
     srv.close()
 
 
@@ -172,7 +175,7 @@ def test_batch_membership_changes_in_place(tbs, ctx, prec):
         s_[2].close()
 
 
-def test_server_fp64_mixed_sizes(tbs, ctx):
+def test_server_fp64_mixed_sizes(tbs, ctx, oracle):
     """FP64 server with tiny (one element, one line) and larger requests in one
     batch: every result is bitwise its solo FP64 solve."""
     print("case server fp64 mixed", flush=True)
@@ -194,6 +197,10 @@ def test_server_fp64_mixed_sizes(tbs, ctx):
         r = done[rid]
         assert r["status"] == 1 and r["nsweeps"] == n
         assert np.array_equal(r["hist"], h) and np.array_equal(r["U"], U), (NL, NK)
+        # the served result against the oracle directly (same sweeps)
+        ref = oracle.lij_run(mesh, jac, n)
+        assert cf.history_close(r["hist"], ref["hist"])[0], (NL, NK)
+        assert np.linalg.norm(r["U"] - ref["U"]) <= 1e-12 * np.linalg.norm(ref["U"]), (NL, NK)
     srv.close()
 
 

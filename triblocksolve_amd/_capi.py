@@ -136,6 +136,7 @@ SIGNATURES = {
     "tbs_read_mesh_header": [C.c_char_p, P],
     "tbs_read_mesh": [C.c_char_p, P, P, P, P, P, P, P],
     "tbs_write_mesh": [C.c_char_p, P, P, P, P, P, P, P, P],
+    "tbs_mesh_validate": [C.POINTER(MeshHost)],
     "tbs_read_jacobian_header": [C.c_char_p, P],
     "tbs_read_jacobian": [C.c_char_p, P, P, P, P, P],
     "tbs_write_jacobian": [C.c_char_p, I, I, I, P, P, P, P, P],

@@ -415,4 +415,47 @@ int tbs_write_jacobian(const char *path, int nvar, int nelem, int nintface, cons
     return TBS_OK;
 }
 
+// Every index the drop-in kernels follow, checked on the host before any of
+// them runs: the reference's kernels trust their mesh (src/okl/lineLU_v2.okl,
+// src/okl/linesmoothLU_TriBlock_v3.okl index with it directly), so a
+// corrupted file or a mesh of another system must be refused here.
+int tbs_mesh_validate(const tbs_mesh_host *m) {
+    TBS_REQUIRE(m != nullptr, TBS_ERR_ARG, "null mesh");
+    TBS_REQUIRE(m->nvar >= 1 && m->nelem >= 1 && m->nintface >= 0 && m->eftot >= 0 && m->nline >= 0 &&
+                    m->nlineelem >= 0 && m->nlineelem <= m->nelem,
+                TBS_ERR_IO, "mesh sizes out of range (nvar %d nelem %d nintface %d eftot %d nline %d nlineelem %d)",
+                m->nvar, m->nelem, m->nintface, m->eftot, m->nline, m->nlineelem);
+    TBS_REQUIRE(m->epoint && (m->eftot == 0 || m->ef) && (m->nintface == 0 || m->fc) && m->linepoint &&
+                    (m->nline == 0 || m->linesize) && (m->nlineelem == 0 || (m->lines && m->lineface)),
+                TBS_ERR_ARG, "null mesh array");
+    TBS_REQUIRE(m->epoint[0] == 0 && m->epoint[m->nelem] == m->eftot, TBS_ERR_IO,
+                "epoint must run from 0 to eftot = %d", m->eftot);
+    for (int e = 0; e < m->nelem; ++e)
+        TBS_REQUIRE(m->epoint[e + 1] >= m->epoint[e], TBS_ERR_IO, "epoint decreases at element %d", e);
+    for (int k = 0; k < m->eftot; ++k)
+        TBS_REQUIRE(m->ef[k] < m->nintface, TBS_ERR_IO, "ef[%d] = %d is not a face (nintface %d)", k, m->ef[k],
+                    m->nintface);
+    for (long f = 0; f < 2L * m->nintface; ++f)
+        TBS_REQUIRE(m->fc[f] >= 0 && m->fc[f] < m->nelem, TBS_ERR_IO, "fc[%ld] = %d is not an element", f, m->fc[f]);
+    TBS_REQUIRE(m->linepoint[0] == 0, TBS_ERR_IO, "linepoint[0] must be 0");
+    for (int l = 0; l < m->nline; ++l)
+        TBS_REQUIRE(m->linesize[l] >= 1 && (long)m->linepoint[l] + m->linesize[l] == m->linepoint[l + 1],
+                    TBS_ERR_IO, "line %d: linepoint/linesize not contiguous", l);
+    TBS_REQUIRE(m->linepoint[m->nline] == m->nlineelem, TBS_ERR_IO, "linepoint[nline] != nlineelem");
+    for (int i = 0; i < m->nlineelem; ++i)
+        TBS_REQUIRE(m->lines[i] >= 0 && m->lines[i] < m->nelem, TBS_ERR_IO, "lines[%d] = %d is not an element",
+                    i, m->lines[i]);
+    for (int l = 0; l < m->nline; ++l)
+        for (int k = 1; k < m->linesize[l]; ++k) {
+            const int mm = m->linepoint[l] + k;
+            const int f = m->lineface[mm];
+            TBS_REQUIRE(f >= 0 && f < m->nintface, TBS_ERR_IO, "lineface[%d] = %d is not an interior face", mm, f);
+            const int e = m->lines[mm], ep = m->lines[mm - 1];
+            const int a = m->fc[2 * f], b = m->fc[2 * f + 1];
+            TBS_REQUIRE((a == e && b == ep) || (a == ep && b == e), TBS_ERR_IO,
+                        "lineface[%d] does not join consecutive line elements", mm);
+        }
+    return TBS_OK;
+}
+
 }  // extern "C"

@@ -31,6 +31,12 @@ bool Mesh::fromFile(const std::string &path) {
         fprintf(stderr, "ERROR: %s\n", tbs_last_error());
         return false;
     }
+    // every index the kernels follow, before any of them can run
+    const tbs_mesh_host mh = host();
+    if (tbs_mesh_validate(&mh) != TBS_OK) {
+        fprintf(stderr, "ERROR: %s: %s\n", path.c_str(), tbs_last_error());
+        return false;
+    }
     finishRead();
     printf("  Mesh Statistics:\n    nvar: %d\n    nelem: %d\n    nintface: %d\n    eftot: %d\n"
            "    nline: %d\n    nlineelem: %d\n  Max Line Element count: %d\n",

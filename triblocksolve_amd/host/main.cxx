@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -90,7 +91,10 @@ int main(int argc, char **argv) {
     printf(" TRIDIAGONAL BLOCK SOLVER: Block size: %d, Compute mode: %d, Device ID: %d\n", block_size,
            compute_mode, device_id);
     try {
-        Platform gpu(compute_mode, device_id, block_size);
+        // the inputs are read and checked before the device is touched; the
+        // platform outlives the mesh and Jacobian (their device arrays are
+        // freed through it)
+        std::unique_ptr<Platform> gpu_holder;
         Mesh mesh;
         Jacobian Jac;
         if (NL > 0) {
@@ -103,7 +107,16 @@ int main(int argc, char **argv) {
         } else {
             if (!mesh.fromFile()) return 1;
             if (!Jac.fromFile(mesh.nlineelem)) return 1;
+            if (Jac.nvar != mesh.nvar || Jac.nelem != mesh.nelem || Jac.nintface != mesh.nintface) {
+                fprintf(stderr,
+                        "ERROR: the Jacobian (nvar %d, nelem %d, nintface %d) is not the mesh's system "
+                        "(nvar %d, nelem %d, nintface %d)\n",
+                        Jac.nvar, Jac.nelem, Jac.nintface, mesh.nvar, mesh.nelem, mesh.nintface);
+                return 1;
+            }
         }
+        gpu_holder.reset(new Platform(compute_mode, device_id, block_size));
+        Platform &gpu = *gpu_holder;
         Jac.assembleTriBlocks(mesh);
         Jac.resizeBlockSize(block_size);
         if (Jac.nvar != mesh.nvar) { mesh.nvar = Jac.nvar; Jac.assembleTriBlocks(mesh); }
