@@ -82,10 +82,12 @@ int tbs_event_destroy(tbs_ctx *ctx, void *ev);
 int tbs_event_record(tbs_ctx *ctx, void *ev);
 int tbs_event_elapsed_ms(void *ev_start, void *ev_stop, float *ms);
 /* Achievable HBM bandwidth of the context's device: a read-only stream and a
- * copy (16-byte non-temporal loads and stores, 8 workgroups per CU) over
- * `bytes` bytes of device memory the call allocates and frees, `reps` timed
- * launches each after one warm-up.  gbps[0] = read GB/s, gbps[1] = copy GB/s
- * (bytes read + bytes written).  Synchronous.  No counterpart in the
+ * copy (16-byte loads and stores) over `bytes` bytes of device memory the
+ * call allocates and frees, each timed over `reps` launches after one
+ * warm-up in 32 launch shapes (grid-stride or contiguous 256 KiB runs per
+ * workgroup, 4 or 8 loads in flight per thread, 4 to 32 workgroups per CU,
+ * streaming or default cache policy); the fastest shape counts.  gbps[0] = read GB/s, gbps[1] = copy GB/s (bytes read + bytes
+ * written).  Synchronous.  No counterpart in the
  * reference: it is the ceiling the sweep kernels' GB/s (the reference prints
  * GB/s at src/main.cxx:243-252) are compared with. */
 int tbs_stream_ceiling(tbs_ctx *ctx, size_t bytes, int reps, double *gbps);
