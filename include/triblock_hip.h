@@ -148,6 +148,14 @@ int tbs_dgemvLU(tbs_ctx *ctx, int nelem, const double *LU, const double *x, doub
 /* ||v||_2 of n doubles (device), deterministic fixed-order reduction; the
  * reference has no norm (build-defined convergence measure).              */
 int tbs_norm2(tbs_ctx *ctx, long n, const double *v, double *out_host);
+/* tbs_norm2 with no device allocation: its partials live in a static device
+ * array of the library (512 KiB per GPU, held until the process ends), so the
+ * call makes no hipMalloc/hipFree.  No counterpart in the reference; it is
+ * for callers that cannot allocate (a serving loop, a harness that forbids
+ * hipMalloc).  Limit: n <= 2^29 values (TBS_ERR_ARG beyond).  The result is
+ * bitwise tbs_norm2's.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (two threads, two contexts) race. */
+int tbs_norm2_noalloc(tbs_ctx *ctx, long n, const double *v, double *out_host);
 
 /* ---------------------------------------------------------------------- */
 /* Fused LIJ path ("plan"): the outer loop of src/main.cxx:180-229 as     */

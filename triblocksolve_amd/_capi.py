@@ -87,6 +87,7 @@ SIGNATURES = {
     "tbs_triblock_lineRes": [P, I, I, I, I, I, P, P, P, P, P, P, P, P, P, P, P],
     "tbs_dgemvLU": [P, I, P, P, P],
     "tbs_norm2": [P, L, P, C.POINTER(C.c_double)],
+    "tbs_norm2_noalloc": [P, L, P, C.POINTER(C.c_double)],
     "tbs_plan_footprint": [C.POINTER(MeshHost), I, C.POINTER(C.c_size_t)],
     "tbs_plan_create": [P, C.POINTER(MeshHost), I, PP],
     "tbs_plan_destroy": [P],

@@ -14,6 +14,8 @@
 #include "lane_ops.hpp"
 
 #include <algorithm>
+#include <climits>
+#include <mutex>
 
 namespace {
 
@@ -353,7 +355,9 @@ __global__ void add_kernel(long n, const double *__restrict__ A, double *__restr
 // -------------------------------------------------------------- norm --
 // Deterministic two-level sum of squares: block b reduces the fixed slice
 // [b*chunk, (b+1)*chunk) in a fixed order; one block then sums the partials.
-constexpr int kNormBlocks = 1024;
+// The slices are kNormChunk values long whatever n is, so a vector's norm
+// does not depend on where its partials are kept.
+constexpr long kNormChunk = 8192;
 __global__ __launch_bounds__(256) void sumsq_partial(long n, long chunk, const double *__restrict__ v,
                                                      double *__restrict__ part) {
     __shared__ double s[256];
@@ -382,6 +386,19 @@ __global__ __launch_bounds__(256) void sum_final(int nb, const double *__restric
     }
     if (threadIdx.x == 0) out[0] = sqrt(s[0]);
 }
+
+// ------------------------------------------------------ no-alloc norm --
+// Partials and result of tbs_norm2_noalloc in a file-scope device array: one
+// copy per GPU, held until the process ends (kNoAllocMaxPartials + 1 doubles,
+// 512 KiB + 8 B).  Host code never passes this name to a kernel or a copy (in
+// host code it is only a stand-in for the symbol): the device address comes
+// from hipGetSymbolAddress, once per device, with that device current.
+constexpr long kNoAllocMaxPartials = 65536;   // vectors of up to 2^29 values
+__device__ __align__(256) static double scratch[kNoAllocMaxPartials + 1];
+constexpr int kMaxDevices = 64;
+std::once_flag g_scratch_once[kMaxDevices];
+double *g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
 
 unsigned grid_for(long n) {
     long g = (n + 255) / 256;
@@ -552,25 +569,71 @@ int tbs_dgemvLU(tbs_ctx *ctx, int nelem, const double *LU, const double *x, doub
     return TBS_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The norm's kernels on ctx's stream with the partials in part[0, nb) and the
+// result in part[nb], then the result to the host (synchronous).
+int norm2_with(tbs_ctx *ctx, long n, const double *v, double *part, long nb, double *out_host) {
+    hipLaunchKernelGGL(sumsq_partial, dim3((unsigned)nb), dim3(256), 0, ctx->stream, n, kNormChunk, v, part);
+    TBS_LAUNCHED("sumsq_partial");
+    hipLaunchKernelGGL(sum_final, dim3(1), dim3(256), 0, ctx->stream, (int)nb, part, part + nb);
+    TBS_LAUNCHED("sum_final");
+    TBS_HIP(hipMemcpyAsync(ctx->red_host, part + nb, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    TBS_HIP(hipStreamSynchronize(ctx->stream));
+    *out_host = *ctx->red_host;
+    return TBS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int tbs_norm2(tbs_ctx *ctx, long n, const double *v, double *out_host) {
     int rc = tbs::use_ctx(ctx);
     if (rc) return rc;
     TBS_REQUIRE(n >= 0 && out_host, TBS_ERR_ARG, "bad norm arguments");
     if (n == 0) { *out_host = 0.0; return TBS_OK; }
     TBS_REQUIRE(v, TBS_ERR_ARG, "null vector");
-    if (!ctx->red_buf) TBS_HIP(hipMalloc((void **)&ctx->red_buf, (kNormBlocks + 1) * sizeof(double)));
-    const long chunk = (n + kNormBlocks - 1) / kNormBlocks;
-    hipLaunchKernelGGL(sumsq_partial, dim3(kNormBlocks), dim3(256), 0, ctx->stream, n, chunk, v,
-                       ctx->red_buf);
-    TBS_LAUNCHED("sumsq_partial");
-    hipLaunchKernelGGL(sum_final, dim3(1), dim3(256), 0, ctx->stream, kNormBlocks, ctx->red_buf,
-                       ctx->red_buf + kNormBlocks);
-    TBS_LAUNCHED("sum_final");
-    TBS_HIP(hipMemcpyAsync(ctx->red_host, ctx->red_buf + kNormBlocks, sizeof(double),
-                           hipMemcpyDeviceToHost, ctx->stream));
-    TBS_HIP(hipStreamSynchronize(ctx->stream));
-    *out_host = *ctx->red_host;
-    return TBS_OK;
+    const long nb = (n + kNormChunk - 1) / kNormChunk;
+    TBS_REQUIRE(nb < INT_MAX, TBS_ERR_ARG, "vector of %ld values too long", n);
+    // the context's partial buffer, grown (after its last use) to nb + 1
+    if ((size_t)nb + 1 > ctx->red_cap) {
+        if (ctx->red_buf) {
+            TBS_HIP(hipStreamSynchronize(ctx->stream));
+            TBS_HIP(hipFree(ctx->red_buf));
+        }
+        ctx->red_buf = nullptr;
+        ctx->red_cap = 0;
+        const size_t cap = std::max<size_t>((size_t)nb + 1, 4096);
+        TBS_HIP(hipMalloc((void **)&ctx->red_buf, cap * sizeof(double)));
+        ctx->red_cap = cap;
+    }
+    return norm2_with(ctx, n, v, ctx->red_buf, nb, out_host);
+}
+
+int tbs_norm2_noalloc(tbs_ctx *ctx, long n, const double *v, double *out_host) {
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    TBS_REQUIRE(n >= 0 && out_host, TBS_ERR_ARG, "bad norm arguments");
+    if (n == 0) { *out_host = 0.0; return TBS_OK; }
+    TBS_REQUIRE(v, TBS_ERR_ARG, "null vector");
+    const long nb = (n + kNormChunk - 1) / kNormChunk;
+    TBS_REQUIRE(nb <= kNoAllocMaxPartials, TBS_ERR_ARG,
+                "tbs_norm2_noalloc: %ld values exceed its maximum of %ld (its static scratch holds %ld "
+                "partials); use tbs_norm2", n, kNoAllocMaxPartials * kNormChunk, kNoAllocMaxPartials);
+    const int dev = ctx->device;
+    TBS_REQUIRE(dev >= 0 && dev < kMaxDevices, TBS_ERR_UNSUPPORTED, "device %d beyond %d", dev, kMaxDevices);
+    // the device's copy of the scratch array: its address, taken once per
+    // device with that device current (use_ctx above)
+    std::call_once(g_scratch_once[dev], [dev]() {
+        g_scratch_err[dev] = hipGetSymbolAddress((void **)&g_scratch_ptr[dev], HIP_SYMBOL(scratch));
+    });
+    if (g_scratch_err[dev] != hipSuccess || !g_scratch_ptr[dev])
+        return tbs::hip_fail(g_scratch_err[dev] != hipSuccess ? g_scratch_err[dev] : hipErrorInvalidSymbol,
+                             "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+    return norm2_with(ctx, n, v, g_scratch_ptr[dev], nb, out_host);
 }
 
 }  // extern "C"
