@@ -111,15 +111,21 @@ def test_thomas_variants_bitwise(tbs, ctxs, oracle, prec):
         assert relerr(outs[0]["U"], ref["U"]) < 1e-12
 
 
-@pytest.mark.parametrize("nv", [2, 3, 6, 7, 8, 10, 11, 12, 13, 14, 15, 16])
+@pytest.mark.parametrize("nv", [2, 3, 6, 7, 8, 10, 11, 12, 13, 14, 15, 16] + list(range(17, 33)))
 def test_block_sizes(tbs, ctxs, oracle, nv):
-    """Every compiled block size of the fused path: FP64 history and U against
-    the oracle, FP32 factors converge to the same solution."""
+    """Every compiled block size of the fused path (17..32: the two-columns-
+    per-lane factor kernel): FP64 history and U against the oracle, FP32
+    factors converge to the same solution.  Five sweeps: on these small
+    systems (207 elements) the residual reaches the round-off floor (~1e-16
+    of h_0) by sweep 7-8 for the larger blocks, where two correct
+    implementations differ by the floor itself and the 1e-12 * max(h, 1e-4)
+    bound is no longer a parity test."""
     print(f"case block size {nv}", flush=True)
     m = ragged_lines([9, 1, 40, 23, 64, 70], nv)
     jac = synth_np.generate(20 + nv, nv, m.nelem, m.nintface, 2.0 * nv + 2)
-    ref = oracle.lij_run(m, jac, 8)
-    out = solve_plan(tbs, ctxs(nv), m, jac, 8)
+    ref = oracle.lij_run(m, jac, 5)
+    out = solve_plan(tbs, ctxs(nv), m, jac, 5)
+    assert out["hist"][-1] > 1e-13 * out["hist"][0]       # still above the floor
     ok, worst = cf.history_close(out["hist"], ref["hist"])
     assert ok, worst
     assert relerr(out["U"], ref["U"]) < 1e-12
@@ -141,7 +147,7 @@ def test_fp32_factors_reach_fp64_accuracy(tbs, ctxs, oracle):
     assert relerr(o32["U"], ref["U"]) < 1e-9
 
 
-@pytest.mark.parametrize("nv", [2, 3, 5, 9, 16])
+@pytest.mark.parametrize("nv", [2, 3, 5, 9, 16, 17, 21, 22, 32])
 def test_one_sweep_is_the_line_solve(tbs, ctxs, nv):
     """From U = 0 one sweep is U = -M^-1 rhs exactly (M: the lines' block
     tridiagonal, assembled and solved by scipy), whatever the line lengths:

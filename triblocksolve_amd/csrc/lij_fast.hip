@@ -1,1166 +1,9 @@
-// lij_fast.hip -- the fused LIJ sweep path ("plan" / "batch").
-//
-// One LIJ sweep of the reference (src/main.cxx:207-229) is
-//     dU = -M^-1 R  (Thomas per line),  U += dU,  R = rhs + J U.
-// Here it is two kernels plus a tiny norm/convergence kernel, captured in a
-// hipGraph:
-//   fres_kernel   residual R = rhs + D U + sum_faces Off U(nb) for a 64-element
-//                 group per wave (thread per element, SoA-64 block layout, all
-//                 block loads 512 B coalesced), the per-group sum of R^2, and
-//                 the Thomas right-hand side g = D'^-1 (-R) (W = D'^-1 is a
-//                 factor).  R itself is only written when asked for.
-//   fthomas_kernel two waves per line, one from each end ("twisted" block
-//                 Thomas: the factorisation runs top-down to step m-1 and
-//                 bottom-up to step m+1 and closes at step m).  Forward:
-//                 d_k = g_k - E_k d_{k-1} with E_k = D'_k^-1 A_k (bottom half:
-//                 d_{k+1}, E_k = D'_k^-1 C_{k+1}); step m takes both
-//                 neighbours; backward: d_k = d'_k - Gamma_k d_{k+1} with
-//                 Gamma_k = D'_k^-1 C_{k+1} (the reference's DinvC; bottom
-//                 half: d_{k-1}, D'_k^-1 A_k), and U_k += d_k.  Each wave's
-//                 chain is half a line long.  A window of SPW = floor(64/NV) consecutive steps
-//                 is spread over the wave, lane = step*NV + row; each step's
-//                 9x9 matvec uses the previous step's vector as uniform SGPR
-//                 operands (v_readlane), so the per-step dependency chain is one
-//                 FMA column plus the broadcast.  Factor windows are stored
-//                 [window][column j][lane], so each load is contiguous.
-//   ffinal_kernel per system: fixed-order sum of the group partials -> ||R||,
-//                 history, convergence / divergence flags.
-// These are algebraically the reference's sweep; rounding differs from the
-// reference's solveLU-per-step form by O(eps * cond(D')) because the factors
-// are applied as D'^-1 products (computed from the reference's LU factors by
-// its own solveLU).
-//
-// Many independent systems (a batch) run in one launch per kernel through
-// descriptor tables: lines -> (system, m0, n, window offset), groups ->
-// (system, group).  Every per-system quantity is computed from that system's
-// data alone and reduced in a fixed order, so a system's result is bitwise
-// the same alone and inside any batch.
-#include "tbs_common.hpp"
-#include "lane_ops.hpp"
-
-#include <hip/hip_ext.h>
-
-#include <algorithm>
-#include <array>
-#include <cmath>
-#include <climits>
-#include <cstring>
-#include <functional>
-#include <type_traits>
-#include <map>
-#include <memory>
-
-namespace {
-
-using tbs::rdlane;
-
-constexpr int kGroup = 64;      // elements per residual group (one wave)
-constexpr int kHistCap = 4096;  // sweeps of history kept per system
-constexpr int kPad = 64;                  // zero elements before and after U, rhs, R
-constexpr int kSlotNone = INT_MIN;        // no lane of the group has this face
-constexpr int kSlotGather = INT_MIN + 1;  // irregular neighbours: per-lane gather
-
-// Two-ended ("twisted") split of a line of n elements between the two waves
-// of its workgroup: one runs steps 0..m-1 top-down (q whole windows of spw
-// steps), the other steps n-1..m+1 bottom-up (nb windows, the first pad
-// steps of its first window empty), and step m closes the line from both
-// sides.  A line's factor windows: the q top windows, one window holding
-// step m (at its first step), then the nb bottom windows in bottom-up order.
-struct Split { int q, m, B, nb, pad; };
-__host__ __device__ inline Split split_line(int n, int spw) {
-    Split s;
-    const int wall = (n - 1 + spw - 1) / spw;   // windows of the n - 1 steps off the middle
-    s.q = wall / 2;
-    s.m = s.q * spw;
-    s.B = n - 1 - s.m;
-    s.nb = (s.B + spw - 1) / spw;
-    s.pad = s.nb * spw - s.B;
-    return s;
-}
-__host__ __device__ inline int line_windows(int n, int spw) {
-    const Split s = split_line(n, spw);
-    return s.q + 1 + s.nb;
-}
-
-template <typename TF>
-struct SysDev {
-    const double *blk;   // [ngroups][F+1][BS][64] residual blocks (slot 0 = D)
-    const int *nbr;      // [ngroups][F][64] neighbour internal index or -1
-    const int *slotc;    // [ngroups][F] neighbour map of the slot: kSlotNone,
-                         // kSlotGather, or the shift c with nbr = m + c for
-                         // every lane that has a face in this slot
-    const TF *W;         // [ngroups][BS][64] D'^-1
-    const TF *EF;        // [windows][NV][SPW*NV] E_k rows, zero padded
-    const TF *GB;        // [windows][NV][SPW*NV] Gamma_k rows, zero padded
-    const double *rhs;   // [n][NV] internal order; kPad zero elements either side
-    double *U;           // [n][NV]; kPad zero elements either side
-    TF *g;               // [n][NV] Thomas right-hand side
-    TF *dUp;             // [n][NV] forward-sweep intermediate
-    double *R;           // [n][NV] residual (written on request); padded like U
-    double *gpart;       // [ngroups] sum of R^2 per group
-    double *hist;        // [kHistCap]
-    int *state;          // [0] = sweeps recorded, [1] = active, [2] = status
-    int n, ngroups, F, pad;
-};
-
-// ----------------------------------------------------------- residual --
-// acc[i] += B(i,j) x[j] over one SoA-64 block slot, in entry order q = i + NV*j
-// (ascending j for every row i, the reference's summation order).
-template <int NV, typename T>
-__device__ __forceinline__ void slot_matvec(tbs::rsrc_t r, uint32_t slot_soff, int lane,
-                                            const double (&x)[NV], double (&acc)[NV]) {
-    using L = tbs::Soa64<NV * NV, T>;
-#pragma unroll
-    for (int c = 0; c < L::NC; ++c) {
-        const tbs::u32x4 v = tbs::bload16_nt(r, (uint32_t)lane * 16u, slot_soff + (uint32_t)c * 1024u);
-#pragma unroll
-        for (int k = 0; k < L::VW; ++k) {
-            const int q = c * L::VW + k;
-            acc[q % NV] += (double)tbs::lane_elem<T>(v, k) * x[q / NV];
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < L::REM; ++k) {
-        const int q = L::NC * L::VW + k;
-        const T b = tbs::bload_nt<T>(r, (uint32_t)lane * sizeof(T),
-                                     slot_soff + (uint32_t)(L::NC * L::VW * 64 + k * 64) * sizeof(T));
-        acc[q % NV] += (double)b * x[q / NV];
-    }
-}
-
-template <int NV>
-__device__ __forceinline__ void load_vec(tbs::rsrc_t r, uint32_t byte_off, double (&v)[NV]) {
-#pragma unroll
-    for (int j = 0; j < NV; ++j) v[j] = tbs::bload<double>(r, byte_off + 8u * j, 0);
-}
-
-// Copy the vector tile of elements [m_base, m_base + 64) (AoS, NV doubles per
-// element, contiguous) into this wave's LDS tile with 16-byte loads; elements
-// outside the vector read as 0.  Each lane then reads its own element.  NT:
-// the vector is read once (streaming cache policy).
-template <int NV, bool NT = false>
-__device__ __forceinline__ void stage_tile(tbs::rsrc_t r, int m_base, int lane, double *tile) {
-    constexpr int TB = kGroup * NV * 8;          // tile bytes
-    constexpr int NC = TB / 1024;                // full 1 KiB wave loads
-    const uint32_t base = (uint32_t)(m_base * NV * 8);
-#pragma unroll
-    for (int k = 0; k < NC; ++k) {
-        const tbs::u32x4 v = NT ? tbs::bload16_nt(r, base + 16u * lane + 1024u * k, 0)
-                                : tbs::bload16(r, base + 16u * lane + 1024u * k, 0);
-        tile[2 * lane + 128 * k] = tbs::lane_elem<double>(v, 0);
-        tile[2 * lane + 128 * k + 1] = tbs::lane_elem<double>(v, 1);
-    }
-    static_assert(TB - NC * 1024 == 0 || TB - NC * 1024 == 512, "tile = 512*NV bytes");
-    if (TB - NC * 1024 == 512)   // odd NV: one more 8-byte load per lane
-        tile[NC * 128 + lane] = NT ? tbs::bload_nt<double>(r, base + (uint32_t)(NC * 1024) + 8u * lane, 0)
-                                   : tbs::bload<double>(r, base + (uint32_t)(NC * 1024) + 8u * lane, 0);
-    __builtin_amdgcn_wave_barrier();
-}
-
-template <int NV>
-__device__ __forceinline__ void read_tile(const double *tile, int lane, double (&v)[NV]) {
-#pragma unroll
-    for (int j = 0; j < NV; ++j) v[j] = tile[lane * NV + j];
-    __builtin_amdgcn_wave_barrier();
-}
-
-template <int NV, typename TF>
-__global__ __launch_bounds__(256) void fres_kernel(const SysDev<TF> *__restrict__ sys,
-                                                   const int2 *__restrict__ gtab, int ng,
-                                                   int want_R, int force) {
-    constexpr int BS = NV * NV;
-    __shared__ double tiles[4][kGroup * NV];
-    // XCD-aware order: hardware block b runs on XCD b % 8; give each XCD a
-    // contiguous run of groups so neighbouring lines share an L2.
-    const int nblk = (ng + 3) / 4;
-    const int per = (nblk + 7) / 8;
-    const int blk = (int)(blockIdx.x % 8u) * per + (int)(blockIdx.x / 8u);
-    if (blk >= nblk) return;
-    const int wave = threadIdx.x >> 6;
-    const int wid = __builtin_amdgcn_readfirstlane(blk * 4 + wave);
-    if (wid >= ng) return;
-    double *tile = tiles[wave];
-    const int lane = threadIdx.x & 63;
-    const int2 sg = gtab[wid];
-    const SysDev<TF> &S = sys[sg.x];
-    // force bit 0: run even if the system is inactive; bit 1: U is zero
-    // (the first residual of a solve from zero is R = rhs: skip J U)
-    if (!(force & 1) && !S.state[1]) return;
-    const bool u_zero = (force & 2) != 0;
-    const int g = sg.y;
-    const int F = S.F;
-    const int m = g * kGroup + lane;
-    const int m0 = g * kGroup;
-    constexpr uint32_t kSlot = (uint32_t)BS * kGroup * sizeof(double);
-    const tbs::rsrc_t rB = tbs::make_rsrc(S.blk + (size_t)g * (F + 1) * BS * kGroup, (F + 1) * kSlot);
-    const tbs::rsrc_t rN = tbs::make_rsrc(S.nbr + (size_t)g * F * kGroup, (uint32_t)F * kGroup * 4u);
-    const tbs::rsrc_t rW = tbs::make_rsrc(S.W + (size_t)g * BS * kGroup, (uint32_t)BS * kGroup * sizeof(TF));
-    // U and rhs descriptors start kPad elements before element 0, so tiles
-    // shifted by a neighbour offset never start at a negative offset while
-    // they still touch real elements.
-    const uint32_t pbytes = (uint32_t)(S.n + 2 * kPad) * NV * 8u;
-    const tbs::rsrc_t rU = tbs::make_rsrc(S.U - kPad * NV, pbytes);
-    const tbs::rsrc_t rRhs = tbs::make_rsrc(S.rhs - kPad * NV, pbytes);
-    const uint32_t moff = (uint32_t)m * NV * 8u;
-
-    double u[NV], acc[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) acc[i] = 0.0;
-    if (!u_zero) {
-    stage_tile<NV>(rU, m0 + kPad, lane, tile);
-    read_tile<NV>(tile, lane, u);
-    slot_matvec<NV, double>(rB, 0u, lane, u, acc);                     // D U
-    for (int f = 0; f < F; ++f) {
-        const int c = S.slotc[g * F + f];
-        if (c == kSlotNone) continue;
-        const int nb = tbs::bload<int>(rN, (uint32_t)(f * kGroup + lane) * 4u, 0);
-        double un[NV];
-        if (c != kSlotGather) {                                           // wave-uniform
-            stage_tile<NV>(rU, m0 + c + kPad, lane, tile);
-            read_tile<NV>(tile, lane, un);
-        }
-        if (nb >= 0) {                                                    // interior face
-            if (c == kSlotGather) load_vec<NV>(rU, (uint32_t)(nb + kPad) * NV * 8u, un);
-            double t[NV];
-#pragma unroll
-            for (int i = 0; i < NV; ++i) t[i] = 0.0;
-            slot_matvec<NV, double>(rB, (uint32_t)(1 + f) * kSlot, lane, un, t);
-#pragma unroll
-            for (int i = 0; i < NV; ++i) acc[i] += t[i];
-        }
-    }
-    }
-    double rh[NV], r[NV], nr[NV];
-    stage_tile<NV, true>(rRhs, m0 + kPad, lane, tile);
-    read_tile<NV>(tile, lane, rh);
-    double rr = 0.0;
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        r[i] = rh[i] + acc[i];                                            // 0 for padded lanes
-        rr += r[i] * r[i];
-        nr[i] = -r[i];
-    }
-    double gv[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) gv[i] = 0.0;
-    slot_matvec<NV, TF>(rW, 0u, lane, nr, gv);                          // g = D'^-1 (-R)
-    const tbs::rsrc_t rG = tbs::make_rsrc(S.g, (uint32_t)S.n * NV * sizeof(TF));
-    const uint32_t goff = (uint32_t)m * NV * sizeof(TF);
-#pragma unroll
-    for (int i = 0; i < NV; ++i) tbs::bstore<TF>(rG, (TF)gv[i], goff + (uint32_t)i * sizeof(TF), 0);
-    if (want_R) {
-        const tbs::rsrc_t rR = tbs::make_rsrc(S.R, (uint32_t)S.n * NV * 8u);
-#pragma unroll
-        for (int i = 0; i < NV; ++i) tbs::bstore<double>(rR, r[i], moff + 8u * i, 0);
-    }
-    rr = tbs::wave_sum(rr);
-    if (lane == 0) S.gpart[g] = rr;
-}
-
-// -------------------------------------------------------------- Thomas --
-// Vector I/O of a group of windows: V values of T, contiguous in memory,
-// moved with 16-byte wave loads/stores and staged in LDS ([flat] layout equal
-// to the memory layout).  NCH 1 KiB chunks cover V values (+1 spare value so
-// the idle lane of a window may read one past the end).
-template <int V, typename T>
-struct GroupIO {
-    static constexpr int VW = 16 / (int)sizeof(T);
-    static constexpr int NCH = (V + 1 + 64 * VW - 1) / (64 * VW);
-    static constexpr int LDS = NCH * 64 * VW;            // padded LDS length
-};
-
-template <int V, typename T, bool NT = false>
-__device__ __forceinline__ void bulk_load(tbs::rsrc_t r, uint32_t base, int lane,
-                                          tbs::u32x4 (&reg)[GroupIO<V, T>::NCH]) {
-#pragma unroll
-    for (int c = 0; c < GroupIO<V, T>::NCH; ++c)
-        reg[c] = NT ? tbs::bload16_nt(r, base + (uint32_t)(c * 1024 + lane * 16), 0)
-                    : tbs::bload16(r, base + (uint32_t)(c * 1024 + lane * 16), 0);
-}
-
-template <int V, typename T>
-__device__ __forceinline__ void bulk_commit(const tbs::u32x4 (&reg)[GroupIO<V, T>::NCH], int lane, T *lds) {
-    using IO = GroupIO<V, T>;
-#pragma unroll
-    for (int c = 0; c < IO::NCH; ++c) {
-#pragma unroll
-        for (int k = 0; k < IO::VW; ++k) lds[c * 64 * IO::VW + lane * IO::VW + k] = tbs::lane_elem<T>(reg[c], k);
-    }
-    __builtin_amdgcn_wave_barrier();
-}
-
-// Store exactly V values from LDS (16-byte stores for whole chunks, per-value
-// stores for the tail so nothing past the group is overwritten).
-template <int V, typename T, bool NT = false>
-__device__ __forceinline__ void bulk_store(tbs::rsrc_t r, uint32_t base, int lane, const T *lds) {
-    using IO = GroupIO<V, T>;
-    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-    for (int c = 0; c < IO::NCH; ++c) {
-        const int first = c * 64 * IO::VW + lane * IO::VW;
-        if ((c + 1) * 64 * IO::VW <= V) {                      // whole chunk inside the group
-            tbs::u32x4 v;
-            if constexpr (sizeof(T) == 8) {
-#pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    const unsigned long long bits = __builtin_bit_cast(unsigned long long, (double)lds[first + k]);
-                    v[2 * k] = (unsigned)bits;
-                    v[2 * k + 1] = (unsigned)(bits >> 32);
-                }
-            } else {
-#pragma unroll
-                for (int k = 0; k < 4; ++k) v[k] = __builtin_bit_cast(unsigned, (float)lds[first + k]);
-            }
-            __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)(base + (uint32_t)(c * 1024 + lane * 16)), 0,
-                                                   NT ? tbs::kCacheNT : 0);
-        } else {
-#pragma unroll
-            for (int k = 0; k < IO::VW; ++k) {
-                const int q = first + k;
-                const uint32_t off = q < V ? base + (uint32_t)q * sizeof(T) : tbs::kOOB;
-                if (NT) tbs::bstore_nt<T>(r, lds[q], off, 0);
-                else tbs::bstore<T>(r, lds[q], off, 0);
-            }
-        }
-    }
-}
-
-// Two waves per line (split_line): wave 0 eliminates steps 0..m-1 top-down,
-// wave 1 steps n-1..m+1 bottom-up, each over windows of SPW steps with PF
-// factor windows prefetched in a register ring and its vectors staged per
-// group of G = K*(PF+1) windows in LDS; wave 0 then closes step m from both
-// sides and both waves substitute back outwards from it.  The bottom wave
-// runs the same loops over its own windows; only the place of a step's
-// values in memory differs (element m0+n-1-s for its step s, less pad).
-template <int NV, typename TF, int PF, int K>
-__global__ __launch_bounds__(128) void fthomas_kernel(const SysDev<TF> *__restrict__ sys,
-                                                      const int4 *__restrict__ ltab, int nl) {
-    constexpr int SPW = 64 / NV;      // steps per window
-    constexpr int LW = SPW * NV;      // lanes carrying data
-    constexpr int NB = PF + 1;        // register ring depth
-    constexpr int G = K * NB;         // windows per group (loop iteration)
-    constexpr int VG = G * LW;        // vector values per group
-    using IOT = GroupIO<VG, TF>;
-    using IOD = GroupIO<VG, double>;
-    __shared__ TF s_vin[2][2][IOT::LDS];    // [wave] forward: g; backward: d'
-    __shared__ TF s_out[2][IOT::LDS];       // [wave] forward: d' staging
-    __shared__ double s_u[2][2][IOD::LDS];  // [wave] backward: U (updated in place)
-    __shared__ TF s_x[3][NV];               // d' at step m-1, d' at step m+1, d at step m
-    const int l = blockIdx.x;
-    if (l >= nl) return;
-    // {system, m0, n, window offset}, wave-uniform by construction: the
-    // group loops below must be scalar loops (the chain broadcasts read
-    // lanes that every step writes)
-    int4 lt = ltab[l];
-    lt.x = __builtin_amdgcn_readfirstlane(lt.x);
-    lt.y = __builtin_amdgcn_readfirstlane(lt.y);
-    lt.z = __builtin_amdgcn_readfirstlane(lt.z);
-    lt.w = __builtin_amdgcn_readfirstlane(lt.w);
-    const SysDev<TF> &S = sys[lt.x];
-    if (!S.state[1]) return;                                            // the whole workgroup
-    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-    const bool bot = wave == 1;
-    const int lane = threadIdx.x & 63;
-    const int st_of_lane = lane / NV;
-    const int row = lane - st_of_lane * NV;
-    const int m0 = lt.y, n = lt.z;
-    const Split sp = split_line(n, SPW);
-    const int nw = bot ? sp.nb : sp.q;                                  // the wave's windows
-    const int ngrp = (nw + G - 1) / G;
-    // the bottom wave runs voff empty windows before its first one, so that
-    // its groups of windows start at whole groups of elements from element
-    // m+1 (every group's values start at an offset >= 0 in the wave's range)
-    const int voff = bot ? ngrp * G - nw : 0;
-    constexpr int WSTRIDE = tbs::WinLayout<NV, LW, TF>::STRIDE;
-    const uint32_t wbytes = (uint32_t)WSTRIDE * sizeof(TF);           // one window of factors
-    const long wfirst = (long)lt.w + (bot ? sp.q + 1 : 0);
-    const tbs::rsrc_t rEF = tbs::make_rsrc(S.EF + wfirst * WSTRIDE, (uint32_t)nw * wbytes);
-    const tbs::rsrc_t rGB = tbs::make_rsrc(S.GB + wfirst * WSTRIDE, (uint32_t)nw * wbytes);
-    // the wave's elements: [m0, m0+m) top-down, [m0+m+1, m0+n) bottom-up
-    const int e0 = bot ? m0 + sp.m + 1 : m0;
-    const uint32_t vb = (uint32_t)(bot ? sp.B : sp.m) * NV;           // vector values of the wave
-    const tbs::rsrc_t rg = tbs::make_rsrc(S.g + (size_t)NV * e0, vb * sizeof(TF));
-    const tbs::rsrc_t rd = tbs::make_rsrc(S.dUp + (size_t)NV * e0, vb * sizeof(TF));
-    const tbs::rsrc_t rU = tbs::make_rsrc(S.U + (size_t)NV * e0, vb * 8u);
-    // Where the values of a group start in memory (first element, relative
-    // to the wave's first element; always >= 0) and where a step of window bb
-    // (lane = st*NV + row) sits in the group's LDS copy, which has the memory
-    // layout.  Groups are numbered in the direction of the forward pass: the
-    // forward pass visits groups 0, 1, ..., the backward pass the reverse;
-    // group jg holds the wave's windows jg*G ... jg*G+G-1 (bottom wave:
-    // counted with its voff empty windows).
-    const int lofs = lane < LW ? (bot ? row - st_of_lane * NV : lane) : 0;
-    auto gbase = [&](int jg) -> int { return (bot ? ngrp - 1 - jg : jg) * G * SPW; };
-    auto fwd_slot = [&](int bb) -> int { return lane < LW ? (bot ? VG - NV - bb * LW + lofs : bb * LW + lofs) : 0; };
-    auto bwd_slot = [&](int bb) -> int { return lane < LW ? (bot ? (bb + 1) * LW - NV + lofs : (G - 1 - bb) * LW + lofs) : 0; };
-    auto vbytes = [&](int elem, int bytes_per) -> uint32_t { return (uint32_t)(elem * NV * bytes_per); };
-
-    using WL = tbs::WinLayout<NV, LW, TF>;
-    auto wbase = [&](int w) -> uint32_t { return w >= 0 ? (uint32_t)w * wbytes : tbs::kOOBBase; };
-    auto load_row = [&](tbs::rsrc_t r, int w, TF (&e)[NV]) {
-        const uint32_t b = wbase(w);
-#pragma unroll
-        for (int c = 0; c < WL::NC; ++c) {
-            const tbs::u32x4 v = tbs::bload16_nt(r, b + (uint32_t)(c * LW * 16 + lane * 16), 0);
-#pragma unroll
-            for (int k = 0; k < WL::VW; ++k) e[c * WL::VW + k] = tbs::lane_elem<TF>(v, k);
-        }
-#pragma unroll
-        for (int k = 0; k < WL::REM; ++k)
-            e[WL::NC * WL::VW + k] = tbs::bload_nt<TF>(r, b + (uint32_t)((WL::NC * WL::VW * LW + k * LW + lane) * sizeof(TF)), 0);
-    };
-
-    if (lane < NV) s_x[wave][lane] = TF(0);                             // an empty side contributes 0
-    TF d[NV];
-    TF eb[NB][NV];
-    // ---- forward: d_k = g_k - E_k d_{k-1} (bottom wave: d_{k+1})
-#pragma unroll
-    for (int j = 0; j < NV; ++j) d[j] = TF(0);
-#pragma unroll
-    for (int p = 0; p < PF; ++p) load_row(rEF, p - voff, eb[p]);
-    {
-        tbs::u32x4 vr[IOT::NCH];
-        bulk_load<VG, TF, true>(rg, vbytes(gbase(0), sizeof(TF)), lane, vr);
-        bulk_commit<VG, TF>(vr, lane, s_vin[wave][0]);
-    }
-    for (int jg = 0; jg < ngrp; ++jg) {
-        const int cur = jg & 1;
-        tbs::u32x4 vr[IOT::NCH];
-        if (jg + 1 < ngrp)                                              // next group's g
-            bulk_load<VG, TF, true>(rg, vbytes(gbase(jg + 1), sizeof(TF)), lane, vr);
-#pragma unroll
-        for (int bb = 0; bb < G; ++bb) {
-            const int w = jg * G + bb - voff;                           // the wave's window
-            const int b = bb % NB;
-            const int bn = (b + PF) % NB;
-            load_row(rEF, w + PF, eb[bn]);
-            const int slot = fwd_slot(bb);
-            const TF gv = s_vin[wave][cur][slot];
-            TF out = TF(0);
-#pragma unroll
-            for (int st = 0; st < SPW; ++st) {
-                TF x0 = TF(0), x1 = TF(0), x2 = TF(0);
-#pragma unroll
-                for (int j = 0; j < NV; j += 3) {
-                    x0 += eb[b][j] * d[j];
-                    if (j + 1 < NV) x1 += eb[b][j + 1] * d[j + 1];
-                    if (j + 2 < NV) x2 += eb[b][j + 2] * d[j + 2];
-                }
-                const TF v = gv - ((x0 + x1) + x2);
-#pragma unroll
-                for (int j = 0; j < NV; ++j) d[j] = rdlane(v, st * NV + j);
-                out = (st_of_lane == st) ? v : out;
-            }
-            if (lane < LW) s_out[wave][slot] = out;
-            // the wave's last step (step m-1, or m+1 for the bottom wave)
-            if (w == nw - 1 && st_of_lane == SPW - 1 && lane < LW) s_x[wave][row] = out;
-        }
-        bulk_store<VG, TF>(rd, vbytes(gbase(jg), sizeof(TF)), lane, s_out[wave]);
-        if (jg + 1 < ngrp) bulk_commit<VG, TF>(vr, lane, s_vin[wave][cur ^ 1]);
-    }
-    __syncthreads();
-    // ---- step m: d_m = g_m - E_m d'_{m-1} - F_m d'_{m+1}, U_m += d_m
-    if (!bot && lane < NV) {
-        const tbs::rsrc_t rME = tbs::make_rsrc(S.EF + ((long)lt.w + sp.q) * WSTRIDE, wbytes);
-        const tbs::rsrc_t rMF = tbs::make_rsrc(S.GB + ((long)lt.w + sp.q) * WSTRIDE, wbytes);
-        TF em[NV], fm[NV];
-        load_row(rME, 0, em);
-        load_row(rMF, 0, fm);
-        const size_t at = (size_t)NV * (m0 + sp.m) + lane;
-        TF x0 = TF(0), x1 = TF(0);
-#pragma unroll
-        for (int j = 0; j < NV; ++j) {
-            x0 += em[j] * s_x[0][j];
-            x1 += fm[j] * s_x[1][j];
-        }
-        const TF v = S.g[at] - (x0 + x1);
-        s_x[2][lane] = v;
-        S.U[at] += (double)v;
-    }
-    __syncthreads();
-
-    // ---- backward: d_k = d'_k - Gamma_k d_{k+1} (bottom wave: d_{k-1}); U_k += d_k
-    // The groups in reverse, each from its last window down.  The top wave's
-    // last group may end past its last window: those windows are skipped
-    // (d must reach window nw-1 unchanged); the bottom wave's empty windows
-    // come last and change nothing that is stored.
-#pragma unroll
-    for (int j = 0; j < NV; ++j) d[j] = s_x[2][j];
-    const int wtop = ngrp * G - voff;                                   // one past the last window visited
-#pragma unroll
-    for (int p = 0; p < PF; ++p) load_row(rGB, wtop - 1 - p, eb[p]);
-    {
-        tbs::u32x4 vr[IOT::NCH];
-        tbs::u32x4 ur[IOD::NCH];
-        bulk_load<VG, TF, true>(rd, vbytes(gbase(ngrp - 1), sizeof(TF)), lane, vr);
-        bulk_load<VG, double>(rU, vbytes(gbase(ngrp - 1), 8), lane, ur);
-        bulk_commit<VG, TF>(vr, lane, s_vin[wave][0]);
-        bulk_commit<VG, double>(ur, lane, s_u[wave][0]);
-    }
-    for (int t = 0; t < ngrp; ++t) {
-        const int jg = ngrp - 1 - t;
-        const int cur = t & 1;
-        tbs::u32x4 vr[IOT::NCH];
-        tbs::u32x4 ur[IOD::NCH];
-        if (jg > 0) {                                                   // next group's d', U
-            bulk_load<VG, TF, true>(rd, vbytes(gbase(jg - 1), sizeof(TF)), lane, vr);
-            bulk_load<VG, double>(rU, vbytes(gbase(jg - 1), 8), lane, ur);
-        }
-#pragma unroll
-        for (int bb = 0; bb < G; ++bb) {
-            const int w = wtop - 1 - (t * G + bb);
-            const int b = bb % NB;
-            const int bn = (b + PF) % NB;
-            load_row(rGB, w - PF, eb[bn]);
-            if (w >= nw) continue;                                      // past the top wave's last window
-            const int slot = bwd_slot(bb);
-            const TF pv = s_vin[wave][cur][slot];
-            TF out = TF(0);
-#pragma unroll
-            for (int st = SPW - 1; st >= 0; --st) {
-                TF x0 = TF(0), x1 = TF(0), x2 = TF(0);
-#pragma unroll
-                for (int j = 0; j < NV; j += 3) {
-                    x0 += eb[b][j] * d[j];
-                    if (j + 1 < NV) x1 += eb[b][j + 1] * d[j + 1];
-                    if (j + 2 < NV) x2 += eb[b][j + 2] * d[j + 2];
-                }
-                const TF v = pv - ((x0 + x1) + x2);
-#pragma unroll
-                for (int j = 0; j < NV; ++j) d[j] = rdlane(v, st * NV + j);
-                out = (st_of_lane == st) ? v : out;
-            }
-            if (lane < LW) s_u[wave][cur][slot] += (double)out;
-        }
-        bulk_store<VG, double>(rU, vbytes(gbase(jg), 8), lane, s_u[wave][cur]);
-        if (jg > 0) {
-            bulk_commit<VG, TF>(vr, lane, s_vin[wave][cur ^ 1]);
-            bulk_commit<VG, double>(ur, lane, s_u[wave][cur ^ 1]);
-        }
-    }
-}
-
-// ------------------------------------------------ norm / convergence --
-// One block per system: fixed-order sum of the group partials (thread t
-// adds partials t, t + 1024, ... in order, then a fixed LDS tree).  Each
-// thread issues the loads of up to kFinalPer partials before it adds any, so
-// the pass costs one memory round trip per kFinalPer * 1024 partials.
-constexpr int kFinalThreads = 1024;
-constexpr int kFinalPer = 32;
-template <typename TF>
-__global__ __launch_bounds__(kFinalThreads) void ffinal_kernel(const SysDev<TF> *__restrict__ sys, int nsys,
-                                                               int max_sweeps, double rtol, int force) {
-    __shared__ double s[kFinalThreads];
-    const int sidx = blockIdx.x;
-    if (sidx >= nsys) return;
-    const SysDev<TF> &S = sys[sidx];
-    if (!force && !S.state[1]) return;
-    const int ng = S.ngroups;
-    double acc = 0.0;
-    for (int base = 0; base < ng; base += kFinalPer * kFinalThreads) {
-        double v[kFinalPer];
-#pragma unroll
-        for (int q = 0; q < kFinalPer; ++q) {
-            const int i = base + q * kFinalThreads + (int)threadIdx.x;
-            v[q] = i < ng ? S.gpart[i] : 0.0;
-        }
-#pragma unroll
-        for (int q = 0; q < kFinalPer; ++q)
-            if (base + q * kFinalThreads + (int)threadIdx.x < ng) acc += v[q];
-    }
-    s[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = kFinalThreads / 2; w >= 1; w >>= 1) {
-        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        const double h = sqrt(s[0]);
-        const int k = S.state[0];
-        if (k < kHistCap) S.hist[k] = h;
-        S.state[0] = k + 1;
-        int active = 1, status = 0;
-        if (!(h == h) || h > 1.0e300) { active = 0; status = 2; }          // non-finite
-        else if (k >= 1 && h <= rtol * S.hist[0]) { active = 0; status = 1; }  // converged
-        else if (k >= max_sweeps || k + 1 >= kHistCap) { active = 0; status = 3; }
-        else if (k >= 8 && h > 1.0e6 * S.hist[0]) { active = 0; status = 4; }  // diverging
-        S.state[1] = active;
-        S.state[2] = status;
-    }
-}
-
-template <typename TF>
-__global__ void reset_state_kernel(const SysDev<TF> *__restrict__ sys, int nsys) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nsys) {
-        sys[i].state[0] = 0;
-        sys[i].state[1] = 1;
-        sys[i].state[2] = 0;
-    }
-}
-
-template <typename TF>
-__global__ void gather_states_kernel(const SysDev<TF> *__restrict__ sys, int nsys, int *__restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < 4 * nsys) out[i] = sys[i >> 2].state[i & 3];
-}
-
-// ------------------------------------------------------------ packing --
-// Residual blocks into the SoA-64 layout (tbs::Soa64<BS, double>): src code
-// per (internal element, slot): x = array (0 none, 1 jacD, 2 jacO1, 3 jacO2),
-// y = block index.  The destination is enumerated in order (coalesced writes).
-template <int NV>
-__global__ void pack_blocks_kernel(long total, int S, const int2 *__restrict__ src,
-                                   const double *__restrict__ D, const double *__restrict__ O1,
-                                   const double *__restrict__ O2, double *__restrict__ blk) {
-    constexpr int BS = NV * NV;
-    using L = tbs::Soa64<BS, double>;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (long)gridDim.x * blockDim.x) {
-        const int idx = (int)(t % L::ELEMS);
-        const long gs = t / L::ELEMS;
-        const int slot = (int)(gs % S);
-        const long g = gs / S;
-        int q, lane;
-        if (idx < L::NC * L::VW * 64) {
-            const int c = idx / (64 * L::VW), rem = idx % (64 * L::VW);
-            lane = rem / L::VW;
-            q = c * L::VW + rem % L::VW;
-        } else {
-            const int r = idx - L::NC * L::VW * 64;
-            q = L::NC * L::VW + r / 64;
-            lane = r % 64;
-        }
-        const long m = g * kGroup + lane;
-        const int2 c = src[m * S + slot];
-        double v = 0.0;
-        if (c.x == 1) v = D[(size_t)BS * c.y + q];
-        else if (c.x == 2) v = O1[(size_t)BS * c.y + q];
-        else if (c.x == 3) v = O2[(size_t)BS * c.y + q];
-        blk[t] = v;
-    }
-}
-
-// Same packing through LDS: one 256-thread workgroup per (group, slot) reads
-// the group's 64 source blocks whole (consecutive threads read consecutive
-// entries of a block, so every load is contiguous), then writes the slot in
-// SoA-64 order (consecutive threads write consecutive addresses).
-// One 128-thread workgroup per half of a (group, slot): the 32 source blocks
-// of lanes h*32 .. h*32+31 come into LDS whole, in 16-byte pieces (consecutive
-// threads read consecutive pieces of a block; a block is contiguous in the
-// source array, 8-byte aligned), and leave as those lanes' part of the SoA-64
-// slot: 512-byte runs of 16-byte pieces.  (Half groups keep the LDS of a
-// workgroup at 21 KB, so seven fit a CU.)
-typedef double f64x2u __attribute__((ext_vector_type(2), aligned(8)));
-constexpr int kPackLanes = 32;
-template <int NV>
-__global__ __launch_bounds__(128) void pack_lds_kernel(int S, const int2 *__restrict__ src,
-                                                       const double *__restrict__ D,
-                                                       const double *__restrict__ O1,
-                                                       const double *__restrict__ O2,
-                                                       double *__restrict__ blk) {
-    constexpr int BS = NV * NV;
-    constexpr int PB = (BS + 1) / 2;           // 16-byte pieces per block (the last may be 8 bytes)
-    constexpr int EL = kPackLanes;
-    using L = tbs::Soa64<BS, double>;
-    static_assert(L::VW == 2, "SoA-64 slots of doubles hold 2 entries per 16-byte chunk");
-    __shared__ double s[EL * BS];              // [element lane - h*EL][entry]
-    __shared__ const double *s_src[EL];
-    const long w = blockIdx.x >> 1;            // (group, slot)
-    const int h = (int)(blockIdx.x & 1u);      // half of the group's lanes
-    const long g = w / S;
-    const int slot = (int)(w - g * S);
-    const int t = threadIdx.x;
-    if (t < EL) {
-        const int2 c = src[(g * kGroup + h * EL + t) * S + slot];
-        const double *a = c.x == 1 ? D : (c.x == 2 ? O1 : (c.x == 3 ? O2 : nullptr));
-        s_src[t] = a ? a + (size_t)BS * c.y : nullptr;
-    }
-    __syncthreads();
-    for (int i = t; i < EL * PB; i += 128) {
-        const int e = i / PB, k = i - e * PB;
-        const double *p = s_src[e];
-        double *d = s + e * BS + 2 * k;
-        if (2 * k + 1 < BS) {
-            const f64x2u v = p ? *reinterpret_cast<const f64x2u *>(p + 2 * k) : f64x2u{0.0, 0.0};
-            d[0] = v[0];
-            d[1] = v[1];
-        } else {
-            d[0] = p ? p[2 * k] : 0.0;
-        }
-    }
-    __syncthreads();
-    double *out = blk + w * L::ELEMS;
-    for (int i = t; i < L::NC * EL; i += 128) {              // chunk c, lane h*EL + e: entries 2c, 2c+1
-        const int c = i / EL, e = i - c * EL;
-        const double *q = s + e * BS + 2 * c;
-        *reinterpret_cast<f64x2u *>(out + c * 128 + (h * EL + e) * 2) = f64x2u{q[0], q[1]};
-    }
-    for (int i = t; i < L::REM * EL; i += 128)               // odd BS: the last entry, [lane]
-        out[L::NC * 128 + h * EL + i] = s[i * BS + BS - 1];
-}
-
-// ------------------------------------------------------------- factor --
-// 1/x from the hardware reciprocal estimate refined by two Newton steps
-// (each doubles the correct bits), for the Gauss-Jordan pivots.
-__device__ __forceinline__ double recip(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-x, r, 1.0);
-    return fma(r, e, r);
-}
-
-// Block-Thomas factorisation of every line straight into the sweep-form
-// factors.  src/okl/lineLU_v2.okl:58-182 runs the same recurrence on LU
-// factors; with A_k, C_k the blocks at face f_k = lineface[m0+k] between
-// elements k-1 and k (A = Of2 and C = Of1 when element k is the face's
-// first element fc[2f], else the reverse, lineLU_v2.okl:116-117):
-//   S_0 = D_0,   S_k = D_k - A_k Gamma_{k-1}               (:147-162)
-//   [Gamma_k | E_k | W_k] = S_k^{-1} [C_{k+1} | A_k | I]
-// by Gauss-Jordan elimination without pivoting (the reference's LU does not
-// pivot either).  Gamma_k is the reference's DinvC, W_k = D'_k^{-1} and
-// E_k = D'_k^{-1} A_k are what the sweep kernels read.
-//
-// Two waves per line (split_line): wave 0 runs steps 0..m-1 as above, wave 1
-// runs steps n-1..m+1 of the same recurrence bottom-up (S_k = D_k -
-// C_{k+1} Gamma''_{k+1}, [Gamma''_k | E''_k | W_k] = S_k^{-1} [A_k | C_{k+1} |
-// I]: the same step with the roles of A and C exchanged), and wave 0 then
-// closes step m with both neighbours, S_m = D_m - A_m Gamma_{m-1} -
-// C_{m+1} Gamma''_{m+1}, [F_m | E_m | W_m] = S_m^{-1} [C_{m+1} | A_m | I]
-// (F_m is stored in Gamma's place).
-// In each wave, one lane per column: lanes P..P+NV-1 hold S_k (the
-// pivot columns), X..X+NV-1 hold C_{k+1} and end with Gamma_k, 2NV.. hold A_k
-// then E_k, 3NV.. hold I then W_k.  P and X swap every step, so the lanes
-// that end a step holding Gamma_k form S_{k+1} in place; each pivot step
-// broadcasts one column with readlane.  A lone wave issues one instruction
-// every four cycles whatever its kind, so the step is written as one
-// instruction stream for every lane, with no branches on the lane's role:
-// each lane reads its new column through a per-lane LDS pointer (D, C or A
-// of the step, or a resident identity block), every lane applies the
-// S_k update with its old column scaled by 1 (pivot lanes) or 0 (the rest),
-// and the outputs go out through per-lane destination offsets
-// u_i + s * v_i precomputed for the lane's role.  The D, A, C blocks of 4
-// steps at a time (2 for blocks larger than 10x10) are prefetched into
-// double-buffered LDS one chunk ahead;
-// their sources come from the per-element table fidx = {e, A source, C
-// source of the face before} (source = 2 * face + 1 for Of2, 2 * face for
-// Of1, -1 none: the zero block).
-// What the factor kernel needs of one plan: its line tables, its Jacobian
-// and its factor arrays.  One plan's record is a kernel argument; many plans
-// factored in one launch read theirs from device memory through a line table
-// {record, line}.
-struct FactorRec {
-    const int *linepoint, *linesize, *woff;
-    const int4 *fidx;
-    const double *D, *O1, *O2, *zero;   // zero: a block of zeros (the rhs padding)
-    void *W, *EF, *GB;
-};
-
-template <int NV, typename TF>
-__global__ __launch_bounds__(128) void ffactor_kernel(FactorRec one, int nline, const FactorRec *__restrict__ recs,
-                                                      const int2 *__restrict__ ltab) {
-    constexpr int BS = NV * NV;
-    constexpr int CH = NV <= 10 ? 4 : 2;        // steps per chunk (LDS: two waves, two buffers)
-    constexpr int NB = 3 * CH;                  // blocks per chunk: D_k, A_k, C_{k+1} of each step
-    constexpr int SB = 3 * BS;                  // doubles per step
-    constexpr int NLD = (CH * SB + 63) / 64;    // loads per lane per chunk
-    constexpr int SPW = 64 / NV, LW = SPW * NV;
-    using WL = tbs::WinLayout<NV, LW, TF>;
-    using WS = tbs::Soa64<BS, TF>;
-    static_assert(4 * NV <= 64, "lane roles need 4*NV lanes");
-    static_assert(BS <= 256, "entry index kept in 8 bits");
-    __shared__ double sblk_all[2][2][CH * SB];  // [wave][buffer]
-    __shared__ double sI[BS];                   // identity: the W lanes' starting column
-    __shared__ const double *sptr_all[2][2][NB];  // [wave][buffer]: sources of a chunk's blocks
-
-    if ((int)blockIdx.x >= nline) return;
-    int l = blockIdx.x;
-    FactorRec fr = one;
-    if (ltab) {
-        const int2 e = ltab[blockIdx.x];
-        fr = recs[__builtin_amdgcn_readfirstlane(e.x)];
-        l = __builtin_amdgcn_readfirstlane(e.y);
-    }
-    const int *__restrict__ linepoint = fr.linepoint;
-    const int *__restrict__ linesize = fr.linesize;
-    const int *__restrict__ woff = fr.woff;
-    const int4 *__restrict__ fidx = fr.fidx;
-    const double *__restrict__ jacD = fr.D;
-    const double *__restrict__ O1 = fr.O1;
-    const double *__restrict__ O2 = fr.O2;
-    const double *__restrict__ zero_blk = fr.zero;
-    TF *__restrict__ W = (TF *)fr.W;
-    TF *__restrict__ EF = (TF *)fr.EF;
-    TF *__restrict__ GB = (TF *)fr.GB;
-    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-    const bool bot = wave == 1;
-    const int lane = threadIdx.x & 63;
-    auto sblk = sblk_all[wave];
-    auto sptr = sptr_all[wave];
-    // Gamma''_{m+1} for step m, from the bottom wave's last step, in its
-    // block buffer (no longer read by then)
-    double *sG = &sblk_all[1][0][0];
-    // wave-uniform by construction: the chunk and step loops must be scalar
-    // loops (the pivot broadcasts read lanes that every step writes)
-    const int m0 = __builtin_amdgcn_readfirstlane(linepoint[l]);
-    const int n = __builtin_amdgcn_readfirstlane(linesize[l]);
-    const Split sp = split_line(n, SPW);
-    const int nsteps = bot ? sp.B : sp.m + 1;   // the top wave's last step is step m
-    // the wave's windows: top q + 1 (the last holds step m), bottom nb
-    const long wbase = ((long)__builtin_amdgcn_readfirstlane(woff[l]) + (bot ? sp.q + 1 : 0)) * WL::STRIDE;
-    // a block of 128 threads advances its two waves' LDS use apart: within a
-    // wave, LDS written by some lanes is read by others in program order
-    auto wsync = []() { __builtin_amdgcn_wave_barrier(); };
-
-    // ---- per-lane constants
-    const int grp = lane / NV;                  // role group 0..3 (4+: idle lanes)
-    const int j = lane - grp * NV;              // the lane's column
-    if (!bot)
-        for (int t = lane; t < BS; t += 64) sI[t] = (t % NV == t / NV) ? 1.0 : 0.0;
-    // chunk entry t = lane + 64 r: block (step * 3 + which) << 8 | entry
-    int code[NLD];
-#pragma unroll
-    for (int r = 0; r < NLD; ++r) {
-        const int t = lane + 64 * r;
-        const int st = t / SB, rem = t - st * SB;
-        const int which = rem / BS;
-        code[r] = t < CH * SB ? ((st * 3 + which) << 8) | (rem - which * BS) : -1;
-    }
-    // outputs: lanes of group 0/1 (X, parity-dependent) and 2 write window
-    // entries WL::index(j, st * NV + i) = u_i + st * v_i; group 3 writes W
-    // entries WS::index(i + NV j, lane of the element) = u_i + elane * v_i
-    int u[NV], v[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        if (grp == 3) {
-            u[i] = (int)WS::index(i + NV * j, 0);
-            v[i] = (int)(WS::index(i + NV * j, 1) - u[i]);
-        } else {
-            u[i] = (int)WL::index(j < NV ? j : 0, i);
-            v[i] = (int)(WL::index(j < NV ? j : 0, NV) - WL::index(j < NV ? j : 0, 0));
-        }
-    }
-    const double fP[2] = {grp == 0 ? 1.0 : 0.0, grp == 1 ? 1.0 : 0.0};   // pivot lanes at parity 0 / 1
-
-    // lane t < NB: block t of a chunk is (wave step t / 3, slot t % 3) and
-    // its source is a component of fidx ({element, A source, C source of the
-    // face before}): top wave, line step k = s: slots D_k, A_k, C_{k+1}
-    // (fidx[k].x, fidx[k].y, fidx[k+1].z); bottom wave, k = n-1-s: slots D_k,
-    // C_{k+1}, A_k (fidx[k].x, fidx[k+1].z, fidx[k].y); -1: none
-    const int lst = lane / 3, lwhich = lane - 3 * lst;
-    const int nextslot = bot ? 1 : 2;
-    auto load_idx = [&](int c) -> int {
-        const int s = c * CH + lst;
-        const int k = (bot ? n - 1 - s : s) + (lwhich == nextslot ? 1 : 0);
-        const int comp = lwhich == 0 ? 0 : (lwhich == nextslot ? 2 : 1);
-        int code_src = -1;
-        if (lane < NB && s < nsteps && k < n) code_src = reinterpret_cast<const int *>(fidx + m0 + k)[comp];
-        return code_src;
-    };
-    // (pointers are chosen by arithmetic on their values: a select between
-    // two captured pointers becomes a select of their addresses, which puts
-    // them in scratch memory)
-    const uintptr_t aD = (uintptr_t)jacD, aO1 = (uintptr_t)O1, aZ = (uintptr_t)zero_blk;
-    const uintptr_t dO = (uintptr_t)O2 - aO1;
-    auto set_ptrs = [&](int b, int cs) {
-        if (lane < NB) {
-            const uintptr_t face = aO1 + (uintptr_t)(cs & 1) * dO + (uintptr_t)((long)(cs >> 1) * BS * 8);
-            const uintptr_t elem = aD + (uintptr_t)((long)cs * BS * 8);
-            const uintptr_t p = lwhich == 0 ? elem : face;
-            sptr[b][lane] = (const double *)(cs >= 0 ? p : aZ);
-        }
-    };
-    // (the sources are global memory: loads through a global-space pointer,
-    // so they count only against vmcnt and the step's LDS waits do not wait
-    // for the prefetch)
-    using gptr = const __attribute__((address_space(1))) double *;
-    auto load_blocks = [&](int b, double (&R)[NLD]) {
-#pragma unroll
-        for (int r = 0; r < NLD; ++r) {
-            const int cd = code[r] < 0 ? 0 : code[r];
-            const gptr src = (gptr)sptr[b][cd >> 8];
-            R[r] = src[cd & 255];
-        }
-    };
-    auto store_blocks = [&](int b, const double (&R)[NLD]) {
-#pragma unroll
-        for (int r = 0; r < NLD; ++r) {
-            const int t = lane + 64 * r;
-            if (r < NLD - 1 || t < CH * SB) sblk[b][t] = R[r];
-        }
-    };
-
-    double R[NLD];
-    // prologue: chunk 0 in LDS, chunk 1 in flight, indices of chunk 2 in flight
-    set_ptrs(0, load_idx(0));
-    __syncthreads();                            // sI, and wave-local sptr
-    load_blocks(0, R);
-    store_blocks(0, R);
-    set_ptrs(1, load_idx(1));
-    wsync();
-    load_blocks(1, R);
-    int xnext = load_idx(2);
-
-    double col[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) col[i] = 0.0;
-
-    const uintptr_t aW = (uintptr_t)W, aEF = (uintptr_t)EF, dGB = (uintptr_t)GB - (uintptr_t)EF;
-    auto step = [&](auto par, int b, int s, int sw) {
-        constexpr int PAR = decltype(par)::value;
-        constexpr int P = PAR ? NV : 0;
-        const int k = bot ? n - 1 - sw : sw;    // line step of wave step sw
-        const double *Dk = &sblk[b][s * SB];
-        const double *Ak = Dk + BS;
-        // the lane's new column: pivot lanes D_k, X lanes C_{k+1}, A lanes
-        // A_k, W lanes the identity (idle lanes read the identity too)
-        const int xg = PAR ? 0 : 1;
-        const double *cp0 = grp == (PAR ? 1 : 0) ? Dk : (grp == xg ? Dk + 2 * BS : (grp == 2 ? Ak : sI));
-        const double *bcol = cp0 + (grp < 4 ? NV * j : 0);
-        double vold[NV];
-#pragma unroll
-        for (int m = 0; m < NV; ++m) vold[m] = col[m] * fP[PAR];
-        // S_k = D_k - A_k Gamma_{k-1} on the pivot lanes (A_k uniform); the
-        // other lanes subtract exact zeros
-#pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            double tot = 0.0;
-#pragma unroll
-            for (int m = 0; m < NV; ++m) tot += Ak[i + NV * m] * vold[m];
-            col[i] = bcol[i] - tot;
-        }
-        // Gauss-Jordan on the pivot columns
-#pragma unroll
-        for (int p = 0; p < NV; ++p) {
-            double cpv[NV];
-#pragma unroll
-            for (int i = 0; i < NV; ++i) cpv[i] = tbs::rdlane(col[i], P + p);
-            const double rp = recip(cpv[p]);
-            const double x = col[p] * rp;
-#pragma unroll
-            for (int i = 0; i < NV; ++i)
-                if (i != p) col[i] -= cpv[i] * x;
-            col[p] = x;
-        }
-        // outputs: Gamma_k (X lanes), E_k (A lanes), W_k, at the wave's
-        // window position of the step (the bottom wave's windows start with
-        // pad empty steps)
-        const int pos = bot ? sw + sp.pad : sw;
-        const int w = pos / SPW, st = pos - w * SPW;
-        const int mm = m0 + k;
-        const bool isW = grp == 3;
-        // every window entry of a real step is written: E_0, E''_{n-1} and
-        // Gamma_{n-1} (no neighbour) as zeros, so the factor arrays need no
-        // zero fill
-        const bool act = grp == xg || grp == 2 || isW;
-        const bool zero = bot ? (grp == 2 && sw == 0) : ((grp == xg && k == n - 1) || (grp == 2 && k == 0));
-        const uintptr_t wdst = aW + (uintptr_t)((size_t)(mm / kGroup) * BS * kGroup * sizeof(TF));
-        const uintptr_t fdst = aEF + (uintptr_t)(grp == xg) * dGB +
-                               (uintptr_t)((wbase + (long)w * WL::STRIDE) * (long)sizeof(TF));
-        using gdst = __attribute__((address_space(1))) TF *;
-        const gdst dst = (gdst)(TF *)(isW ? wdst : fdst);
-        const int sc = isW ? mm % kGroup : st;
-        if (act) {
-#pragma unroll
-            for (int i = 0; i < NV; ++i) dst[u[i] + sc * v[i]] = zero ? TF(0) : (TF)col[i];
-        }
-    };
-
-    // one uniform loop over step pairs (CH is even); at the first step of
-    // chunk c > 0: chunk c into LDS, chunk c+1's blocks and chunk c+2's
-    // indices into flight
-    // step m (top wave): wait for the bottom wave's last Gamma'' (the one
-    // block-wide barrier of the two waves after the start), then D_m -=
-    // C_{m+1} Gamma''_{m+1} in the chunk's copy of D_m, so that the plain
-    // step gives S_m = D_m - C_{m+1} Gamma''_{m+1} - A_m Gamma_{m-1}
-    auto middle = [&](int b, int s) {
-        __syncthreads();
-        double *Dm = &sblk[b][s * SB];
-        const double *Cm = Dm + 2 * BS;
-        for (int t = lane; t < BS; t += 64) {
-            const int i = t % NV, jj = t / NV;
-            double tot = 0.0;
-#pragma unroll
-            for (int mm = 0; mm < NV; ++mm) tot += Cm[i + NV * mm] * sG[NV * jj + mm];
-            Dm[t] -= tot;
-        }
-        wsync();
-    };
-    for (int k = 0; k < nsteps; k += 2) {
-        const int c = k / CH, s = k - c * CH;
-        if (s == 0 && c > 0) {
-            wsync();                                    // LDS of chunk c-1 read
-            store_blocks(c & 1, R);                     // chunk c
-            set_ptrs((c + 1) & 1, xnext);               // sources of chunk c+1
-            wsync();
-            load_blocks((c + 1) & 1, R);
-            xnext = load_idx(c + 2);
-        }
-        if (!bot && k == sp.m) middle(c & 1, s);
-        step(std::integral_constant<int, 0>(), c & 1, s, k);
-        if (k + 1 < nsteps) {
-            if (!bot && k + 1 == sp.m) middle(c & 1, s + 1);
-            step(std::integral_constant<int, 1>(), c & 1, s + 1, k + 1);
-        }
-    }
-    if (bot) {
-        // Gamma''_{m+1}: the X lanes of the last step (group 1 after an even
-        // step, 0 after an odd one); zeros when the bottom side is empty
-        wsync();
-        const int xl = sp.B == 0 ? -1 : (((sp.B - 1) & 1) ? 0 : 1);
-        if (xl < 0) {
-            for (int t = lane; t < BS; t += 64) sG[t] = 0.0;
-        } else if (grp == xl) {
-#pragma unroll
-            for (int i = 0; i < NV; ++i) sG[NV * j + i] = col[i];
-        }
-        __syncthreads();
-    }
-    // empty steps of the wave's windows (top: window q after step m; bottom:
-    // the first pad steps of window 0): E and Gamma zero (the Thomas sweep
-    // runs whole windows and needs finite factors there); and each window's
-    // alignment padding (never read as a factor; written so the arrays hold
-    // no stale memory)
-    using gdst = __attribute__((address_space(1))) TF *;
-    const int e_lo = bot ? 0 : 1, e_hi = bot ? sp.pad : SPW;
-    const int wlast = bot ? 0 : sp.q;
-    for (int st = e_lo; st < e_hi; ++st) {
-        if (grp == 2) {
-            const uintptr_t base = (uintptr_t)((wbase + (long)wlast * WL::STRIDE) * (long)sizeof(TF));
-            const gdst e = (gdst)(TF *)(aEF + base), g = (gdst)(TF *)(aEF + dGB + base);
-#pragma unroll
-            for (int i = 0; i < NV; ++i) {
-                e[u[i] + st * v[i]] = TF(0);
-                g[u[i] + st * v[i]] = TF(0);
-            }
-        }
-    }
-    if (WL::STRIDE > WL::ELEMS && grp == 2 && j < WL::STRIDE - WL::ELEMS) {
-        const int nwin = bot ? sp.nb : sp.q + 1;
-        for (int w = 0; w < nwin; ++w) {
-            const long at = wbase + (long)w * WL::STRIDE + WL::ELEMS + j;
-            ((gdst)(TF *)aEF)[at] = TF(0);
-            ((gdst)(TF *)(aEF + dGB))[at] = TF(0);
-        }
-    }
-}
-
-// ------------------------------------------------------- plan tables --
-// The derived tables of a plan, built on the device from the mesh arrays
-// (uploaded as they are): per line element the factor sources fidx and the
-// line index; per group lane the residual block sources src and neighbour
-// indices nbr; per group slot the neighbour map slotc.
-
-// One block per line.  fidx = {e, A source, C source}: A_k = Of2, C_k = Of1
-// at face f_k when element k is the face's first element, else the reverse
-// (src/okl/lineLU_v2.okl:116-117); source = 2f + 1 for Of2, 2f for Of1.
-__global__ void tab_lines_kernel(const int *__restrict__ linepoint, const int *__restrict__ linesize,
-                                 const int *__restrict__ woff, const int *__restrict__ lines,
-                                 const int *__restrict__ lineface, const int *__restrict__ fc,
-                                 int *__restrict__ mline, int4 *__restrict__ fidx, int4 *__restrict__ ltab) {
-    const int l = blockIdx.x;
-    const int m0 = linepoint[l], n = linesize[l];
-    if (threadIdx.x == 0) ltab[l] = make_int4(0, m0, n, woff[l]);
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        const int mm = m0 + k, e = lines[mm];
-        int a = -1, c = -1;
-        if (k >= 1) {
-            const int f = lineface[mm];
-            const bool first = e == fc[2 * f];
-            a = 2 * f + (first ? 1 : 0);
-            c = 2 * f + (first ? 0 : 1);
-        }
-        mline[mm] = l;
-        fidx[mm] = make_int4(e, a, c, 0);
-    }
-}
-
-// inv[lines[i]] = i: internal position of every element.
-__global__ void tab_inv_kernel(int nle, const int *__restrict__ lines, int *__restrict__ inv) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nle; i += gridDim.x * blockDim.x) inv[lines[i]] = i;
-}
-
-// One thread per group lane mm < G: src row [S] = {0 none, 1 D, 2 Of1, 3 Of2;
-// index} (slot 0 = D, slot 1 + s = the element's s-th face in ef order: Of2
-// when the element is the face's first element), nbr[g][s][lane] = internal
-// index of the neighbour across that face or -1; lanes past the last
-// element stay empty.
-__global__ void tab_src_kernel(long G, int nle, int S, int Fm, const int *__restrict__ lines,
-                               const int *__restrict__ epoint, const int *__restrict__ ef,
-                               const int *__restrict__ fc, const int *__restrict__ inv, int2 *__restrict__ src,
-                               int *__restrict__ nbr) {
-    for (long mm = (long)blockIdx.x * blockDim.x + threadIdx.x; mm < G; mm += (long)gridDim.x * blockDim.x) {
-        const long g = mm / kGroup;
-        const int lane = (int)(mm % kGroup);
-        int2 *row = src + mm * S;
-        for (int q = 0; q < S; ++q) row[q] = make_int2(0, 0);
-        for (int s2 = 0; s2 < Fm; ++s2) nbr[(g * Fm + s2) * kGroup + lane] = -1;
-        if (mm >= nle) continue;
-        const int e = lines[mm];
-        row[0] = make_int2(1, e);
-        int s2 = 0;
-        for (int k2 = epoint[e]; k2 < epoint[e + 1]; ++k2, ++s2) {
-            const int f = ef[k2];
-            if (f < 0) continue;
-            const int e1 = fc[2 * f], e2 = fc[2 * f + 1];
-            row[1 + s2] = make_int2(e == e1 ? 3 : 2, f);
-            nbr[(g * Fm + s2) * kGroup + lane] = inv[e == e1 ? e2 : e1];
-        }
-    }
-}
-
-// One thread per (group, slot): the slot's neighbour map is the shift c when
-// every lane with a face there has nbr = m + c, kSlotGather when they differ,
-// kSlotNone when no lane has the face.  Also the group table.
-__global__ void tab_slot_kernel(int ngroups, int F, int Fm, int nle, const int *__restrict__ nbr,
-                                int *__restrict__ slotc, int2 *__restrict__ gtab) {
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < (long)ngroups * Fm;
-         t += (long)gridDim.x * blockDim.x) {
-        const int g = (int)(t / Fm), f = (int)(t % Fm);
-        int c = kSlotNone;
-        for (int lane = 0; lane < kGroup && f < F; ++lane) {
-            const int mm = g * kGroup + lane;
-            const int nb = nbr[((long)g * Fm + f) * kGroup + lane];
-            if (mm >= nle || nb < 0) continue;
-            if (c == kSlotNone) c = nb - mm;
-            else if (c != nb - mm) { c = kSlotGather; break; }
-        }
-        slotc[t] = c;
-        if (f == 0) gtab[g] = make_int2(0, g);
-    }
-}
-
-// Vector gathers between reference order (element id) and internal order.
-__global__ void gather_vec_kernel(long total, int nv, const int *__restrict__ lines,
-                                  const double *__restrict__ src, double *__restrict__ dst) {
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (long)gridDim.x * blockDim.x) {
-        const long m = t / nv;
-        const int i = (int)(t % nv);
-        dst[t] = src ? src[(size_t)nv * lines[m] + i] : 0.0;
-    }
-}
-__global__ void scatter_vec_kernel(long total, int nv, const int *__restrict__ lines,
-                                   const double *__restrict__ src, double *__restrict__ dst) {
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (long)gridDim.x * blockDim.x) {
-        const long m = t / nv;
-        const int i = (int)(t % nv);
-        dst[(size_t)nv * lines[m] + i] = src[t];
-    }
-}
-__global__ void zero_kernel(long n, double *__restrict__ p) {
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
-        p[t] = 0.0;
-}
-
-unsigned gs_grid(long n) {
-    long g = (n + 255) / 256;
-    return (unsigned)std::max(1L, std::min(g, 65536L));
-}
-
-}  // namespace
+// lij_fast.hip -- the fused LIJ sweep path ("plan" / "batch"): host side and
+// the kernels' instances for block sizes 2..16 (the kernels themselves are in
+// lij_kernels.hpp; sizes 17..32 are instantiated in lij_wide_lo.hip and
+// lij_wide_hi.hip).
+#include "lij_kernels.hpp"
+#include "lij_wide.hpp"
 
 // ======================================================================
 // Host side
@@ -1435,65 +278,6 @@ void free_plan(tbs_plan *p) {
     delete p;
 }
 
-// Launch on st; with a start event, through hipExtLaunchKernelGGL, which
-// records the two events at the kernel's own start and end (timing of exactly
-// this kernel, without the dispatch or the completion signal around it).
-template <typename K, typename... Args>
-void launch_k(K kernel, dim3 grid, dim3 block, hipStream_t st, hipEvent_t e0, hipEvent_t e1, Args... args) {
-    if (e0) hipExtLaunchKernelGGL(kernel, grid, block, 0, st, e0, e1, 0, args...);
-    else hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
-}
-
-template <int NV, typename TF>
-int launch_residual(hipStream_t st, const void *sys, const int2 *gtab, int ng, int want_R, int force,
-                    hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
-    if (ng == 0) return TBS_OK;
-    const int nblk = (ng + 3) / 4;
-    launch_k(fres_kernel<NV, TF>, dim3(8 * ((nblk + 7) / 8)), dim3(256), st, e0, e1,
-             (const SysDev<TF> *)sys, gtab, ng, want_R, force);
-    TBS_LAUNCHED("fres_kernel");
-    return TBS_OK;
-}
-
-// Thomas variants the autotuner picks from: {prefetch windows PF, ring
-// cycles K per loop iteration}; index 0 is the default.
-struct TV { int pf, k; };
-template <int NV, typename TF> struct ThomasV { static constexpr int n = 1; static constexpr TV v[1] = {{2, 2}}; };
-// (LDS per two-wave workgroup stays <= 32 KiB so five lines fit a CU.)
-template <> struct ThomasV<9, double> {
-    static constexpr int n = 4;
-    static constexpr TV v[4] = {{2, 2}, {3, 1}, {1, 3}, {1, 2}};
-};
-template <> struct ThomasV<9, float> {
-    static constexpr int n = 4;
-    static constexpr TV v[4] = {{3, 2}, {2, 2}, {5, 1}, {3, 1}};
-};
-
-template <int NV, typename TF, int PF, int K>
-int launch_thomas_pf(hipStream_t st, const void *sys, const int4 *ltab, int nl, hipEvent_t e0, hipEvent_t e1) {
-    launch_k(fthomas_kernel<NV, TF, PF, K>, dim3(nl), dim3(128), st, e0, e1, (const SysDev<TF> *)sys, ltab, nl);
-    TBS_LAUNCHED("fthomas_kernel");
-    return TBS_OK;
-}
-
-template <int NV, typename TF>
-int launch_thomas(hipStream_t st, const void *sys, const int4 *ltab, int nl, int variant, hipEvent_t e0 = nullptr,
-                  hipEvent_t e1 = nullptr) {
-    if (nl == 0) return TBS_OK;
-    using P = ThomasV<NV, TF>;
-    if (variant < 0 || variant >= P::n) variant = 0;
-    constexpr TV a = P::v[0];
-    constexpr TV b = P::v[P::n > 1 ? 1 : 0];
-    constexpr TV c = P::v[P::n > 2 ? 2 : 0];
-    constexpr TV d = P::v[P::n > 3 ? 3 : 0];
-    switch (variant) {
-        case 1: return launch_thomas_pf<NV, TF, b.pf, b.k>(st, sys, ltab, nl, e0, e1);
-        case 2: return launch_thomas_pf<NV, TF, c.pf, c.k>(st, sys, ltab, nl, e0, e1);
-        case 3: return launch_thomas_pf<NV, TF, d.pf, d.k>(st, sys, ltab, nl, e0, e1);
-        default: return launch_thomas_pf<NV, TF, a.pf, a.k>(st, sys, ltab, nl, e0, e1);
-    }
-}
-
 int thomas_variants(int nv, int prec) {
     if (nv == 9) return prec == TBS_FP32 ? ThomasV<9, float>::n : ThomasV<9, double>::n;
     return 1;
@@ -1507,17 +291,14 @@ int thomas_pf_of(int nv, int prec, int variant) {
     return 100 * t.pf + t.k;
 }
 
-template <typename TF>
-int launch_final(hipStream_t st, const void *sys, int nsys, int max_sweeps, double rtol, int force,
-                 hipEvent_t e0, hipEvent_t e1) {
-    launch_k(ffinal_kernel<TF>, dim3(nsys), dim3(kFinalThreads), st, e0, e1, (const SysDev<TF> *)sys, nsys,
-             max_sweeps, rtol, force);
-    TBS_LAUNCHED("ffinal_kernel");
-    return TBS_OK;
-}
-
-// Dispatch on (nvar, precision) for the fused kernels.
+// Dispatch on (nvar, precision) for the fused kernels: 2..16 instantiated
+// here, 17..24 and 25..32 in lij_wide_lo.hip / lij_wide_hi.hip.
 #define TBS_PLAN_NVARS(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
+#define TBS_WIDE_CALL(nv, fn, ...)                                    \
+    do {                                                              \
+        if ((nv) >= 17 && (nv) <= 24) return tbs::wide_lo::fn(__VA_ARGS__); \
+        if ((nv) >= 25 && (nv) <= 32) return tbs::wide_hi::fn(__VA_ARGS__); \
+    } while (0)
 
 FactorRec factor_rec(const tbs_plan *p, const double *D, const double *O1, const double *O2) {
     FactorRec r{};
@@ -1535,14 +316,21 @@ FactorRec factor_rec(const tbs_plan *p, const double *D, const double *O1, const
     return r;
 }
 
+int launch_factor_wide(int nv, int prec, hipStream_t st, const FactorRec &one, int nline, const FactorRec *recs,
+                       const int2 *ltab) {
+    TBS_WIDE_CALL(nv, factor, nv, prec, st, one, nline, recs, ltab);
+    tbs::set_error("nvar %d not built for the fused path", nv);
+    return TBS_ERR_UNSUPPORTED;
+}
+
 int launch_factor(int nv, int prec, hipStream_t st, const FactorRec &one, int nline, const FactorRec *recs,
                   const int2 *ltab) {
     if (nline == 0) return TBS_OK;
+    if (nv > 16) TBS_TRY(launch_factor_wide(nv, prec, st, one, nline, recs, ltab));
     switch (nv) {
 #define X(v) case v: \
-        if (prec == TBS_FP32) hipLaunchKernelGGL((ffactor_kernel<v, float>), dim3(nline), dim3(128), 0, st, one, \
-                                                 nline, recs, ltab); \
-        else hipLaunchKernelGGL((ffactor_kernel<v, double>), dim3(nline), dim3(128), 0, st, one, nline, recs, ltab); \
+        if (prec == TBS_FP32) launch_factor_nv<v, float>(st, one, nline, recs, ltab); \
+        else launch_factor_nv<v, double>(st, one, nline, recs, ltab); \
         break;
         TBS_PLAN_NVARS(X)
 #undef X
@@ -1554,16 +342,21 @@ int launch_factor(int nv, int prec, hipStream_t st, const FactorRec &one, int nl
 int launch_pack(const tbs_plan *p, hipStream_t pst, const double *jacD, const double *jacO1, const double *jacO2) {
     const int nv = p->nv, BS = nv * nv;
     const long nws = (long)p->ngroups * p->S;
-    if (2 * nws < INT_MAX) {
+    if (2 * nws < INT_MAX && nv <= 16) {
         switch (nv) {
-#define X(v) case v: hipLaunchKernelGGL(pack_lds_kernel<v>, dim3((unsigned)(2 * nws)), dim3(128), 0, pst, \
-                   p->S, p->d_src, jacD, jacO1, jacO2, p->d_blk); break;
+#define X(v) case v: launch_pack_lds<v>(pst, nws, p->S, p->d_src, jacD, jacO1, jacO2, p->d_blk); break;
             TBS_PLAN_NVARS(X)
 #undef X
         }
         TBS_LAUNCHED("pack_lds_kernel");
     } else {
         const long total = (long)p->ngroups * kGroup * p->S * BS;
+        if (nv > 16) {
+            int rc = TBS_ERR_UNSUPPORTED;
+            if (nv <= 24) rc = tbs::wide_lo::pack(nv, pst, total, p->S, p->d_src, jacD, jacO1, jacO2, p->d_blk);
+            else rc = tbs::wide_hi::pack(nv, pst, total, p->S, p->d_src, jacD, jacO1, jacO2, p->d_blk);
+            TBS_TRY(rc);
+        }
         switch (nv) {
 #define X(v) case v: hipLaunchKernelGGL(pack_blocks_kernel<v>, dim3(gs_grid(total)), dim3(256), 0, pst, \
                    total, p->S, p->d_src, jacD, jacO1, jacO2, p->d_blk); break;
@@ -1583,6 +376,7 @@ int k_residual(int nv, int prec, hipStream_t st, const void *sys, const int2 *gt
         TBS_PLAN_NVARS(X)
 #undef X
     }
+    TBS_WIDE_CALL(nv, residual, nv, prec, st, sys, gtab, ng, want_R, force, e0, e1);
     tbs::set_error("nvar %d not built for the fused path", nv);
     return TBS_ERR_UNSUPPORTED;
 }
@@ -1594,6 +388,7 @@ int k_thomas(int nv, int prec, hipStream_t st, const void *sys, const int4 *ltab
         TBS_PLAN_NVARS(X)
 #undef X
     }
+    TBS_WIDE_CALL(nv, thomas, nv, prec, st, sys, ltab, nl, variant, e0, e1);
     tbs::set_error("nvar %d not built for the fused path", nv);
     return TBS_ERR_UNSUPPORTED;
 }
@@ -1613,14 +408,7 @@ int k_reset(int prec, hipStream_t st, const void *sys, int nsys) {
     return TBS_OK;
 }
 
-bool plan_nvar_ok(int nv) {
-    switch (nv) {
-#define X(v) case v: return true;
-        TBS_PLAN_NVARS(X)
-#undef X
-    }
-    return false;
-}
+bool plan_nvar_ok(int nv) { return nv >= 2 && nv <= 32; }
 
 // One sweep: Thomas (+U update), residual, norm/convergence.
 int batch_sweeps(tbs_batch *b, hipStream_t st, int nsweeps, int max_sweeps, double rtol, bool split);
@@ -1710,9 +498,8 @@ extern "C" {
 
 int tbs_plan_supported_nvar(int *out, int cap) {
     int n = 0;
-#define X(v) if (out && n < cap) out[n] = v; ++n;
-    TBS_PLAN_NVARS(X)
-#undef X
+    for (int v = 2; v <= 32; ++v, ++n)
+        if (out && n < cap) out[n] = v;
     return n;
 }
 
