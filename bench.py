@@ -308,7 +308,7 @@ def bench_serving(tbs, ctx, args, nreq=160, seed=7, oracle=None):
     def run(sizes, arrivals, keep=()):
         srv = tbs.Server(ctx, precision=tbs.TBS_FP32, rtol=1e-10, max_sweeps=100, sweeps_per_step=2)
         t0 = srv.info()["now"]
-        rid_of, lat, nsw, status = {}, [], [], []
+        rid_of, lat, nsw, status, phases, order = {}, [], [], [], [], []
         kept = {}
         k = 0
         while len(lat) < len(sizes):
@@ -322,6 +322,11 @@ def bench_serving(tbs, ctx, args, nreq=160, seed=7, oracle=None):
                 r = srv.result(rid, want_U=rid_of[rid] in keep)
                 if rid_of[rid] in keep:
                     kept[rid_of[rid]] = r["U"]
+                order.append(rid_of[rid])
+                tl = srv.timeline(rid) - t0
+                # arrival -> submitted -> admitted -> set-up done -> converged -> result
+                phases.append([tl[0] - arrivals[rid_of[rid]], tl[1] - tl[0], tl[2] - tl[1], tl[3] - tl[2],
+                               tl[5] - tl[3]])
                 lat.append(r["t_done"] - t0 - arrivals[rid_of[rid]])
                 nsw.append(r["nsweeps"])
                 status.append(r["status"])
@@ -331,6 +336,8 @@ def bench_serving(tbs, ctx, args, nreq=160, seed=7, oracle=None):
         srv.close()
         run.stats = stats
         run.kept = kept
+        run.phases = np.array(phases)
+        run.order = order
         return np.array(lat), wall, nsw, status
 
     sizes = draw(nreq)
@@ -356,11 +363,18 @@ def bench_serving(tbs, ctx, args, nreq=160, seed=7, oracle=None):
         sizes = draw(nreq)
         arr = np.cumsum(rng.exponential(1.0 / (frac * sat), nreq))
         lat, wall, nsw, st = run(sizes, arr)
+        ph = run.phases
+        tail = lat >= np.percentile(lat, 98)
+        names = ("arrival_to_submit", "queued", "setup", "in_batch", "result_out")
         out["loads"][f"{frac:g}"] = {"offered_req_per_s": frac * sat, "achieved_req_per_s": nreq / wall,
                                      "p50_ms": float(np.percentile(lat, 50) * 1e3),
                                      "p99_ms": float(np.percentile(lat, 99) * 1e3),
                                      "all_converged": bool(all(s == 1 for s in st)),
-                                     "host_phase_s": run.stats}
+                                     "host_phase_s": run.stats,
+                                     "median_phase_ms": {k: float(np.median(ph[:, i]) * 1e3) for i, k in enumerate(names)},
+                                     "tail2pct_phase_ms": {k: float(np.mean(ph[tail, i]) * 1e3) for i, k in enumerate(names)},
+                                     "tail2pct_elements": [int(sizes[run.order[i]][0] * sizes[run.order[i]][1])
+                                                           for i in np.argsort(lat)[-max(1, int(tail.sum())):]]}
     return out
 
 

@@ -310,7 +310,9 @@ typedef struct tbs_server_request {
     uint64_t seed;
     double shift;        /* diagonal shift of the generator                  */
 } tbs_server_request;
-/* hbm_budget 0 = 80% of the device's free memory at creation. */
+/* hbm_budget 0 = 80% of the device's free memory at creation.  The server's
+ * HBM arena is allocated at the full budget here (in smaller pieces later
+ * only if the device refuses that), so admissions allocate nothing. */
 int tbs_server_create(tbs_ctx *ctx, size_t hbm_budget, int precision, double rtol,
                       int max_sweeps, int sweeps_per_step, tbs_server **out);
 int tbs_server_destroy(tbs_server *srv);
@@ -321,6 +323,21 @@ int tbs_server_destroy(tbs_server *srv);
 int tbs_server_footprint(tbs_server *srv, const tbs_server_request *req, size_t *bytes);
 /* TBS_ERR_NOMEM if the request alone exceeds the budget. */
 int tbs_server_submit(tbs_server *srv, const tbs_server_request *req, long *id);
+/* A caller's system: the mesh as the host arrays of the Mesh class
+ * (src/Mesh.cxx:11-73, checked as tbs_plan_create checks it; block size =
+ * the server context's), the Jacobian blocks jacD/jacO1/jacO2 and rhs in the
+ * reference layout (Jacobian::jacD/jacO1/jacO2/rhs, src/Jacobian.cxx:11-74).
+ * where = TBS_HOST: the values are copied during the call.  where =
+ * TBS_DEVICE: device pointers, read on the device when the request is
+ * admitted; they must stay valid until tbs_server_poll reports the request.
+ * The result U is in element order, as for tbs_server_submit's requests. */
+#define TBS_HOST   0
+#define TBS_DEVICE 1
+int tbs_server_submit_system(tbs_server *srv, const tbs_mesh_host *mesh, const double *jacD,
+                             const double *jacO1, const double *jacO2, const double *rhs, int where,
+                             long *id);
+/* HBM bytes such a request holds when admitted (as tbs_server_footprint). */
+int tbs_server_footprint_system(tbs_server *srv, const tbs_mesh_host *mesh, size_t *bytes);
 /* One scheduling step: collect the running batch's states, let finished
  * systems leave and set-up-complete admissions join, launch the next
  * sweeps, start copying finished results out (a request is reported done
@@ -336,6 +353,12 @@ int tbs_server_poll(tbs_server *srv, long *ids, int cap, int *n);
 int tbs_server_result(tbs_server *srv, long id, double *U, size_t ucap, double *hist,
                       int hcap, int *nsweeps, int *status, double *times);
 int tbs_server_release(tbs_server *srv, long id);
+/* Where a finished request's time went (seconds on the server's clock; up to
+ * n values): t[0] submitted, [1] admitted (its set-up enqueued), [2] set-up
+ * seen complete by a step (it joins the batch), [3] convergence seen by a
+ * step, [4] result copies enqueued, [5] copies seen complete (the result is
+ * ready; times[2] of tbs_server_result). */
+int tbs_server_timeline(tbs_server *srv, long id, double *t, int n);
 int tbs_server_info(tbs_server *srv, size_t *budget, size_t *used, double *now);
 /* Host time per engine phase, cumulative seconds: [0] waiting for the batch's
  * sweeps, [1] membership update, [2] sweep launch, [3] retirement, [4]

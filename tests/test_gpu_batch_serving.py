@@ -225,3 +225,66 @@ def test_plans_factor_rejects_mixed_plans(tbs, ctx):
         tbs.factor_plans(ctx, [a, b], [D, D], [O1, O1], [O2, O2])
     a.close()
     b.close()
+
+
+def test_server_serves_callers_systems(tbs, ctx, oracle, tmp_path):
+    """A caller's system -- read back from the reference's binary files that
+    triblock.exe --write-data writes, through the host API's readers -- served
+    from host arrays and again from device arrays, next to a ragged system and
+    seeded synthetic requests: every result equals the oracle's (FP64 server:
+    history within 1e-12 * max(h, 1e-4), U within 1e-12) and the same system
+    solved alone, bitwise."""
+    import os
+    import subprocess
+    from conftest import ROOT
+    print("case server caller systems", flush=True)
+    exe = os.path.join(ROOT, "triblocksolve_amd", "triblock.exe")
+    p = subprocess.run([exe, "1", "0", "9", "--synthetic", "30x200", "--seed", "9", "--shift", "12",
+                        "--iters", "1", "--write-data"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    m_file = tbs.read_mesh(os.path.join(tmp_path, "gpuline.mesh.data.bin"))
+    nvar, nelem, nif, j_file = tbs.read_jacobian(os.path.join(tmp_path, "gpuline.jacobian.data.bin"))
+    assert (nvar, nelem, nif) == (9, m_file.nelem, m_file.nintface)
+    m_rag = tbs.ragged_mesh([5, 300, 17, 1, 64, 130])
+    j_rag = tbs.generate_host(77, 9, m_rag.nelem, m_rag.nintface, 12.0)
+    srv = tbs.Server(ctx, precision=tbs.TBS_FP64, rtol=1e-10, max_sweeps=60, sweeps_per_step=2)
+    assert srv.footprint_system(m_file) > tbs.Plan.footprint(m_file, tbs.TBS_FP64)
+    ids = {}
+    ids[srv.submit_system(m_file, j_file["jacD"], j_file["jacO1"], j_file["jacO2"], j_file["rhs"])] = \
+        ("file_host", m_file, j_file)
+    ids[srv.submit(20, 96, 31, 12.0)] = ("synthetic", None, None)
+    devj = [dev(j_file[k]) for k in ("jacD", "jacO1", "jacO2", "rhs")]
+    ids[srv.submit_system(m_file, *devj)] = ("file_device", m_file, j_file)
+    ids[srv.submit_system(m_rag, j_rag["jacD"], j_rag["jacO1"], j_rag["jacO2"], j_rag["rhs"])] = \
+        ("ragged_host", m_rag, j_rag)
+    ids[srv.submit(3, 700, 32, 12.0)] = ("synthetic", None, None)
+    # a mesh the fused path cannot take is refused at submit
+    bad = tbs.ragged_mesh([4, 4])
+    bad.lineface[2] = 0                   # not the face joining the line's elements 1 and 2
+    with pytest.raises(tbs.TbsError):
+        srv.submit_system(bad, *[np.zeros(81 * 8)] * 3, np.zeros(72))
+    done = {}
+    for _ in range(5000):
+        srv.step()
+        for rid in srv.poll():
+            done[rid] = (srv.result(rid), srv.timeline(rid))
+            srv.release(rid)
+        if len(done) == len(ids):
+            break
+    srv.close()
+    assert len(done) == len(ids)
+    results = {}
+    for rid, (name, m, j) in ids.items():
+        r, tl = done[rid]
+        assert r["status"] == 1 and r["hist"][-1] <= 1e-10 * r["hist"][0], name
+        assert np.all(np.diff(tl) >= 0), (name, tl)
+        if m is None:
+            continue
+        ref = oracle.lij_run(m, j, r["nsweeps"])
+        ok, worst = cf.history_close(r["hist"], ref["hist"])
+        assert ok, (name, worst)
+        assert np.linalg.norm(r["U"] - ref["U"]) <= 1e-12 * np.linalg.norm(ref["U"]), name
+        h, n, U = _solve_alone(tbs, ctx, m, j, tbs.TBS_FP64, 60, 1e-10)
+        assert r["nsweeps"] == n and np.array_equal(r["hist"], h) and np.array_equal(r["U"], U), name
+        results[name] = r
+    assert np.array_equal(results["file_host"]["U"], results["file_device"]["U"])

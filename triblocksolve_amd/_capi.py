@@ -21,6 +21,8 @@ TBS_FP32 = 1
 TBS_STEP_THOMAS = 0
 TBS_STEP_RESIDUAL = 1
 TBS_STEP_NORM = 2
+TBS_HOST = 0
+TBS_DEVICE = 1
 
 
 class TbsError(RuntimeError):
@@ -121,6 +123,9 @@ SIGNATURES = {
     "tbs_server_destroy": [P],
     "tbs_server_footprint": [P, C.POINTER(ServerRequest), C.POINTER(C.c_size_t)],
     "tbs_server_submit": [P, C.POINTER(ServerRequest), C.POINTER(C.c_long)],
+    "tbs_server_submit_system": [P, C.POINTER(MeshHost), P, P, P, P, I, C.POINTER(C.c_long)],
+    "tbs_server_footprint_system": [P, C.POINTER(MeshHost), C.POINTER(C.c_size_t)],
+    "tbs_server_timeline": [P, C.c_long, P, I],
     "tbs_server_step": [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "tbs_server_poll": [P, P, I, C.POINTER(C.c_int)],
     "tbs_server_result": [P, C.c_long, P, C.c_size_t, P, I, C.POINTER(C.c_int), C.POINTER(C.c_int), P],

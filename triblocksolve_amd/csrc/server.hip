@@ -1,20 +1,23 @@
 // server.hip -- continuous-batching LIJ solve service on one GPU.
 //
-// Requests (seeded structured systems, FP32-factor or FP64 plans) wait in a
-// FIFO queue.  The server owns an HBM arena, grown in segments of at least
-// kSegment bytes up to the budget; every admitted request gets one slab of
-// it (first fit), from which its plan, its rhs/U and its admission
-// temporaries are carved, so steady-state serving makes no hipMalloc/hipFree
-// calls.  Each engine step
+// Requests (a caller's system -- mesh, Jacobian blocks and rhs in the
+// reference's layout, from host or device memory -- or a seeded structured
+// system the server generates; FP32-factor or FP64 plans) wait in a FIFO
+// queue.  The server owns an HBM arena, sized to the budget when the server
+// is created (grown in segments of at least kSegment bytes only if that
+// first allocation was refused); every admitted request gets one slab of it
+// (first fit), from which its plan, its rhs/U and its admission temporaries
+// are carved, so serving makes no hipMalloc/hipFree calls.  Each engine step
 //   1. reads the states of the running batch (waits for its last sweeps),
 //   2. drops finished systems from the batch and adds admitted ones whose
 //      set-up has completed (descriptor tables rewritten in place),
 //   3. enqueues K sweeps of the batch,
 //   4. while those run, retires the finished systems on a third stream (U
 //      scattered to element order and copied to the host, slab returned), and
-//   5. admits queued requests that fit: mesh tables, Jacobian generation,
-//      factorisation and the initial residual are enqueued on the admission
-//      stream and overlap the batch's sweeps on the device.
+//   5. admits queued requests that fit: mesh tables, the Jacobian (copied
+//      from the caller's arrays or generated), factorisation and the initial
+//      residual are enqueued on the admission stream and overlap the batch's
+//      sweeps on the device.
 // A request's arithmetic depends on its own data alone (per-line Thomas,
 // per-element residual, per-system fixed-order norm), so its result is
 // bitwise the same whatever it shares the batch with.
@@ -80,6 +83,12 @@ class Arena {
 struct Request {
     long id = 0;
     tbs_server_request spec{};
+    // a caller's system: its Jacobian and rhs, host copies taken at submit
+    // (where == TBS_HOST) or the caller's device arrays, read at admission
+    bool caller = false;
+    int where = TBS_HOST;
+    std::vector<double> hD, hO1, hO2, hrhs;
+    const double *uD = nullptr, *uO1 = nullptr, *uO2 = nullptr, *urhs = nullptr;
     int nelem = 0, nintface = 0;
     size_t footprint = 0;   // slab bytes while admitting
     size_t persist = 0;     // slab bytes kept once admitted (plan, rhs, U)
@@ -94,7 +103,10 @@ struct Request {
     size_t nU = 0, hU_cls = 0;
     std::vector<double> hist;
     int nsweeps = 0, status = 0;
-    double t_submit = 0, t_admit = 0, t_done = 0;
+    // timeline (seconds on the server's clock): submitted, admitted (set-up
+    // enqueued), set-up seen complete (joins the batch), convergence seen by
+    // a step, result copies enqueued, copies seen complete (result ready)
+    double t_submit = 0, t_admit = 0, t_ready = 0, t_conv = 0, t_retire = 0, t_done = 0;
 };
 
 int build_mesh(Request *r) {
@@ -249,9 +261,21 @@ int admit_setup(tbs_server *s, Request *r) {
         if ((rc = tbs::dev_alloc(a, bs * r->nelem * 8, (void **)&r->tD, nullptr))) break;
         if ((rc = tbs::dev_alloc(a, bs * std::max(r->nintface, 1) * 8, (void **)&r->tO1, nullptr))) break;
         if ((rc = tbs::dev_alloc(a, bs * std::max(r->nintface, 1) * 8, (void **)&r->tO2, nullptr))) break;
-        if ((rc = tbs_generate_device(a, r->spec.seed, nv, r->nelem, r->nintface, r->spec.shift, r->tD, r->tO1,
-                                      r->tO2, r->d_rhs)))
+        if (r->caller) {
+            // the caller's Jacobian and rhs into the slab (rhs stays, in
+            // element order, for the plan's gather)
+            const bool dv = r->where == TBS_DEVICE;
+            const hipMemcpyKind kind = dv ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+            const size_t nD = bs * r->nelem * 8, nO = bs * (size_t)r->nintface * 8, nr = (size_t)nv * r->nelem * 8;
+            hipError_t e = hipMemcpyAsync(r->tD, dv ? r->uD : r->hD.data(), nD, kind, a->stream);
+            if (!e && nO) e = hipMemcpyAsync(r->tO1, dv ? r->uO1 : r->hO1.data(), nO, kind, a->stream);
+            if (!e && nO) e = hipMemcpyAsync(r->tO2, dv ? r->uO2 : r->hO2.data(), nO, kind, a->stream);
+            if (!e) e = hipMemcpyAsync(r->d_rhs, dv ? r->urhs : r->hrhs.data(), nr, kind, a->stream);
+            if (e) { rc = tbs::hip_fail(e, "request copy-in", __FILE__, __LINE__); break; }
+        } else if ((rc = tbs_generate_device(a, r->spec.seed, nv, r->nelem, r->nintface, r->spec.shift, r->tD,
+                                             r->tO1, r->tO2, r->d_rhs))) {
             break;
+        }
         if ((rc = tbs_memset(a, r->d_U, 0, (size_t)nv * r->nelem * 8))) break;
     } while (0);
     a->region = nullptr;
@@ -342,6 +366,30 @@ int reap(tbs_server *s, bool wait) {
     return rc;
 }
 
+// A caller's system as a request: mesh checked as tbs_plan_create checks it,
+// mesh arrays copied, Jacobian and rhs copied now (host) or at admission
+// (device).
+int caller_request(tbs_server *s, const tbs_mesh_host *m, std::unique_ptr<Request> *out) {
+    TBS_REQUIRE(m, TBS_ERR_ARG, "null mesh");
+    TBS_REQUIRE(m->nvar == s->ctx->nvar, TBS_ERR_ARG, "mesh nvar %d != server nvar %d", m->nvar, s->ctx->nvar);
+    size_t plan_bytes = 0;
+    TBS_TRY(tbs::plan_footprint(m, s->prec, &plan_bytes, true));
+    std::unique_ptr<Request> r(new Request());
+    r->caller = true;
+    r->nelem = m->nelem;
+    r->nintface = m->nintface;
+    r->epoint.assign(m->epoint, m->epoint + m->nelem + 1);
+    r->ef.assign(m->ef, m->ef + m->eftot);
+    r->fc.assign(m->fc, m->fc + 2 * (size_t)m->nintface);
+    r->linesize.assign(m->linesize, m->linesize + m->nline);
+    r->linepoint.assign(m->linepoint, m->linepoint + m->nline + 1);
+    r->lines.assign(m->lines, m->lines + m->nlineelem);
+    r->lineface.assign(m->lineface, m->lineface + m->nlineelem);
+    TBS_TRY(request_sizes(*r, s->prec, s->ctx->nvar, &r->persist, &r->footprint));
+    *out = std::move(r);
+    return TBS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -366,6 +414,13 @@ int tbs_server_create(tbs_ctx *ctx, size_t hbm_budget, int precision, double rto
     s->rtol = rtol;
     s->max_sweeps = max_sweeps;
     s->k = sweeps_per_step;
+    // the arena at its full size now, so that no allocation (which waits
+    // for the device) falls inside a request's latency; if the device
+    // refuses that much at once, smaller first segments, and the rest grows
+    // on demand
+    for (size_t want = s->budget; want >= std::min(kSegment, s->budget) && want > 0; want /= 2)
+        if (grow_arena(s.get(), want) == TBS_OK) break;
+    hipGetLastError();
     s->t0 = clk::now();
     *out = s.release();
     return TBS_OK;
@@ -395,6 +450,56 @@ int tbs_server_footprint(tbs_server *s, const tbs_server_request *req, size_t *b
     TBS_TRY(build_mesh(&r));
     size_t persist = 0;
     return request_sizes(r, s->prec, s->ctx->nvar, &persist, bytes);
+}
+
+int tbs_server_footprint_system(tbs_server *s, const tbs_mesh_host *mesh, size_t *bytes) {
+    TBS_REQUIRE(s && bytes, TBS_ERR_ARG, "null argument");
+    std::unique_ptr<Request> r;
+    TBS_TRY(caller_request(s, mesh, &r));
+    *bytes = r->footprint;
+    return TBS_OK;
+}
+
+int tbs_server_submit_system(tbs_server *s, const tbs_mesh_host *mesh, const double *jacD, const double *jacO1,
+                             const double *jacO2, const double *rhs, int where, long *id) {
+    TBS_REQUIRE(s && id && jacD && rhs, TBS_ERR_ARG, "null argument");
+    TBS_REQUIRE(where == TBS_HOST || where == TBS_DEVICE, TBS_ERR_ARG, "where must be TBS_HOST or TBS_DEVICE");
+    std::unique_ptr<Request> r;
+    TBS_TRY(caller_request(s, mesh, &r));
+    TBS_REQUIRE(r->nintface == 0 || (jacO1 && jacO2), TBS_ERR_ARG, "null face blocks");
+    TBS_REQUIRE(r->footprint <= s->budget, TBS_ERR_NOMEM, "request needs %zu bytes, budget is %zu", r->footprint,
+                s->budget);
+    const int nv = s->ctx->nvar;
+    const size_t bs = (size_t)nv * nv, nD = bs * r->nelem, nO = bs * (size_t)r->nintface, nr = (size_t)nv * r->nelem;
+    r->where = where;
+    if (where == TBS_HOST) {
+        r->hD.assign(jacD, jacD + nD);
+        if (nO) {
+            r->hO1.assign(jacO1, jacO1 + nO);
+            r->hO2.assign(jacO2, jacO2 + nO);
+        }
+        r->hrhs.assign(rhs, rhs + nr);
+    } else {
+        r->uD = jacD;
+        r->uO1 = jacO1;
+        r->uO2 = jacO2;
+        r->urhs = rhs;
+    }
+    r->id = s->next_id++;
+    r->t_submit = s->now();
+    *id = r->id;
+    s->queue.push_back(std::move(r));
+    return TBS_OK;
+}
+
+int tbs_server_timeline(tbs_server *s, long id, double *t, int n) {
+    TBS_REQUIRE(s && t, TBS_ERR_ARG, "null argument");
+    auto it = s->finished.find(id);
+    TBS_REQUIRE(it != s->finished.end(), TBS_ERR_ARG, "request %ld is not finished", id);
+    const Request *r = it->second.get();
+    const double v[6] = {r->t_submit, r->t_admit, r->t_ready, r->t_conv, r->t_retire, r->t_done};
+    for (int i = 0; i < n && i < 6; ++i) t[i] = v[i];
+    return TBS_OK;
 }
 
 int tbs_server_submit(tbs_server *s, const tbs_server_request *req, long *id) {
@@ -432,6 +537,7 @@ int tbs_server_step(tbs_server *s, int *nactive, int *nqueued, int *nadmitting) 
             keep.push_back(std::move(s->active[i]));
         } else {
             s->active[i]->status = st[4 * i + 2];
+            s->active[i]->t_conv = s->now();
             s->active[i]->nsweeps = std::max(std::min(st[4 * i], 4096) - 1, 0);
             done.push_back(std::move(s->active[i]));
         }
@@ -440,7 +546,13 @@ int tbs_server_step(tbs_server *s, int *nactive, int *nqueued, int *nadmitting) 
     for (auto it = s->admitting.begin(); it != s->admitting.end();) {
         Request *r = it->get();
         if (hipEventQuery(r->ready) != hipSuccess) { ++it; continue; }
-        // set-up complete: the temporaries of the slab go back to the arena
+        // set-up complete: the temporaries of the slab go back to the arena,
+        // and a caller's host copies are no longer needed
+        r->t_ready = s->now();
+        std::vector<double>().swap(r->hD);
+        std::vector<double>().swap(r->hO1);
+        std::vector<double>().swap(r->hO2);
+        std::vector<double>().swap(r->hrhs);
         s->arena.release(r->slab_off + r->persist, r->slab_len - r->persist);
         s->used -= r->slab_len - r->persist;
         r->slab_len = r->persist;
@@ -477,6 +589,7 @@ int tbs_server_step(tbs_server *s, int *nactive, int *nqueued, int *nadmitting) 
             TBS_HIP(hipMemcpyAsync(r->hist.data(), tbs::plan_hist(r->plan), r->hist.size() * 8,
                                    hipMemcpyDeviceToHost, rs));
             TBS_HIP(hipEventRecord(r->ready, rs));
+            r->t_retire = s->now();
             s->retiring.push_back(std::move(r));
         }
     }

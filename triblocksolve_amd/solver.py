@@ -588,6 +588,7 @@ class Server:
         self.max_sweeps = max_sweeps
         self.nvar = ctx.nvar
         self.sizes = {}
+        self._keep = {}       # device inputs of submitted systems, until released
         ctx._children.add(self)
 
     def close(self):
@@ -614,6 +615,35 @@ class Server:
         self.sizes[rid.value] = NL * NK
         return rid.value
 
+    def submit_system(self, mesh, jacD, jacO1, jacO2, rhs):
+        """Submit a caller's system: mesh (HostMesh), Jacobian blocks and rhs in
+        the reference layout, all host numpy arrays (copied now) or all
+        DeviceBuffers (read at admission; keep them until the id is polled)."""
+        ms = mesh.as_struct()
+        dev = isinstance(jacD, DeviceBuffer)
+        if not dev:
+            jacD, jacO1, jacO2, rhs = (np.ascontiguousarray(a, dtype=np.float64) for a in (jacD, jacO1, jacO2, rhs))
+        rid = C.c_long()
+        A.check(self.ctx._lib.tbs_server_submit_system(self.handle, C.byref(ms), _ptr(jacD), _ptr(jacO1), _ptr(jacO2),
+                                                       _ptr(rhs), A.TBS_DEVICE if dev else A.TBS_HOST,
+                                                       C.byref(rid)))
+        self.sizes[rid.value] = mesh.nelem
+        if dev:
+            self._keep[rid.value] = (jacD, jacO1, jacO2, rhs)
+        return rid.value
+
+    def footprint_system(self, mesh):
+        ms = mesh.as_struct()
+        out = C.c_size_t()
+        A.check(self.ctx._lib.tbs_server_footprint_system(self.handle, C.byref(ms), C.byref(out)))
+        return out.value
+
+    def timeline(self, rid):
+        """(submitted, admitted, set-up done, converged, copies enqueued, done), server clock seconds."""
+        t = np.zeros(6)
+        A.check(self.ctx._lib.tbs_server_timeline(self.handle, rid, _ptr(t), 6))
+        return t
+
     def step(self):
         a, q, m = C.c_int(), C.c_int(), C.c_int()
         A.check(self.ctx._lib.tbs_server_step(self.handle, C.byref(a), C.byref(q), C.byref(m)))
@@ -639,6 +669,7 @@ class Server:
     def release(self, rid):
         A.check(self.ctx._lib.tbs_server_release(self.handle, rid))
         self.sizes.pop(rid, None)
+        self._keep.pop(rid, None)
 
     def info(self):
         b, u, t = C.c_size_t(), C.c_size_t(), C.c_double()
