@@ -170,9 +170,42 @@ __global__ __launch_bounds__(64) void lineLU_kernel(int nlines, const int *__res
 }
 
 // -------------------------------------------------------------- solveDU --
-// src/okl/linesmoothLU_TriBlock_v3.okl:92-226.  One wave per line.
+// src/okl/linesmoothLU_TriBlock_v3.okl:92-226: forward dU_k = D'_k^-1 (-R_k -
+// A_k dU_{k-1}) with D'_k's packed LU factors, then dU_k -= DinvC_k dU_{k+1}.
+// Two waves per line, in chunks of G line steps:
+//   - the producer wave stages a chunk's D', A and R (forward) or DinvC and
+//     dU (backward) from HBM (all 64 lanes, entry t of the chunk by lane
+//     t % 64, loaded one chunk ahead and written to LDS when the chunk
+//     starts) and, forward, solves lane-parallel for every step s of the
+//     chunk and every column c: D'_s w = A_s(:,c) (c < NV: E_s = D'^-1 A) or
+//     D'_s w = -R_s (c = NV: g_s), with the reference's forward and back
+//     substitution on the packed factors (the back substitution multiplies
+//     by the reciprocal of the U pivot where the reference divides);
+//   - the consumer wave runs the chain one chunk behind: lane i = row i,
+//     forward dU_k = g_k - E_k dU_{k-1}, backward dU_k -= DinvC_k dU_{k+1}
+//     (ascending j, as the reference), each with the previous step's vector
+//     broadcast from the lanes that produced it.
+// One workgroup barrier per chunk hands a chunk from the producer to the
+// consumer (the producer works on chunk c+1 while the consumer runs chunk c;
+// the chunk buffers alternate).  The forward step is the reference's with its
+// two solves distributed over the right-hand sides; the rounding differs by
+// O(eps * cond(D')).  The backward step is the reference's arithmetic.
 template <int NV>
-__global__ __launch_bounds__(64) void solveDU_kernel(int nlines, const int *__restrict__ linesize,
+struct DUChunk {
+    static constexpr int BS = NV * NV;
+    static constexpr int SF = 2 * BS + NV;   // forward staging per step: D', A, R
+    static constexpr int SX = BS + NV;       // per step: E and g (forward), DinvC and dU (backward)
+    // steps per chunk: LDS for the staging, two chunk buffers and the
+    // reciprocal pivots within about 24 KiB, so six lines fit a CU
+    static constexpr int G0 = 3072 / (SF + 2 * SX + NV);
+    static constexpr int G = G0 < 1 ? 1 : (G0 > 8 ? 8 : G0);
+    static constexpr int NLF = (G * SF + 63) / 64;   // producer loads per lane per forward chunk
+    static constexpr int NLB = (G * SX + 63) / 64;   // per backward chunk
+    static constexpr int P = 64 / (NV + 1);          // steps solved side by side by the producer
+};
+
+template <int NV>
+__global__ __launch_bounds__(128) void solveDU_kernel(int nlines, const int *__restrict__ linesize,
                                                      const int *__restrict__ linepoint,
                                                      const int *__restrict__ lines,
                                                      const double *__restrict__ Dia,
@@ -180,63 +213,208 @@ __global__ __launch_bounds__(64) void solveDU_kernel(int nlines, const int *__re
                                                      const double *__restrict__ A,
                                                      double *__restrict__ dU,
                                                      const double *__restrict__ R) {
-    constexpr int BS = NV * NV;
+    using C = DUChunk<NV>;
+    constexpr int BS = NV * NV, SF = C::SF, SX = C::SX, G = C::G, P = C::P;
+    __shared__ double sbuf[G * SF];        // producer: the forward chunk being solved
+    __shared__ double sx[2][G * SX];       // chunk buffers handed to the consumer
+    __shared__ double srd[G * NV];         // producer: 1 / U(i,i) of each step's D'
+    __shared__ int pse[2][G];              // producer: element indices of the chunks it stages
+    __shared__ int se[2][G];               // the consumer's element indices, per chunk buffer
     const int l = blockIdx.x;
     if (l >= nlines) return;
-    const int i = threadIdx.x;
-    const int ir = (i < NV) ? i : 0;   // lanes >= NV shadow row 0 (results unused)
-    const int n = linesize[l];
-    const int m0 = linepoint[l];
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    // wave-uniform: the step loops must be scalar loops (the chain
+    // broadcasts read lanes that every step writes)
+    const int n = __builtin_amdgcn_readfirstlane(linesize[l]);
+    const int m0 = __builtin_amdgcn_readfirstlane(linepoint[l]);
     if (n <= 0) return;
+    auto wsync = []() { __builtin_amdgcn_wave_barrier(); };
+    const int ncf = (n + G - 1) / G;
+    const int nb = n - 1;                               // backward steps: n-2 .. 0
+    const int ncb = (nb + G - 1) / G;
+    auto bwd_k = [&](int c, int s) { return nb - 1 - (c * G + s); };
 
-    double prev[NV];
-    {
-        const int e0 = lines[m0];
-        double d[NV];
+    if (wave == 1) {
+        // ================================================= producer ==
+        // element index of line step k, clamped into the line: every load is
+        // unconditional and in bounds (a select after a load would make the
+        // loop wait for the loads in flight); steps past a chunk's end are
+        // never used
+        auto elem = [&](int k) -> int { return lines[m0 + min(max(k, 0), n - 1)]; };
+        const int gl = min(lane, G - 1);
+        const uintptr_t aD = (uintptr_t)Dia, aA = (uintptr_t)A, aR = (uintptr_t)R;
+        const uintptr_t aC = (uintptr_t)DinvC, aU = (uintptr_t)dU;
+        // global-space loads count only against vmcnt, so LDS waits do not
+        // wait for the loads in flight
+        using gptr = const __attribute__((address_space(1))) double *;
+        const int pp = lane / (NV + 1), cc = lane - pp * (NV + 1);   // step of a round, column
+        double Rr[C::NLF > C::NLB ? C::NLF : C::NLB];
+
+        // ---- forward
+        auto fwd_load = [&](int buf) {
 #pragma unroll
-        for (int j = 0; j < NV; ++j) d[j] = Dia[(size_t)BS * e0 + ir + NV * j];
-        const double S = -R[(size_t)NV * e0 + ir];                    // :144
-        const double x = solve_rows<NV>(d, S, i);                     // :153-155
-        if (i < NV) dU[(size_t)NV * e0 + i] = x;                      // :157
+            for (int r = 0; r < C::NLF; ++r) {
+                const int t = min(lane + 64 * r, G * SF - 1);
+                const int sidx = t / SF, q = t - sidx * SF;
+                const size_t e = (size_t)pse[buf][sidx];
+                const uintptr_t src = q < BS ? aD + 8 * (BS * e + q)
+                                             : (q < 2 * BS ? aA + 8 * (BS * e + (q - BS)) : aR + 8 * (NV * e + (q - 2 * BS)));
+                Rr[r] = *(gptr)src;
+            }
+        };
+        if (lane < G) pse[0][lane] = elem(lane);
+        wsync();
+        fwd_load(0);
+        int en = elem(G + gl);
+        for (int c = 0; c < ncf; ++c) {
+            const int cur = c & 1, nxt = cur ^ 1;
+            if (lane < G) pse[nxt][lane] = en;                   // chunk c+1's indices
+            wsync();
 #pragma unroll
-        for (int j = 0; j < NV; ++j) prev[j] = rdlane(x, j);
-    }
-    for (int k = 1; k < n; ++k) {                                     // :163-196
-        const int e = lines[m0 + k];
-        double a[NV], d[NV];
+            for (int r = 0; r < C::NLF; ++r) {                   // chunk c into LDS
+                const int t = lane + 64 * r;
+                if (t < G * SF) sbuf[t] = Rr[r];
+            }
+            if (lane < G) se[cur][lane] = pse[cur][lane];
+            wsync();
+            if (c + 1 < ncf) fwd_load(nxt);                      // chunk c+1 into flight
+            en = elem((c + 2) * G + gl);
+            const int kend = min(G, n - c * G);
+            for (int t = lane; t < kend * NV; t += 64) {
+                const int s = t / NV, ii = t - s * NV;
+                srd[t] = 1.0 / sbuf[s * SF + ii + NV * ii];
+            }
+            wsync();
+#pragma unroll 1
+            for (int s0 = 0; s0 < kend; s0 += P) {
+                const int s = s0 + pp;
+                if (pp < P && s < kend) {
+                    const double *LU = &sbuf[s * SF];
+                    // the lane's right-hand side: column cc of A_s, or -R_s
+                    // (both contiguous in the staged step; no branch, so the
+                    // LDS reads stay in flight together)
+                    const double *rhs = LU + (cc < NV ? BS + NV * cc : 2 * BS);
+                    const double sgn = cc < NV ? 1.0 : -1.0;
+                    double w[NV];
 #pragma unroll
-        for (int j = 0; j < NV; ++j) {
-            a[j] = A[(size_t)BS * e + ir + NV * j];
-            d[j] = Dia[(size_t)BS * e + ir + NV * j];
+                    for (int ii = 0; ii < NV; ++ii) w[ii] = sgn * rhs[ii];
+                    // forward: y_i = b_i - sum_{j<i} L(i,j) y_j (ascending j)
+#pragma unroll
+                    for (int ii = 1; ii < NV; ++ii) {
+                        double tot = 0.0;
+#pragma unroll
+                        for (int j = 0; j < ii; ++j) tot += LU[ii + NV * j] * w[j];
+                        w[ii] -= tot;
+                    }
+                    // back: x_i = (y_i - sum_{j>i} U(i,j) x_j) / U(i,i)
+#pragma unroll
+                    for (int ii = NV - 1; ii >= 0; --ii) {
+                        double tot = 0.0;
+#pragma unroll
+                        for (int j = ii + 1; j < NV; ++j) tot += LU[ii + NV * j] * w[j];
+                        w[ii] = (w[ii] - tot) * srd[s * NV + ii];
+                    }
+                    // E_s row-major (entry (i, cc) at i * NV + cc), g_s after it
+                    double *X = &sx[cur][s * SX] + (cc < NV ? cc : BS);
+                    const int xs = cc < NV ? NV : 1;
+#pragma unroll
+                    for (int ii = 0; ii < NV; ++ii) X[ii * xs] = w[ii];
+                }
+            }
+            __syncthreads();                                     // chunk c to the consumer
         }
-        const double rv = R[(size_t)NV * e + ir];
-        double tot = 0.0;
+        if (nb <= 0) return;
+        __syncthreads();                                         // the forward dU, all written
+
+        // ---- backward: chunk c holds steps n-2-cG .. n-2-cG-G+1; each step
+        // hands over DinvC(e) and the forward dU(e)
+        auto bwd_load = [&](int buf) {
 #pragma unroll
-        for (int j = 0; j < NV; ++j) tot += a[j] * prev[j];
-        const double S = -rv - tot;                                   // :188
-        const double x = solve_rows<NV>(d, S, i);
-        if (i < NV) dU[(size_t)NV * e + i] = x;
+            for (int r = 0; r < C::NLB; ++r) {
+                const int t = min(lane + 64 * r, G * SX - 1);
+                const int sidx = t / SX, q = t - sidx * SX;
+                const size_t e = (size_t)pse[buf][sidx];
+                const uintptr_t src = q < BS ? aC + 8 * (BS * e + q) : aU + 8 * (NV * e + (q - BS));
+                Rr[r] = *(gptr)src;
+            }
+        };
+        if (lane < G) pse[0][lane] = elem(bwd_k(0, lane));
+        wsync();
+        bwd_load(0);
+        en = elem(bwd_k(1, gl));
+        for (int c = 0; c < ncb; ++c) {
+            const int cur = c & 1, nxt = cur ^ 1;
+            if (lane < G) pse[nxt][lane] = en;
+            wsync();
 #pragma unroll
-        for (int j = 0; j < NV; ++j) prev[j] = rdlane(x, j);
+            for (int r = 0; r < C::NLB; ++r) {
+                const int t = lane + 64 * r;
+                if (t < G * SX) sx[cur][t] = Rr[r];
+            }
+            if (lane < G) se[cur][lane] = pse[cur][lane];
+            wsync();
+            if (c + 1 < ncb) bwd_load(nxt);
+            en = elem(bwd_k(c + 2, gl));
+            __syncthreads();
+        }
+        return;
     }
-    for (int k = n - 2; k >= 0; --k) {                                // :199-223
-        const int e = lines[m0 + k];
-        double g[NV];
+
+    // ===================================================== consumer ==
+    const int i = lane < NV ? lane : 0;    // lanes >= NV shadow row 0 (results unused)
+    double prev[NV];
 #pragma unroll
-        for (int j = 0; j < NV; ++j) g[j] = DinvC[(size_t)BS * e + ir + NV * j];
-        double tot = 0.0;
+    for (int j = 0; j < NV; ++j) prev[j] = 0.0;
+    for (int c = 0; c < ncf; ++c) {
+        const int cur = c & 1;
+        __syncthreads();                                         // chunk c from the producer
+        const int kend = min(G, n - c * G);
+#pragma unroll 1
+        for (int s = 0; s < kend; ++s) {
+            const double *X = &sx[cur][s * SX];
+            double x = X[BS + i];
+            if (c + s > 0) {
+                const double *Er = X + i * NV;
+                double t0 = 0.0, t1 = 0.0, t2 = 0.0;
 #pragma unroll
-        for (int j = 0; j < NV; ++j) tot += g[j] * prev[j];          // prev = dU(elast)
-        const double v = dU[(size_t)NV * e + ir] - tot;
-        if (i < NV) dU[(size_t)NV * e + i] = v;
+                for (int j = 0; j < NV; j += 3) {
+                    t0 += Er[j] * prev[j];
+                    if (j + 1 < NV) t1 += Er[j + 1] * prev[j + 1];
+                    if (j + 2 < NV) t2 += Er[j + 2] * prev[j + 2];
+                }
+                x -= (t0 + t1) + t2;
+            }
+            const size_t e = (size_t)__builtin_amdgcn_readfirstlane(se[cur][s]);
+            if (lane < NV) dU[NV * e + lane] = x;                // :157, :194
 #pragma unroll
-        for (int j = 0; j < NV; ++j) prev[j] = rdlane(v, j);
+            for (int j = 0; j < NV; ++j) prev[j] = rdlane(x, j);
+        }
+    }
+    if (nb <= 0) return;
+    __syncthreads();                                             // the forward dU, all written
+    for (int c = 0; c < ncb; ++c) {                              // :199-223
+        const int cur = c & 1;
+        __syncthreads();
+        const int kend = min(G, nb - c * G);
+#pragma unroll 1
+        for (int s = 0; s < kend; ++s) {
+            const double *X = &sx[cur][s * SX];
+            double tot = 0.0;
+#pragma unroll
+            for (int j = 0; j < NV; ++j) tot += X[i + NV * j] * prev[j];   // prev = dU(elast)
+            const double v = X[BS + i] - tot;
+            const size_t e = (size_t)__builtin_amdgcn_readfirstlane(se[cur][s]);
+            if (lane < NV) dU[NV * e + lane] = v;
+#pragma unroll
+            for (int j = 0; j < NV; ++j) prev[j] = rdlane(v, j);
+        }
     }
 }
 
 // -------------------------------------------------------------- lineRes --
 // src/okl/linesmoothLU_TriBlock_v3.okl:228-336.  One thread per (element,
-// row); grid.y = line, grid.x strides over the line's positions.
+// row); grid.y strides over the lines, grid.x over a line's positions.
 template <int NV>
 __global__ void lineRes_kernel(int nlines, const int *__restrict__ epoint,
                                const int *__restrict__ ef, const int *__restrict__ fc,
@@ -247,9 +425,9 @@ __global__ void lineRes_kernel(int nlines, const int *__restrict__ epoint,
                                const double *__restrict__ U, double *__restrict__ R) {
     constexpr int BS = NV * NV;
     constexpr int EPB = (256 / NV) > 0 ? (256 / NV) : 1;   // elements per block
-    const int l = blockIdx.y;
     const int kl = threadIdx.x / NV, i = threadIdx.x % NV;
-    if (l >= nlines || kl >= EPB) return;
+    if (kl >= EPB) return;
+    for (int l = blockIdx.y; l < nlines; l += gridDim.y) {
     const int n = linesize[l];
     const int m0 = linepoint[l];
     for (int k = blockIdx.x * EPB + kl; k < n; k += gridDim.x * EPB) {
@@ -272,6 +450,7 @@ __global__ void lineRes_kernel(int nlines, const int *__restrict__ epoint,
             }
         }
         R[(size_t)NV * e + i] += sR;                                  // :331
+    }
     }
 }
 
@@ -325,8 +504,7 @@ __global__ void assemble_kernel(int nline, const int *__restrict__ linesize,
                                 const double *__restrict__ O1, const double *__restrict__ O2,
                                 double *__restrict__ A) {
     constexpr int BS = NV * NV;
-    const int l = blockIdx.y;
-    if (l >= nline) return;
+    for (int l = blockIdx.y; l < nline; l += gridDim.y) {
     const int n = linesize[l];
     const int m0 = linepoint[l];
     const long total = (long)(n - 1) * BS;
@@ -339,6 +517,7 @@ __global__ void assemble_kernel(int nline, const int *__restrict__ linesize,
         const int e = lines[m];
         const double *src = (e == fc[2 * f]) ? O2 : O1;
         A[(size_t)BS * e + q] = src[(size_t)BS * f + q];
+    }
     }
 }
 
@@ -464,12 +643,12 @@ int tbs_assembleTriBlocks(tbs_ctx *ctx, int nline, const int *linesize, const in
                           const double *jacO1, const double *jacO2, double *A) {
     int rc = tbs::use_ctx(ctx);
     if (rc) return rc;
-    TBS_REQUIRE(nline >= 0 && nline <= 65535, TBS_ERR_ARG, "nline %d outside [0, 65535]", nline);
+    TBS_REQUIRE(nline >= 0, TBS_ERR_ARG, "nline %d < 0", nline);
     if (nline == 0) return TBS_OK;
     TBS_REQUIRE(linesize && linepoint && lines && lineface && fc && jacO1 && jacO2 && A,
                 TBS_ERR_ARG, "null array");
     switch (ctx->nvar) {
-#define X(v) case v: hipLaunchKernelGGL(assemble_kernel<v>, dim3(16, nline), dim3(256), 0, ctx->stream, \
+#define X(v) case v: hipLaunchKernelGGL(assemble_kernel<v>, dim3(16, std::min(nline, 65535)), dim3(256), 0, ctx->stream, \
                    nline, linesize, linepoint, lines, lineface, fc, jacO1, jacO2, A); break;
         TBS_REF_NVARS(X)
 #undef X
@@ -513,7 +692,7 @@ int tbs_triblock_solveDU(tbs_ctx *ctx, int nelem, int nintfaces, int eftot, int 
     TBS_REQUIRE(linesize && linepoint && lines && Dia && DinvC && A && dU && R, TBS_ERR_ARG,
                 "null array");
     switch (ctx->nvar) {
-#define X(v) case v: hipLaunchKernelGGL(solveDU_kernel<v>, dim3(nlines), dim3(64), 0, ctx->stream, \
+#define X(v) case v: hipLaunchKernelGGL(solveDU_kernel<v>, dim3(nlines), dim3(128), 0, ctx->stream, \
                    nlines, linesize, linepoint, lines, Dia, DinvC, A, dU, R); break;
         TBS_REF_NVARS(X)
 #undef X
@@ -531,7 +710,7 @@ int tbs_triblock_lineRes(tbs_ctx *ctx, int nelem, int nintfaces, int eftot, int 
     (void)nelem; (void)eftot; (void)linelemtot;
     int rc = tbs::use_ctx(ctx);
     if (rc) return rc;
-    TBS_REQUIRE(nlines >= 0 && nlines <= 65535, TBS_ERR_ARG, "nlines %d outside [0, 65535]", nlines);
+    TBS_REQUIRE(nlines >= 0, TBS_ERR_ARG, "nlines %d < 0", nlines);
     if (nlines == 0) return TBS_OK;
     TBS_REQUIRE(epoint && ef && linesize && linepoint && lines && Dia && U && R, TBS_ERR_ARG,
                 "null array");
@@ -539,7 +718,7 @@ int tbs_triblock_lineRes(tbs_ctx *ctx, int nelem, int nintfaces, int eftot, int 
     const int nv = ctx->nvar;
     const int epb = (256 / nv) > 0 ? 256 / nv : 1;
     switch (nv) {
-#define X(v) case v: hipLaunchKernelGGL(lineRes_kernel<v>, dim3(8, nlines), dim3(epb * v), 0, ctx->stream, \
+#define X(v) case v: hipLaunchKernelGGL(lineRes_kernel<v>, dim3(8, std::min(nlines, 65535)), dim3(epb * v), 0, ctx->stream, \
                    nlines, epoint, ef, fc, linesize, linepoint, lines, Dia, Of1, Of2, U, R); break;
         TBS_REF_NVARS(X)
 #undef X
