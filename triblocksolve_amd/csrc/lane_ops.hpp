@@ -17,6 +17,21 @@ __device__ __forceinline__ float rdlane(float v, int lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+// out[j] = v of lane j, for j < N.  N <= 16: a DPP row broadcast per j
+// (row_newbcast: each row of 16 lanes receives its own lane j, a VGPR result,
+// one instruction per double); N > 16: v_readlane pairs (SGPR results).  In
+// both cases lanes 0..15 see lane j's value, bit for bit.
+template <int N, int J = 0>
+__device__ __forceinline__ void bcast_rows(double v, double (&out)[N]) {
+    if constexpr (J < N) {
+        if constexpr (N <= 16)
+            out[J] = __builtin_amdgcn_mov_dpp(v, 0x150 + J, 0xF, 0xF, true);
+        else
+            out[J] = rdlane(v, J);
+        bcast_rows<N, J + 1>(v, out);
+    }
+}
+
 // Wave-wide sum (all 64 lanes), fixed butterfly order.
 __device__ __forceinline__ double wave_sum(double v) {
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);

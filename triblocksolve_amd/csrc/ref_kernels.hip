@@ -172,40 +172,44 @@ __global__ __launch_bounds__(64) void lineLU_kernel(int nlines, const int *__res
 // -------------------------------------------------------------- solveDU --
 // src/okl/linesmoothLU_TriBlock_v3.okl:92-226: forward dU_k = D'_k^-1 (-R_k -
 // A_k dU_{k-1}) with D'_k's packed LU factors, then dU_k -= DinvC_k dU_{k+1}.
-// Two waves per line, in chunks of G line steps:
-//   - the producer wave stages a chunk's D', A and R (forward) or DinvC and
-//     dU (backward) from HBM (all 64 lanes, entry t of the chunk by lane
-//     t % 64, loaded one chunk ahead and written to LDS when the chunk
-//     starts) and, forward, solves lane-parallel for every step s of the
-//     chunk and every column c: D'_s w = A_s(:,c) (c < NV: E_s = D'^-1 A) or
-//     D'_s w = -R_s (c = NV: g_s), with the reference's forward and back
-//     substitution on the packed factors (the back substitution multiplies
-//     by the reciprocal of the U pivot where the reference divides);
+// Three waves per line, in chunks of G line steps:
+//   - two producer waves take the chunks in turn (producer p the chunks
+//     c with c % 2 == p, in chunk buffer p).  A producer keeps its next
+//     chunk's loads in flight while the other producer's chunk is worked on,
+//     so two chunks per line are in flight.  Forward it stages D', A and R
+//     (64 / G lanes per step: a load reads one array, so a lane's address is
+//     its step's block plus a constant) and solves lane-parallel, in place,
+//     for every step s and every column c: D'_s w = A_s(:,c) (c < NV: E_s =
+//     D'^-1 A) or D'_s w = -R_s (c = NV: g_s), with the reference's forward
+//     and back substitution on the packed factors (the back substitution
+//     multiplies by the reciprocal of the U pivot where the reference
+//     divides).  Backward it stages DinvC and the forward dU;
 //   - the consumer wave runs the chain one chunk behind: lane i = row i,
 //     forward dU_k = g_k - E_k dU_{k-1}, backward dU_k -= DinvC_k dU_{k+1}
 //     (ascending j, as the reference), each with the previous step's vector
 //     broadcast from the lanes that produced it.
-// One workgroup barrier per chunk hands a chunk from the producer to the
-// consumer (the producer works on chunk c+1 while the consumer runs chunk c;
-// the chunk buffers alternate).  The forward step is the reference's with its
-// two solves distributed over the right-hand sides; the rounding differs by
-// O(eps * cond(D')).  The backward step is the reference's arithmetic.
+// One workgroup barrier per chunk hands a chunk to the consumer.  E_s is
+// column-major like the reference's blocks.  The forward step is the
+// reference's with its two solves distributed over the right-hand sides; the
+// rounding differs by O(eps * cond(D')).  The backward step is the
+// reference's arithmetic.
 template <int NV>
 struct DUChunk {
     static constexpr int BS = NV * NV;
-    static constexpr int SF = 2 * BS + NV;   // forward staging per step: D', A, R
-    static constexpr int SX = BS + NV;       // per step: E and g (forward), DinvC and dU (backward)
-    // steps per chunk: LDS for the staging, two chunk buffers and the
-    // reciprocal pivots within about 24 KiB, so six lines fit a CU
-    static constexpr int G0 = 3072 / (SF + 2 * SX + NV);
-    static constexpr int G = G0 < 1 ? 1 : (G0 > 8 ? 8 : G0);
-    static constexpr int NLF = (G * SF + 63) / 64;   // producer loads per lane per forward chunk
-    static constexpr int NLB = (G * SX + 63) / 64;   // per backward chunk
-    static constexpr int P = 64 / (NV + 1);          // steps solved side by side by the producer
+    static constexpr int SF = 2 * BS + NV;   // per step, forward: D', A then E, R then g
+    static constexpr int SX = BS + NV;       // per step, backward: DinvC, dU
+    // steps per chunk (a power of two up to 8): the two chunk buffers and
+    // the reciprocal pivots within about 32 KiB, so five lines fit a CU
+    static constexpr int G0 = 4096 / (2 * SF + NV);
+    static constexpr int G = G0 >= 8 ? 8 : (G0 >= 4 ? 4 : (G0 >= 2 ? 2 : 1));
+    static constexpr int LPS = 64 / G;               // producer lanes per step
+    static constexpr int NM = (BS + LPS - 1) / LPS;  // loads per lane of a step's block
+    static constexpr int NVL = (NV + LPS - 1) / LPS; // of a step's vector
+    static constexpr int KC = (NV + 1 + LPS - 1) / LPS;   // right-hand sides per producer lane
 };
 
 template <int NV>
-__global__ __launch_bounds__(128) void solveDU_kernel(int nlines, const int *__restrict__ linesize,
+__global__ __launch_bounds__(192) void solveDU_kernel(int nlines, const int *__restrict__ linesize,
                                                      const int *__restrict__ linepoint,
                                                      const int *__restrict__ lines,
                                                      const double *__restrict__ Dia,
@@ -214,12 +218,11 @@ __global__ __launch_bounds__(128) void solveDU_kernel(int nlines, const int *__r
                                                      double *__restrict__ dU,
                                                      const double *__restrict__ R) {
     using C = DUChunk<NV>;
-    constexpr int BS = NV * NV, SF = C::SF, SX = C::SX, G = C::G, P = C::P;
-    __shared__ double sbuf[G * SF];        // producer: the forward chunk being solved
-    __shared__ double sx[2][G * SX];       // chunk buffers handed to the consumer
-    __shared__ double srd[G * NV];         // producer: 1 / U(i,i) of each step's D'
-    __shared__ int pse[2][G];              // producer: element indices of the chunks it stages
-    __shared__ int se[2][G];               // the consumer's element indices, per chunk buffer
+    constexpr int BS = NV * NV, SF = C::SF, SX = C::SX, G = C::G, LPS = C::LPS, KC = C::KC;
+    __shared__ double sbuf[2][G * SF];     // chunk buffers, one per producer
+    __shared__ double srd[G * NV];         // 1 / U(i,i) of each step's D' (one producer at a time)
+    __shared__ int pse[2][G];              // producer p: element indices of its chunk in flight
+    __shared__ int se[2][G];               // element indices of the chunk in each buffer
     const int l = blockIdx.x;
     if (l >= nlines) return;
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
@@ -235,92 +238,131 @@ __global__ __launch_bounds__(128) void solveDU_kernel(int nlines, const int *__r
     const int ncb = (nb + G - 1) / G;
     auto bwd_k = [&](int c, int s) { return nb - 1 - (c * G + s); };
 
-    if (wave == 1) {
-        // ================================================= producer ==
+    if (wave > 0) {
+        // ================================================ producers ==
+        const int p = wave - 1;
+        double *const buf = sbuf[p];
         // element index of line step k, clamped into the line: every load is
         // unconditional and in bounds (a select after a load would make the
         // loop wait for the loads in flight); steps past a chunk's end are
         // never used
         auto elem = [&](int k) -> int { return lines[m0 + min(max(k, 0), n - 1)]; };
         const int gl = min(lane, G - 1);
-        const uintptr_t aD = (uintptr_t)Dia, aA = (uintptr_t)A, aR = (uintptr_t)R;
-        const uintptr_t aC = (uintptr_t)DinvC, aU = (uintptr_t)dU;
         // global-space loads count only against vmcnt, so LDS waits do not
         // wait for the loads in flight
         using gptr = const __attribute__((address_space(1))) double *;
-        const int pp = lane / (NV + 1), cc = lane - pp * (NV + 1);   // step of a round, column
-        double Rr[C::NLF > C::NLB ? C::NLF : C::NLB];
+        const gptr gD = (gptr)Dia, gA = (gptr)A, gR = (gptr)R, gC = (gptr)DinvC, gU = (gptr)dU;
+        // lane (ls, lq) stages entries lq, lq + LPS, ... of step ls's block
+        // and vector (the last of them clamped into the block) and solves
+        // for its right-hand sides lq, lq + LPS, ... (the last clamped to NV)
+        const int ls = lane / LPS, lq = lane - ls * LPS;
+        auto mo = [&](int r) { return (r + 1) * LPS <= BS ? r * LPS : min(r * LPS, BS - 1 - lq); };
+        auto vo = [&](int r) { return (r + 1) * LPS <= NV ? r * LPS : min(r * LPS, NV - 1 - lq); };
+        double Rm[C::NM], Rm2[C::NM], Rv[C::NVL];
 
-        // ---- forward
-        auto fwd_load = [&](int buf) {
+        // ---- forward: chunk c (c % 2 == p) is staged and solved in the
+        // interval before barrier c, while chunk c+2's loads are issued
+        auto fwd_load = [&]() {
+            const size_t e = (unsigned)pse[p][ls];
+            const gptr pD = gD + e * BS + lq, pA = gA + e * BS + lq, pR = gR + e * NV + lq;
 #pragma unroll
-            for (int r = 0; r < C::NLF; ++r) {
-                const int t = min(lane + 64 * r, G * SF - 1);
-                const int sidx = t / SF, q = t - sidx * SF;
-                const size_t e = (size_t)pse[buf][sidx];
-                const uintptr_t src = q < BS ? aD + 8 * (BS * e + q)
-                                             : (q < 2 * BS ? aA + 8 * (BS * e + (q - BS)) : aR + 8 * (NV * e + (q - 2 * BS)));
-                Rr[r] = *(gptr)src;
-            }
+            for (int r = 0; r < C::NM; ++r) { Rm[r] = pD[mo(r)]; Rm2[r] = pA[mo(r)]; }
+#pragma unroll
+            for (int r = 0; r < C::NVL; ++r) Rv[r] = pR[vo(r)];
         };
-        if (lane < G) pse[0][lane] = elem(lane);
+        if (lane < G) pse[p][lane] = elem(p * G + lane);
         wsync();
-        fwd_load(0);
-        int en = elem(G + gl);
+        fwd_load();
+        int ei = elem((p + 2) * G + gl);
         for (int c = 0; c < ncf; ++c) {
-            const int cur = c & 1, nxt = cur ^ 1;
-            if (lane < G) pse[nxt][lane] = en;                   // chunk c+1's indices
-            wsync();
+            if ((c & 1) == p) {
+                {                                                // chunk c into LDS
+                    double *sb = &buf[ls * SF + lq];
 #pragma unroll
-            for (int r = 0; r < C::NLF; ++r) {                   // chunk c into LDS
-                const int t = lane + 64 * r;
-                if (t < G * SF) sbuf[t] = Rr[r];
-            }
-            if (lane < G) se[cur][lane] = pse[cur][lane];
-            wsync();
-            if (c + 1 < ncf) fwd_load(nxt);                      // chunk c+1 into flight
-            en = elem((c + 2) * G + gl);
-            const int kend = min(G, n - c * G);
-            for (int t = lane; t < kend * NV; t += 64) {
-                const int s = t / NV, ii = t - s * NV;
-                srd[t] = 1.0 / sbuf[s * SF + ii + NV * ii];
-            }
-            wsync();
-#pragma unroll 1
-            for (int s0 = 0; s0 < kend; s0 += P) {
-                const int s = s0 + pp;
-                if (pp < P && s < kend) {
-                    const double *LU = &sbuf[s * SF];
-                    // the lane's right-hand side: column cc of A_s, or -R_s
-                    // (both contiguous in the staged step; no branch, so the
-                    // LDS reads stay in flight together)
-                    const double *rhs = LU + (cc < NV ? BS + NV * cc : 2 * BS);
-                    const double sgn = cc < NV ? 1.0 : -1.0;
-                    double w[NV];
+                    for (int r = 0; r < C::NM; ++r)
+                        if (r * LPS + lq < BS) { sb[r * LPS] = Rm[r]; sb[BS + r * LPS] = Rm2[r]; }
 #pragma unroll
-                    for (int ii = 0; ii < NV; ++ii) w[ii] = sgn * rhs[ii];
+                    for (int r = 0; r < C::NVL; ++r)
+                        if (r * LPS + lq < NV) sb[2 * BS + r * LPS] = Rv[r];
+                }
+                if (lane < G) se[p][lane] = pse[p][lane];
+                wsync();
+                if (lane < G) pse[p][lane] = ei;                 // chunk c+2's indices
+                wsync();
+                ei = elem((c + 4) * G + gl);                     // ahead of the data loads
+                const int kend = min(G, n - c * G);
+                for (int t = lane; t < kend * NV; t += 64) {
+                    const int s = t / NV, ii = t - s * NV;
+                    srd[t] = 1.0 / buf[s * SF + ii + NV * ii];
+                }
+                wsync();
+                if (ls < kend) {
+                    const int s = ls;
+                    const double *LU = &buf[s * SF];
+                    // right-hand side k: column cc of A_s, or R_s (cc = NV)
+                    // right after A_s; overwritten by the solution (column
+                    // cc of E_s, or g_s).  The factor entries are read once
+                    // for all of the lane's right-hand sides.
+                    double *X[KC];
+                    double w[KC][NV];
+#pragma unroll
+                    for (int k = 0; k < KC; ++k) {
+                        const int cc = min(lq + k * LPS, NV);
+                        X[k] = &buf[s * SF + BS + NV * cc];
+                        const double sgn = cc < NV ? 1.0 : -1.0;
+#pragma unroll
+                        for (int ii = 0; ii < NV; ++ii) w[k][ii] = sgn * X[k][ii];
+                    }
                     // forward: y_i = b_i - sum_{j<i} L(i,j) y_j (ascending j)
 #pragma unroll
                     for (int ii = 1; ii < NV; ++ii) {
-                        double tot = 0.0;
+                        double tot[KC];
 #pragma unroll
-                        for (int j = 0; j < ii; ++j) tot += LU[ii + NV * j] * w[j];
-                        w[ii] -= tot;
+                        for (int k = 0; k < KC; ++k) tot[k] = 0.0;
+#pragma unroll
+                        for (int j = 0; j < ii; ++j) {
+                            const double lij = LU[ii + NV * j];
+#pragma unroll
+                            for (int k = 0; k < KC; ++k) tot[k] += lij * w[k][j];
+                        }
+#pragma unroll
+                        for (int k = 0; k < KC; ++k) w[k][ii] -= tot[k];
+                        // a row at a time: the factor reads are not all
+                        // hoisted into registers
+                        __builtin_amdgcn_sched_barrier(0);
                     }
                     // back: x_i = (y_i - sum_{j>i} U(i,j) x_j) / U(i,i)
 #pragma unroll
                     for (int ii = NV - 1; ii >= 0; --ii) {
-                        double tot = 0.0;
+                        double tot[KC];
 #pragma unroll
-                        for (int j = ii + 1; j < NV; ++j) tot += LU[ii + NV * j] * w[j];
-                        w[ii] = (w[ii] - tot) * srd[s * NV + ii];
+                        for (int k = 0; k < KC; ++k) tot[k] = 0.0;
+#pragma unroll
+                        for (int j = ii + 1; j < NV; ++j) {
+                            const double uij = LU[ii + NV * j];
+#pragma unroll
+                            for (int k = 0; k < KC; ++k) tot[k] += uij * w[k][j];
+                        }
+                        const double rd = srd[s * NV + ii];
+#pragma unroll
+                        for (int k = 0; k < KC; ++k) w[k][ii] = (w[k][ii] - tot[k]) * rd;
+                        __builtin_amdgcn_sched_barrier(0);
                     }
-                    // E_s row-major (entry (i, cc) at i * NV + cc), g_s after it
-                    double *X = &sx[cur][s * SX] + (cc < NV ? cc : BS);
-                    const int xs = cc < NV ? NV : 1;
+                    // (lanes whose last right-hand side is clamped to g_s
+                    // store the same g_s as the lane that owns it: the same
+                    // operations on the same data; an unconditional store
+                    // keeps the solve one straight block)
 #pragma unroll
-                    for (int ii = 0; ii < NV; ++ii) X[ii * xs] = w[ii];
+                    for (int k = 0; k < KC; ++k) {
+#pragma unroll
+                        for (int ii = 0; ii < NV; ++ii) X[k][ii] = w[k][ii];
+                    }
                 }
+                // chunk c+2 into flight, after the solve so that its
+                // registers are free during it (past the line's end the
+                // clamped indices load the last step again: unconditional
+                // loads keep the waits for the loads in flight exact)
+                fwd_load();
             }
             __syncthreads();                                     // chunk c to the consumer
         }
@@ -329,33 +371,36 @@ __global__ __launch_bounds__(128) void solveDU_kernel(int nlines, const int *__r
 
         // ---- backward: chunk c holds steps n-2-cG .. n-2-cG-G+1; each step
         // hands over DinvC(e) and the forward dU(e)
-        auto bwd_load = [&](int buf) {
+        auto bwd_load = [&]() {
+            const size_t e = (unsigned)pse[p][ls];
+            const gptr pC = gC + e * BS + lq, pU = gU + e * NV + lq;
 #pragma unroll
-            for (int r = 0; r < C::NLB; ++r) {
-                const int t = min(lane + 64 * r, G * SX - 1);
-                const int sidx = t / SX, q = t - sidx * SX;
-                const size_t e = (size_t)pse[buf][sidx];
-                const uintptr_t src = q < BS ? aC + 8 * (BS * e + q) : aU + 8 * (NV * e + (q - BS));
-                Rr[r] = *(gptr)src;
-            }
+            for (int r = 0; r < C::NM; ++r) Rm[r] = pC[mo(r)];
+#pragma unroll
+            for (int r = 0; r < C::NVL; ++r) Rv[r] = pU[vo(r)];
         };
-        if (lane < G) pse[0][lane] = elem(bwd_k(0, lane));
+        if (lane < G) pse[p][lane] = elem(bwd_k(p, lane));
         wsync();
-        bwd_load(0);
-        en = elem(bwd_k(1, gl));
+        bwd_load();
+        ei = elem(bwd_k(p + 2, gl));
         for (int c = 0; c < ncb; ++c) {
-            const int cur = c & 1, nxt = cur ^ 1;
-            if (lane < G) pse[nxt][lane] = en;
-            wsync();
+            if ((c & 1) == p) {
+                {
+                    double *sb = &buf[ls * SX + lq];
 #pragma unroll
-            for (int r = 0; r < C::NLB; ++r) {
-                const int t = lane + 64 * r;
-                if (t < G * SX) sx[cur][t] = Rr[r];
+                    for (int r = 0; r < C::NM; ++r)
+                        if (r * LPS + lq < BS) sb[r * LPS] = Rm[r];
+#pragma unroll
+                    for (int r = 0; r < C::NVL; ++r)
+                        if (r * LPS + lq < NV) sb[BS + r * LPS] = Rv[r];
+                }
+                if (lane < G) se[p][lane] = pse[p][lane];
+                wsync();
+                if (lane < G) pse[p][lane] = ei;
+                wsync();
+                ei = elem(bwd_k(c + 4, gl));
+                bwd_load();                                      // chunk c+2 into flight
             }
-            if (lane < G) se[cur][lane] = pse[cur][lane];
-            wsync();
-            if (c + 1 < ncb) bwd_load(nxt);
-            en = elem(bwd_k(c + 2, gl));
             __syncthreads();
         }
         return;
@@ -366,29 +411,58 @@ __global__ __launch_bounds__(128) void solveDU_kernel(int nlines, const int *__r
     double prev[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) prev[j] = 0.0;
+    // Two register sets take turns: a step's row of its block, its vector
+    // entry and its element index are read from LDS while the step before
+    // runs, so the chain waits only on itself.  The reads are unconditional
+    // (the step index clamped into the chunk), which keeps the waits for
+    // them exact.
+    double xa[NV + 1], xb[NV + 1];
+    int ea, eb;
+    auto ld = [&](double (&v)[NV + 1], int &ev, const double *X, int stride, const int *ix, int s) {
+#pragma unroll
+        for (int j = 0; j <= NV; ++j) v[j] = X[s * stride + NV * j];
+        ev = ix[s];
+    };
+    // forward: dU_k = g_k - E_k dU_{k-1}, three partial sums over j
+    auto fstep = [&](const double (&X)[NV + 1], int ev, bool first) {
+        const size_t e = (size_t)__builtin_amdgcn_readfirstlane(ev);
+        double x = X[NV];
+        if (!first) {
+            double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+#pragma unroll
+            for (int j = 0; j < NV; j += 3) {
+                t0 += X[j] * prev[j];
+                if (j + 1 < NV) t1 += X[j + 1] * prev[j + 1];
+                if (j + 2 < NV) t2 += X[j + 2] * prev[j + 2];
+            }
+            x -= (t0 + t1) + t2;
+        }
+        if (lane < NV) dU[NV * e + lane] = x;                    // :157, :194
+        tbs::bcast_rows<NV>(x, prev);
+    };
+    // backward: dU_k -= DinvC_k dU_{k+1} (ascending j)
+    auto bstep = [&](const double (&X)[NV + 1], int ev) {
+        const size_t e = (size_t)__builtin_amdgcn_readfirstlane(ev);
+        double tot = 0.0;
+#pragma unroll
+        for (int j = 0; j < NV; ++j) tot += X[j] * prev[j];      // prev = dU(elast)
+        const double v = X[NV] - tot;
+        if (lane < NV) dU[NV * e + lane] = v;
+        tbs::bcast_rows<NV>(v, prev);
+    };
     for (int c = 0; c < ncf; ++c) {
         const int cur = c & 1;
-        __syncthreads();                                         // chunk c from the producer
-        const int kend = min(G, n - c * G);
+        __syncthreads();                                         // chunk c from its producer
+        const int kend = min(G, n - c * G), kl = kend - 1;
+        const double *Xc = &sbuf[cur][BS + i];                   // E_s row i, then g_s(i)
+        ld(xa, ea, Xc, SF, se[cur], 0);
 #pragma unroll 1
-        for (int s = 0; s < kend; ++s) {
-            const double *X = &sx[cur][s * SX];
-            double x = X[BS + i];
-            if (c + s > 0) {
-                const double *Er = X + i * NV;
-                double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-#pragma unroll
-                for (int j = 0; j < NV; j += 3) {
-                    t0 += Er[j] * prev[j];
-                    if (j + 1 < NV) t1 += Er[j + 1] * prev[j + 1];
-                    if (j + 2 < NV) t2 += Er[j + 2] * prev[j + 2];
-                }
-                x -= (t0 + t1) + t2;
-            }
-            const size_t e = (size_t)__builtin_amdgcn_readfirstlane(se[cur][s]);
-            if (lane < NV) dU[NV * e + lane] = x;                // :157, :194
-#pragma unroll
-            for (int j = 0; j < NV; ++j) prev[j] = rdlane(x, j);
+        for (int s = 0; s < kend; s += 2) {
+            ld(xb, eb, Xc, SF, se[cur], min(s + 1, kl));
+            fstep(xa, ea, c + s == 0);
+            if (s + 1 >= kend) break;
+            ld(xa, ea, Xc, SF, se[cur], min(s + 2, kl));
+            fstep(xb, eb, false);
         }
     }
     if (nb <= 0) return;
@@ -396,18 +470,16 @@ __global__ __launch_bounds__(128) void solveDU_kernel(int nlines, const int *__r
     for (int c = 0; c < ncb; ++c) {                              // :199-223
         const int cur = c & 1;
         __syncthreads();
-        const int kend = min(G, nb - c * G);
+        const int kend = min(G, nb - c * G), kl = kend - 1;
+        const double *Xc = &sbuf[cur][i];                        // DinvC_s row i, then dU_s(i)
+        ld(xa, ea, Xc, SX, se[cur], 0);
 #pragma unroll 1
-        for (int s = 0; s < kend; ++s) {
-            const double *X = &sx[cur][s * SX];
-            double tot = 0.0;
-#pragma unroll
-            for (int j = 0; j < NV; ++j) tot += X[i + NV * j] * prev[j];   // prev = dU(elast)
-            const double v = X[BS + i] - tot;
-            const size_t e = (size_t)__builtin_amdgcn_readfirstlane(se[cur][s]);
-            if (lane < NV) dU[NV * e + lane] = v;
-#pragma unroll
-            for (int j = 0; j < NV; ++j) prev[j] = rdlane(v, j);
+        for (int s = 0; s < kend; s += 2) {
+            ld(xb, eb, Xc, SX, se[cur], min(s + 1, kl));
+            bstep(xa, ea);
+            if (s + 1 >= kend) break;
+            ld(xa, ea, Xc, SX, se[cur], min(s + 2, kl));
+            bstep(xb, eb);
         }
     }
 }
@@ -692,7 +764,7 @@ int tbs_triblock_solveDU(tbs_ctx *ctx, int nelem, int nintfaces, int eftot, int 
     TBS_REQUIRE(linesize && linepoint && lines && Dia && DinvC && A && dU && R, TBS_ERR_ARG,
                 "null array");
     switch (ctx->nvar) {
-#define X(v) case v: hipLaunchKernelGGL(solveDU_kernel<v>, dim3(nlines), dim3(128), 0, ctx->stream, \
+#define X(v) case v: hipLaunchKernelGGL(solveDU_kernel<v>, dim3(nlines), dim3(192), 0, ctx->stream, \
                    nlines, linesize, linepoint, lines, Dia, DinvC, A, dU, R); break;
         TBS_REF_NVARS(X)
 #undef X
