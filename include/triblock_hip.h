@@ -71,6 +71,15 @@ int tbs_free(tbs_ctx *ctx, void *dptr);
 int tbs_memcpy_h2d(tbs_ctx *ctx, void *dst, const void *src, size_t bytes);
 int tbs_memcpy_d2h(tbs_ctx *ctx, void *dst, const void *src, size_t bytes);
 int tbs_memset(tbs_ctx *ctx, void *dst, int value, size_t bytes);
+/* Device-to-device copy on the context's stream, finished on return: the
+ * copy behind the reference's Platform::malloc<T>(entries, occa::memory src)
+ * (include/Platform.hxx:70-75). */
+int tbs_memcpy_d2d(tbs_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* Pinned (page-locked) host memory: the reference's Platform::hostMalloc
+ * allocates with the OCCA property "host": true (include/Platform.hxx:82-88).
+ * bytes == 0 gives *hptr = NULL. */
+int tbs_host_malloc(tbs_ctx *ctx, size_t bytes, void **hptr);
+int tbs_host_free(tbs_ctx *ctx, void *hptr);
 /* Timing events on the context's stream (hipEvent_t behind an opaque handle;
  * the reference times with OCCA stream tags, src/main.cxx:181-228).  They are
  * created without the system-scope fence a default event record performs (a

@@ -1,5 +1,6 @@
 """CPU: the C-ABI library loads and exports every entry point that
 include/triblock_hip.h declares (no device work is done here)."""
+import pytest
 import os
 import re
 import subprocess
@@ -49,6 +50,23 @@ def test_plan_footprint_is_host_side(tbs):
     fp64 = tbs.Plan.footprint(m, tbs.TBS_FP64)
     fp32 = tbs.Plan.footprint(m, tbs.TBS_FP32)
     assert 8e9 < fp64 < 20e9 and fp32 < fp64
+
+
+def test_plan_refuses_lines_past_descriptor_range(tbs):
+    """A line's factor windows are addressed through a 32-bit buffer range
+    whose out-of-range base is 2^31 bytes: at nvar 16 in FP64 one window holds
+    4 line steps in 8 KiB, so a line of about 1.05M elements is the longest a
+    plan takes.  Longer lines are refused up front (TBS_ERR_UNSUPPORTED), on
+    the host, rather than addressed with a wrapped range."""
+    ok = tbs.structured_mesh(1, 1_000_000, nvar=16)
+    assert tbs.Plan.footprint(ok, tbs.TBS_FP64) > 0
+    long_line = tbs.structured_mesh(1, 1_100_000, nvar=16)
+    with pytest.raises(tbs.TbsError) as ei:
+        tbs.Plan.footprint(long_line, tbs.TBS_FP64)
+    from triblocksolve_amd._capi import TBS_ERR_UNSUPPORTED
+    assert ei.value.code == TBS_ERR_UNSUPPORTED and "too long" in str(ei.value)
+    # FP32 factors halve the window: the same line fits
+    assert tbs.Plan.footprint(long_line, tbs.TBS_FP32) > 0
 
 
 def test_variant_names(tbs):

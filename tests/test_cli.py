@@ -166,3 +166,14 @@ def test_mesh_validate_accepts_reference_fixture():
     m.fc[0] = -1
     ms = m.as_struct()
     assert tbs.load().tbs_mesh_validate(ms) == _capi.TBS_ERR_IO
+
+
+@pytest.mark.gpu
+def test_platform_memory_api(tmp_path):
+    """Platform::malloc<T>(entries, host src), malloc<T>(entries, device src),
+    the zero-filled malloc and hostMalloc (include/Platform.hxx:62-88 of the
+    reference) round-trip bit for bit (tests/cxx/platform_check.cxx)."""
+    exe = os.path.join(ROOT, "triblocksolve_amd", "platform_check.exe")
+    print("case platform_check.exe", flush=True)
+    p = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0 and "platform ok" in p.stdout, p.stdout + p.stderr

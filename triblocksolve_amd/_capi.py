@@ -75,6 +75,9 @@ SIGNATURES = {
     "tbs_memcpy_h2d": [P, P, P, C.c_size_t],
     "tbs_memcpy_d2h": [P, P, P, C.c_size_t],
     "tbs_memset": [P, P, I, C.c_size_t],
+    "tbs_memcpy_d2d": [P, P, P, C.c_size_t],
+    "tbs_host_malloc": [P, C.c_size_t, PP],
+    "tbs_host_free": [P, P],
     "tbs_event_create": [P, PP],
     "tbs_event_destroy": [P, P],
     "tbs_event_record": [P, P],

@@ -233,6 +233,39 @@ int tbs_memcpy_d2h(tbs_ctx *ctx, void *dst, const void *src, size_t bytes) {
     return TBS_OK;
 }
 
+int tbs_memcpy_d2d(tbs_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    if (bytes == 0) return TBS_OK;
+    TBS_REQUIRE(dst && src, TBS_ERR_ARG, "null pointer");
+    TBS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    TBS_HIP(hipStreamSynchronize(ctx->stream));
+    return TBS_OK;
+}
+
+int tbs_host_malloc(tbs_ctx *ctx, size_t bytes, void **hptr) {
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    TBS_REQUIRE(hptr != nullptr, TBS_ERR_ARG, "null output pointer");
+    *hptr = nullptr;
+    if (bytes == 0) return TBS_OK;
+    hipError_t e = hipHostMalloc(hptr, bytes, hipHostMallocDefault);
+    if (e == hipErrorOutOfMemory) {
+        hipGetLastError();
+        tbs::set_error("pinned host allocation of %zu bytes refused", bytes);
+        return TBS_ERR_NOMEM;
+    }
+    TBS_HIP(e);
+    return TBS_OK;
+}
+
+int tbs_host_free(tbs_ctx *ctx, void *hptr) {
+    int rc = tbs::use_ctx(ctx);
+    if (rc) return rc;
+    if (hptr) TBS_HIP(hipHostFree(hptr));
+    return TBS_OK;
+}
+
 int tbs_memset(tbs_ctx *ctx, void *dst, int value, size_t bytes) {
     int rc = tbs::use_ctx(ctx);
     if (rc) return rc;

@@ -142,6 +142,25 @@ int check_mesh(const tbs_mesh_host *m) {
     return TBS_OK;
 }
 
+// Sizes the plan's 32-bit buffer descriptors can address: whole-system
+// vectors (< 4 GiB), and per line the factor windows and the vector values
+// (< 2^31 bytes each, so the out-of-range base 2^31 stays outside them).
+int check_ranges(const tbs_mesh_host *m, int precision) {
+    TBS_REQUIRE(((size_t)m->nlineelem + 2 * kPad) * m->nvar * 8 < ((size_t)1 << 32) - 4096, TBS_ERR_UNSUPPORTED,
+                "system too large for one plan: %d elements x %d values exceeds 4 GiB per vector",
+                m->nlineelem, m->nvar);
+    const size_t wbytes = win_stride(m->nvar, precision) * tf_size(precision);
+    const int spw = 64 / m->nvar;
+    for (int l = 0; l < m->nline; ++l) {
+        const int n = m->linesize[l];
+        TBS_REQUIRE((size_t)line_windows(n, spw) * wbytes < ((size_t)1 << 31) &&
+                        (size_t)n * m->nvar * 8 < ((size_t)1 << 31),
+                    TBS_ERR_UNSUPPORTED, "line %d has %d elements: its factors exceed 2 GiB, too long for one plan",
+                    l, n);
+    }
+    return TBS_OK;
+}
+
 struct Layout {
     int F, S, ngroups, nwin;
     std::vector<int> woff;
@@ -518,6 +537,8 @@ int tbs::plan_footprint(const tbs_mesh_host *mesh, int precision, size_t *bytes,
     if (validate) TBS_TRY(check_mesh(mesh));
     else TBS_REQUIRE(mesh != nullptr && mesh->nelem >= 1 && mesh->nline >= 1, TBS_ERR_ARG, "bad mesh sizes");
     TBS_REQUIRE(precision == TBS_FP64 || precision == TBS_FP32, TBS_ERR_ARG, "bad precision");
+    TBS_REQUIRE(plan_nvar_ok(mesh->nvar), TBS_ERR_UNSUPPORTED, "nvar %d not built for the fused path", mesh->nvar);
+    TBS_TRY(check_ranges(mesh, precision));
     Layout L = layout_of(mesh);
     *bytes = footprint_of(mesh, L, precision);
     return TBS_OK;
@@ -535,11 +556,7 @@ int tbs::plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pl
     TBS_REQUIRE(m->nvar == ctx->nvar, TBS_ERR_ARG, "mesh nvar %d != context nvar %d", m->nvar, ctx->nvar);
     TBS_REQUIRE(plan_nvar_ok(m->nvar), TBS_ERR_UNSUPPORTED, "nvar %d not built for the fused path", m->nvar);
     TBS_REQUIRE(precision == TBS_FP64 || precision == TBS_FP32, TBS_ERR_ARG, "bad precision");
-    // whole-system vectors are addressed through buffer descriptors with a
-    // 32-bit byte range
-    TBS_REQUIRE(((size_t)m->nlineelem + 2 * kPad) * m->nvar * 8 < ((size_t)1 << 32) - 4096, TBS_ERR_UNSUPPORTED,
-                "system too large for one plan: %d elements x %d values exceeds 4 GiB per vector",
-                m->nlineelem, m->nvar);
+    TBS_TRY(check_ranges(m, precision));
     const int nv = m->nvar;
     Layout L = layout_of(m);
     std::unique_ptr<tbs_plan, void (*)(tbs_plan *)> p(new tbs_plan(), free_plan);

@@ -4,6 +4,7 @@
 #include "Platform.hxx"
 
 #include <cstdio>
+#include <cstring>
 
 void tbs_check(int rc, const char *what) {
     if (rc != TBS_OK)
@@ -28,3 +29,14 @@ void Platform::free(void *p, size_t bytes) {
 }
 
 void Platform::finish() { tbs_check(tbs_ctx_synchronize(ctx), "Platform::finish"); }
+
+void *Platform::hostMalloc(size_t bytes, const void *src, HostMemory &h_mem) {
+    h_mem.reset();
+    void *p = nullptr;
+    tbs_check(tbs_host_malloc(ctx, bytes, &p), "Platform::hostMalloc");
+    h_mem.ctx_ = ctx;
+    h_mem.ptr_ = p;
+    h_mem.bytes_ = bytes;
+    if (p && src) std::memcpy(p, src, bytes);
+    return p;
+}
