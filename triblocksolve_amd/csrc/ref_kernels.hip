@@ -485,44 +485,124 @@ __global__ __launch_bounds__(192) void solveDU_kernel(int nlines, const int *__r
 }
 
 // -------------------------------------------------------------- lineRes --
-// src/okl/linesmoothLU_TriBlock_v3.okl:228-336.  One thread per (element,
-// row); grid.y strides over the lines, grid.x over a line's positions.
+// src/okl/linesmoothLU_TriBlock_v3.okl:228-336: R_e += D_e U_e + the sum over
+// the element's faces (ef order, boundary entries skipped) of the face block
+// times the neighbour's U.  A wave takes W = 64 / NV consecutive positions of
+// a line at a time; lane (s, i) = (position, row).  The work of a position is
+// a sequence of rounds: round 0 is the diagonal block, round g >= 1 the block
+// of face slot g-1.  The W blocks of a round are read whole by the wave (64
+// consecutive doubles of the round's blocks per load) and staged through the
+// wave's LDS buffer with the round's U vectors; RB rounds have their loads
+// issued before the first is used.  The sums run in the reference's order.
 template <int NV>
-__global__ void lineRes_kernel(int nlines, const int *__restrict__ epoint,
-                               const int *__restrict__ ef, const int *__restrict__ fc,
-                               const int *__restrict__ linesize,
-                               const int *__restrict__ linepoint,
-                               const int *__restrict__ lines, const double *__restrict__ Dia,
-                               const double *__restrict__ Of1, const double *__restrict__ Of2,
-                               const double *__restrict__ U, double *__restrict__ R) {
-    constexpr int BS = NV * NV;
-    constexpr int EPB = (256 / NV) > 0 ? (256 / NV) : 1;   // elements per block
-    const int kl = threadIdx.x / NV, i = threadIdx.x % NV;
-    if (kl >= EPB) return;
+struct ResTile {
+    static constexpr int BS = NV * NV;
+    static constexpr int W = 64 / NV;                 // line positions per wave
+    static constexpr int NB = (W * BS + 63) / 64;     // loads per lane of a round's blocks
+    static constexpr int RB0 = 48 / NB;               // about 48 doubles of loads per lane in flight
+    static constexpr int RB = RB0 < 1 ? 1 : (RB0 > 5 ? 5 : RB0);
+};
+
+template <int NV>
+__global__ __launch_bounds__(256) void lineRes_kernel(int nlines, const int *__restrict__ epoint,
+                                                      const int *__restrict__ ef, const int *__restrict__ fc,
+                                                      const int *__restrict__ linesize,
+                                                      const int *__restrict__ linepoint,
+                                                      const int *__restrict__ lines,
+                                                      const double *__restrict__ Dia,
+                                                      const double *__restrict__ Of1,
+                                                      const double *__restrict__ Of2,
+                                                      const double *__restrict__ U, double *__restrict__ R) {
+    using T = ResTile<NV>;
+    constexpr int BS = NV * NV, W = T::W, NB = T::NB, RB = T::RB;
+    __shared__ double sblk[4][W * BS];    // per wave: a round's blocks
+    __shared__ double su[4][W * NV];      // and the U vectors they multiply
+    __shared__ uintptr_t sptr[4][RB][W];  // per round and position: block address,
+    __shared__ int snb[4][RB][W];         // U owner,
+    __shared__ int sval[4][RB][W];        // whether the round contributes
+    __shared__ int se[4][W], sep[4][W], snf[4][W];
+    using gptr = const __attribute__((address_space(1))) double *;
+    auto wsync = []() { __builtin_amdgcn_wave_barrier(); };
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int ps = lane / NV, i = lane - ps * NV;      // the lane's position and row
+    const bool act = lane < W * NV;
     for (int l = blockIdx.y; l < nlines; l += gridDim.y) {
-    const int n = linesize[l];
-    const int m0 = linepoint[l];
-    for (int k = blockIdx.x * EPB + kl; k < n; k += gridDim.x * EPB) {
-        const int e = lines[m0 + k];
-        double tot = 0.0;                                             // :276-293
-#pragma unroll
-        for (int j = 0; j < NV; ++j) tot += Dia[(size_t)BS * e + i + NV * j] * U[(size_t)NV * e + j];
-        double sR = tot;
-        for (int k2 = epoint[e]; k2 < epoint[e + 1]; ++k2) {          // :299-328
-            const int f = ef[k2];
-            if (f >= 0) {
-                const int e1 = fc[2 * f + 0];
-                const int e2 = fc[2 * f + 1];
-                const double *off = (e == e1) ? Of2 : Of1;
-                const int nb = (e == e1) ? e2 : e1;
-                double t2 = 0.0;
-#pragma unroll
-                for (int j = 0; j < NV; ++j) t2 += off[(size_t)BS * f + i + NV * j] * U[(size_t)NV * nb + j];
-                sR += t2;
+        const int n = __builtin_amdgcn_readfirstlane(linesize[l]);
+        const int m0 = __builtin_amdgcn_readfirstlane(linepoint[l]);
+        const int nch = (n + W - 1) / W;
+        for (int c = blockIdx.x * 4 + wave; c < nch; c += gridDim.x * 4) {
+            const int nval = min(W, n - c * W);          // positions of this chunk
+            if (lane < W) {
+                const int e = lines[m0 + c * W + min(lane, nval - 1)];
+                const int ep = epoint[e];
+                se[wave][lane] = e;
+                sep[wave][lane] = ep;
+                snf[wave][lane] = lane < nval ? epoint[e + 1] - ep : 0;
             }
+            wsync();
+            int maxnf = 0;
+#pragma unroll
+            for (int s = 0; s < W; ++s) maxnf = max(maxnf, snf[wave][s]);
+            maxnf = __builtin_amdgcn_readfirstlane(maxnf);
+            double acc = 0.0;
+            for (int g0 = 0; g0 <= maxnf; g0 += RB) {
+                // the rounds' tables (a skipped slot points at the diagonal
+                // block, so every load below stays in bounds)
+                for (int t = lane; t < RB * W; t += 64) {
+                    const int r = t / W, s = t - r * W, g = g0 + r;
+                    const int e = se[wave][s];
+                    uintptr_t p = (uintptr_t)(Dia + (size_t)BS * e);
+                    int nb = e, ok = (g == 0 && s < nval) ? 1 : 0;
+                    if (g >= 1 && g - 1 < snf[wave][s]) {
+                        const int f = ef[sep[wave][s] + g - 1];                      // :299-328
+                        if (f >= 0) {
+                            const int e1 = fc[2 * f + 0], e2 = fc[2 * f + 1];
+                            p = (uintptr_t)(((e == e1) ? Of2 : Of1) + (size_t)BS * f);
+                            nb = (e == e1) ? e2 : e1;
+                            ok = 1;
+                        }
+                    }
+                    sptr[wave][r][s] = p;
+                    snb[wave][r][s] = nb;
+                    sval[wave][r][s] = ok;
+                }
+                wsync();
+                double rb[RB][NB], ru[RB];
+#pragma unroll
+                for (int r = 0; r < RB; ++r) {
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) {
+                        const int t = min(lane + 64 * j, W * BS - 1);
+                        const int s = t / BS, q = t - s * BS;
+                        rb[r][j] = ((gptr)sptr[wave][r][s])[q];
+                    }
+                    const int t = min(lane, W * NV - 1);
+                    const int s = t / NV, q = t - s * NV;
+                    ru[r] = ((gptr)U)[(size_t)NV * snb[wave][r][s] + q];
+                }
+#pragma unroll
+                for (int r = 0; r < RB; ++r) {
+                    if (g0 + r > maxnf) break;
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) {
+                        const int t = lane + 64 * j;
+                        if (t < W * BS) sblk[wave][t] = rb[r][j];
+                    }
+                    if (act) su[wave][lane] = ru[r];
+                    wsync();
+                    if (act && sval[wave][r][ps]) {
+                        double t2 = 0.0;                                             // :276-293, :306-327
+#pragma unroll
+                        for (int j = 0; j < NV; ++j) t2 += sblk[wave][ps * BS + i + NV * j] * su[wave][ps * NV + j];
+                        acc = (g0 + r == 0) ? t2 : acc + t2;
+                    }
+                    wsync();
+                }
+            }
+            if (act && ps < nval) R[(size_t)NV * se[wave][ps] + i] += acc;      // :331
+            wsync();
         }
-        R[(size_t)NV * e + i] += sR;                                  // :331
-    }
     }
 }
 
@@ -788,9 +868,8 @@ int tbs_triblock_lineRes(tbs_ctx *ctx, int nelem, int nintfaces, int eftot, int 
                 "null array");
     TBS_REQUIRE(nintfaces == 0 || (fc && Of1 && Of2), TBS_ERR_ARG, "null face array");
     const int nv = ctx->nvar;
-    const int epb = (256 / nv) > 0 ? 256 / nv : 1;
     switch (nv) {
-#define X(v) case v: hipLaunchKernelGGL(lineRes_kernel<v>, dim3(8, std::min(nlines, 65535)), dim3(epb * v), 0, ctx->stream, \
+#define X(v) case v: hipLaunchKernelGGL(lineRes_kernel<v>, dim3(8, std::min(nlines, 65535)), dim3(256), 0, ctx->stream, \
                    nlines, epoint, ef, fc, linesize, linepoint, lines, Dia, Of1, Of2, U, R); break;
         TBS_REF_NVARS(X)
 #undef X
