@@ -156,6 +156,18 @@ def hist_dev(h, ref, floor=1e-4):
     return float(np.max(np.abs(a - b) / np.maximum(b, floor)))
 
 
+def tuning_tuple(t):
+    """The autotuner's choice: Thomas prefetch windows, LDS windows per group
+    and lines per workgroup, residual waves per workgroup, two-stream split."""
+    return {k: t[k] for k in ("thomas_prefetch", "thomas_lds_windows", "thomas_lines_per_wg",
+                              "residual_waves_per_wg", "two_stream")}
+
+
+def residual_by_rw(t):
+    """Residual launch time (us) per workgroup shape, from the last autotune."""
+    return {str(w): us for w, us in zip(t["residual_waves_per_wg_by_variant"], t["residual_us"])}
+
+
 def bench_batched(tbs, ctx, args, seed=3, oracle=None):
     """64 independent systems of uneven lines (96..1280 elements, 8..40 lines
     per system, 384,000 elements in all) advanced by one launch per kernel per
@@ -193,6 +205,7 @@ def bench_batched(tbs, ctx, args, seed=3, oracle=None):
         t_factor = time.perf_counter() - t0
         b = tbs.Batch(ctx, plans)
         chosen, us = b.autotune(reps=3)
+        tuning = b.tuning()
         model = sum(p.info()["alg_bytes_per_sweep"] for p in plans)
         steps = args.steps
         for p, (_, _, _, rhs) in zip(plans, bufs):
@@ -219,7 +232,8 @@ def bench_batched(tbs, ctx, args, seed=3, oracle=None):
         t_solve = time.perf_counter() - t1
         res[name] = {"sweeps_per_s": steps / t, "gbps_model": model / (t / steps) / 1e9,
                      "frac_hbm_peak": model / (t / steps) / 1e9 / HBM_PEAK_GBS, "bytes_per_sweep_model": model,
-                     "autotune_variant": int(chosen), "factor_s": t_factor, "solve_to_1e-10_s": t_solve,
+                     "autotune_variant": int(chosen), "autotune_tuple": tuning_tuple(tuning),
+                     "autotune_pass_us_one_vs_two_streams": tuning["pass_us"], "factor_s": t_factor, "solve_to_1e-10_s": t_solve,
                      "time_to_1e-10_s": t_factor + t_solve, "max_sweeps_to_1e-10": int(max(ns)),
                      "worst_final_rel_residual": float(max(h[-1] / h[0] for h in hists))}
         if refs:
@@ -531,7 +545,7 @@ def main():
         ctx.sync()
         t_factor = time.perf_counter() - t0
         info = plan.info()
-        tuned = plan.autotune(reps=3)
+        tuned = plan.autotune(reps=3) + (plan.tuning(),)
         plan.begin(rhs, None)
         for _ in range(warmup):
             plan.launch_thomas()
@@ -650,7 +664,9 @@ def main():
         "timed_path": "hipGraph of the K sweeps (tbs_plan_launch_sweeps); eager_sweeps_per_s: the same sweeps "
                       "launched one kernel at a time with timing events between them",
         "factor_s": r64["t_factor"],
-        "autotune": {"chosen_variant": r64["tuned"][0], "thomas_us_by_variant": r64["tuned"][1]},
+        "autotune": {"chosen_variant": r64["tuned"][0], "thomas_us_by_variant": r64["tuned"][1],
+                     "tuple": tuning_tuple(r64["tuned"][2]),
+                     "residual_us_by_waves_per_wg": residual_by_rw(r64["tuned"][2])},
     }
     log("stream ceiling")
     ceil = ctx.stream_ceiling()

@@ -227,14 +227,41 @@ int tbs_plan_solve(tbs_plan *plan, const double *rhs, double *U,
                    int max_sweeps, double rtol, double *hist, int *nsweeps,
                    double *R_final);
 
-/* Autotuner: time every built Thomas prefetch-depth variant on this plan
- * (reps launches each, HIP events), keep the fastest.  Resets the iterate:
- * call before tbs_plan_begin / tbs_plan_solve.  us[v] = mean microseconds.  */
+/* Autotuner (no counterpart in the reference, whose OKL kernels take their
+ * tiling from JIT defines): times every compiled variant of the sweep's
+ * kernels on this plan's own data (reps launches each, HIP events) and keeps
+ * the fastest of each: the Thomas variant (factor windows prefetched per
+ * wave, windows staged per LDS group, lines per workgroup) and the residual
+ * workgroup shape (64-element groups per workgroup).  Every variant gives
+ * bitwise the same results.  Resets the iterate: call before tbs_plan_begin /
+ * tbs_plan_solve.  chosen = the Thomas variant; us[v] = mean microseconds of
+ * Thomas variant v (the whole record: tbs_plan_tuning).  */
 int tbs_plan_autotune(tbs_plan *plan, int reps, int *chosen, double *us, int cap);
 int tbs_plan_set_variant(tbs_plan *plan, int variant);
 /* Number of Thomas variants; pf[v] = 100 * (windows prefetched) + (ring
- * cycles unrolled per loop iteration) of variant v. */
+ * cycles per loop iteration) + 10000 * (lines per workgroup - 1). */
 int tbs_plan_variants(tbs_plan *plan, int *pf, int cap);
+
+/* The tuple the autotuner chose (or set by hand), with its timings. */
+#define TBS_TUNE_THOMAS_MAX 8
+#define TBS_TUNE_RESIDUAL_MAX 4
+typedef struct {
+    int thomas_variant;        /* index of the Thomas variant */
+    int thomas_prefetch;       /* factor windows each wave keeps in flight */
+    int thomas_lds_windows;    /* windows whose vectors a wave stages in LDS per group */
+    int thomas_lines_per_wg;   /* lines per Thomas workgroup (two waves each) */
+    int residual_variant;      /* index of the residual shape */
+    int residual_waves_per_wg; /* 64-element groups (waves) per residual workgroup */
+    int two_stream;            /* batches: eager sweeps as two halves on two streams */
+    int n_thomas, n_residual;  /* variants compiled for this block size and precision */
+    int residual_shapes[TBS_TUNE_RESIDUAL_MAX];  /* waves per workgroup of each residual variant */
+    double thomas_us[TBS_TUNE_THOMAS_MAX];     /* last autotune: mean launch time per variant */
+    double residual_us[TBS_TUNE_RESIDUAL_MAX]; /* (0 = not timed) */
+    double pass_us[2];         /* batches: Thomas + residual pass, one stream / two streams */
+} tbs_tuning;
+int tbs_plan_tuning(tbs_plan *plan, tbs_tuning *out);
+/* Set thomas_variant, residual_variant (two_stream is ignored for a plan). */
+int tbs_plan_set_tuning(tbs_plan *plan, const tbs_tuning *t);
 
 /* Low-level stepping, for timing and graph capture (all asynchronous). */
 int tbs_plan_begin(tbs_plan *plan, const double *rhs, const double *U0);
@@ -270,7 +297,13 @@ int tbs_batch_destroy(tbs_batch *batch);
  * descriptor tables are rewritten in place on the batch's stream, ordered
  * after the sweeps already enqueued; device tables grow when needed. */
 int tbs_batch_set_plans(tbs_batch *batch, tbs_plan *const *plans, int nplans);
+/* The plan autotuner's search over all systems of the batch at once, plus
+ * the two-stream split of eager sweeps (timed both ways with the chosen
+ * kernels). */
 int tbs_batch_autotune(tbs_batch *batch, int reps, int *chosen, double *us, int cap);
+int tbs_batch_tuning(tbs_batch *batch, tbs_tuning *out);
+/* Set thomas_variant, residual_variant and two_stream. */
+int tbs_batch_set_tuning(tbs_batch *batch, const tbs_tuning *t);
 /* Enqueue nsweeps sweeps of every still-active system (asynchronous); the
  * plans must have been started with tbs_plan_begin.                        */
 int tbs_batch_launch_sweeps(tbs_batch *batch, int nsweeps, int max_sweeps, double rtol);

@@ -74,3 +74,4 @@ def test_variant_names(tbs):
     compiled template instance the autotuner reports."""
     assert tbs.variant_name(302) == "pf3_k2"
     assert tbs.variant_name(1001) == "pf10_k1"
+    assert tbs.variant_name(10302) == "pf3_k2_l2"   # two lines per workgroup

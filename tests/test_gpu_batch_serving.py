@@ -60,6 +60,58 @@ def test_uneven_batch_autotuned_graph(tbs, ctx, oracle, prec, one_launch):
             assert np.linalg.norm(U - ref["U"]) <= 1e-12 * np.linalg.norm(ref["U"])
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_tuning_tuples_bitwise(tbs, ctx, prec):
+    """Tuples the autotuner can pick: two lines per Thomas workgroup (lines of
+    different systems in one workgroup, some systems converged while the
+    others sweep on), residual workgroups of 1 and 8 waves, eager sweeps on one
+    stream and as two halves on two streams, and the graph-captured solve: every
+    system's history and U stay bitwise those of its solo solve."""
+    print(f"case batch tuning tuples prec {prec}", flush=True)
+    shapes = tbs.batch_shapes(6, nsys=5, total=12000, lmin=64, lmax=700, nlmin=3, nlmax=7)
+    meshes = [tbs.ragged_mesh(s_) for s_ in shapes]
+    # different diagonal shifts: the systems converge after different sweeps
+    jacs = [tbs.generate_host(200 + i, 9, m.nelem, m.nintface, [12.0, 30.0, 10.0, 20.0, 50.0][i])
+            for i, m in enumerate(meshes)]
+    alone = [_solve_alone(tbs, ctx, m, j, prec, 60, 1e-10) for m, j in zip(meshes, jacs)]
+    plans = [tbs.Plan(ctx, m, prec) for m in meshes]
+    tbs.factor_plans(ctx, plans, *zip(*[(dev(j["jacD"]), dev(j["jacO1"]), dev(j["jacO2"])) for j in jacs]))
+    rhs = [dev(j["rhs"]) for j in jacs]
+    b = tbs.Batch(ctx, plans)
+    codes = plans[0].variants()
+    lpw2 = next(v for v, c in enumerate(codes) if c >= 10000)
+    rw = {w: r for r, w in enumerate(b.tuning()["residual_waves_per_wg_by_variant"])}
+    tuples = [dict(thomas_variant=lpw2, residual_variant=rw[8], two_stream=1),
+              dict(thomas_variant=lpw2, residual_variant=rw[1], two_stream=0),
+              dict(thomas_variant=0, residual_variant=rw[8], two_stream=1)]
+    for t in tuples:
+        b.set_tuning(**t)
+        got = b.tuning()
+        assert all(got[k] == v for k, v in t.items()), (t, got)
+        assert got["thomas_lines_per_wg"] == (2 if t["thomas_variant"] == lpw2 else 1)
+        # eager sweeps (two halves on two streams when two_stream is set)
+        for p, r_ in zip(plans, rhs):
+            p.begin(r_, None)
+        for _ in range(40):
+            b.launch_sweeps(3, 60, 1e-10)
+            if not b.states()[:, 1].any():
+                break
+        for i, (m, p) in enumerate(zip(meshes, plans)):
+            U = zeros(9 * m.nelem)
+            p.end(U)
+            h0, n0, U0 = alone[i]
+            assert np.array_equal(p.history(), h0) and np.array_equal(host(U), U0), (t, i)
+        # the graph-captured solve
+        Us = [zeros(9 * m.nelem) for m in meshes]
+        hists, ns = b.solve(rhs, Us, 60, rtol=1e-10, sweeps_per_check=3)
+        for i in range(len(meshes)):
+            h0, n0, U0 = alone[i]
+            assert ns[i] == n0 and np.array_equal(hists[i], h0) and np.array_equal(host(Us[i]), U0), (t, i)
+    b.close()
+    for p in plans:
+        p.close()
+
+
 def test_server_results_equal_solo_solves(tbs, ctx, oracle):
     print("case server mix", flush=True)
     specs = [(3, 64, 11), (40, 96, 12), (7, 960, 13), (120, 320, 14), (1, 64, 15), (60, 64, 16), (25, 500, 17)]

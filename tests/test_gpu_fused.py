@@ -34,11 +34,13 @@ def ctxs(tbs):
         c.close()
 
 
-def solve_plan(tbs, ctx, m, jac, iters, rtol=0.0, prec=0, want_R=False, variant=None):
+def solve_plan(tbs, ctx, m, jac, iters, rtol=0.0, prec=0, want_R=False, variant=None, rvariant=None):
     plan = tbs.Plan(ctx, m, prec)
     plan.factor(dev(jac["jacD"]), dev(jac["jacO1"]), dev(jac["jacO2"]))
     if variant is not None:
         plan.set_variant(variant)
+    if rvariant is not None:
+        plan.set_tuning(residual_variant=rvariant)
     U = zeros(m.nvar * m.nelem)
     R = zeros(U.size) if want_R else None
     hist, ns = plan.solve(dev(jac["rhs"]), U, iters, rtol, R_out=R)
@@ -87,20 +89,26 @@ def test_variants_fp64(tbs, ctxs, oracle, name, make):
 
 @pytest.mark.parametrize("prec", [0, 1])
 def test_thomas_variants_bitwise(tbs, ctxs, oracle, prec):
-    """Every compiled Thomas variant (prefetch depth, ring unroll) does the same
-    arithmetic in the same order: history and U are bitwise equal across
-    variants, on ragged lines (ends inside a window, lines of one element,
-    lines longer than a window group), and the FP64 ones match the oracle."""
+    """Every compiled Thomas variant (prefetch depth, LDS windows, lines per
+    workgroup) and residual workgroup shape does the same arithmetic in the
+    same order: history and U are bitwise equal across variants, on ragged
+    lines (ends inside a window, lines of one element, lines longer than a
+    window group, an odd line count so a two-line workgroup has an empty
+    slot), and the FP64 ones match the oracle."""
     nv = 9                                     # the block size with several compiled variants
-    m = ragged_lines([1, 2, 17, 64, 3, 100, 1, 65, 7, 8, 9, 130], nv)
+    m = ragged_lines([1, 2, 17, 64, 3, 100, 1, 65, 7, 8, 9, 130, 5], nv)
     jac = synth_np.generate(40 + nv, nv, m.nelem, m.nintface, 2.0 * nv + 2)
     ctx = ctxs(nv)
     probe = tbs.Plan(ctx, m, prec)
     names = [tbs.variant_name(c) for c in probe.variants()]
+    nres = len(probe.tuning()["residual_us"])
     probe.close()
-    print(f"case prec {prec}: variants {names}", flush=True)
-    assert len(names) > 1
+    print(f"case prec {prec}: variants {names}, {nres} residual shapes", flush=True)
+    assert len(names) > 1 and nres > 1
+    assert any(n.endswith("_l2") for n in names)
     outs = [solve_plan(tbs, ctx, m, jac, 8, prec=prec, variant=v) for v in range(len(names))]
+    outs += [solve_plan(tbs, ctx, m, jac, 8, prec=prec, rvariant=r) for r in range(1, nres)]
+    names += [f"residual_{r}" for r in range(1, nres)]
     for name, o in zip(names[1:], outs[1:]):
         assert np.array_equal(o["hist"], outs[0]["hist"]), name
         assert np.array_equal(o["U"], outs[0]["U"]), name

@@ -51,6 +51,18 @@ class PlanInfo(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    """tbs_tuning: the tuple the autotuner chose, with its timings."""
+    _fields_ = [(n, C.c_int) for n in ("thomas_variant", "thomas_prefetch", "thomas_lds_windows",
+                                       "thomas_lines_per_wg", "residual_variant", "residual_waves_per_wg",
+                                       "two_stream", "n_thomas", "n_residual")] + [
+        ("residual_shapes", C.c_int * 4),
+        ("thomas_us", C.c_double * 8),
+        ("residual_us", C.c_double * 4),
+        ("pass_us", C.c_double * 2),
+    ]
+
+
 P = C.c_void_p
 I = C.c_int
 L = C.c_long
@@ -101,6 +113,8 @@ SIGNATURES = {
     "tbs_plans_factor": [P, P, C.c_int, P, P, P],
     "tbs_plan_solve": [P, P, P, I, D, P, C.POINTER(C.c_int), P],
     "tbs_plan_autotune": [P, I, C.POINTER(C.c_int), P, I],
+    "tbs_plan_tuning": [P, P],
+    "tbs_plan_set_tuning": [P, P],
     "tbs_plan_set_variant": [P, I],
     "tbs_plan_variants": [P, P, I],
     "tbs_plan_begin": [P, P, P],
@@ -115,6 +129,8 @@ SIGNATURES = {
     "tbs_batch_destroy": [P],
     "tbs_batch_set_plans": [P, P, I],
     "tbs_batch_autotune": [P, I, C.POINTER(C.c_int), P, I],
+    "tbs_batch_tuning": [P, P],
+    "tbs_batch_set_tuning": [P, P],
     "tbs_batch_solve": [P, P, P, I, D, I, P, P],
     "tbs_batch_launch_sweeps": [P, I, I, D],
     "tbs_batch_states": [P, P],

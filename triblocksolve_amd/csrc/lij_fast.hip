@@ -38,7 +38,9 @@ struct tbs_plan {
     int max_sweeps = kHistCap - 1;
     double rtol = 0.0;
     int solves = 0;           // tbs_plan_solve calls so far
-    int variant = 0;          // Thomas prefetch variant (autotuned)
+    int variant = 0;          // Thomas variant (autotuned): prefetch, LDS windows, lines per workgroup
+    int rvariant = 0;         // residual workgroup shape (autotuned)
+    double t_us[kTuneThomas] = {}, r_us[kTuneResidual] = {};   // last autotune's timings
     size_t bytes = 0;
     int *h_state = nullptr;      // pinned unless region-backed
     bool in_region = false;      // device arrays carved from a context region
@@ -54,14 +56,16 @@ struct tbs_batch {
     int4 *d_ltab = nullptr;
     int2 *d_gtab = nullptr;
     int nl = 0, ng = 0;
-    // eager FP64 sweeps run in two halves on the context's two streams: systems
+    // eager sweeps may run in two halves on the context's two streams: systems
     // [0, nsA) with lines [0, nlA) and groups [0, ngA) of the tables, the
-    // rest after them (nsA = 0: one stream)
+    // rest after them (nsA = 0: one system, one stream)
     int nsA = 0, nlA = 0, ngA = 0;
     std::map<std::pair<int, std::pair<int, double>>, hipGraphExec_t> graphs;
     int *h_states = nullptr;    // pinned [cap_sys][4]
     int *d_states = nullptr;    // [cap_sys][4] gathered system states
-    int variant = 0;
+    int variant = 0, rvariant = 0;
+    bool split = true;          // eager sweeps in two halves on two streams (autotuned)
+    double t_us[kTuneThomas] = {}, r_us[kTuneResidual] = {}, pass_us[2] = {};
     // capacities of the device tables (entries) and of the pinned staging
     size_t cap_sys = 0, cap_l = 0, cap_g = 0, cap_stage = 0;
     char *h_stage = nullptr;    // pinned image of the tables being uploaded
@@ -302,12 +306,12 @@ int thomas_variants(int nv, int prec) {
     return 1;
 }
 
-// encoded as 100 * PF + K
+// encoded as 100 * PF + K + 10000 * (lines per workgroup - 1)
 int thomas_pf_of(int nv, int prec, int variant) {
-    TV t = {2, 2};
+    TV t = {nv > 16 ? 1 : 2, 2, 1};
     if (nv == 9) t = prec == TBS_FP32 ? ThomasV<9, float>::v[variant % ThomasV<9, float>::n]
                                       : ThomasV<9, double>::v[variant % ThomasV<9, double>::n];
-    return 100 * t.pf + t.k;
+    return 100 * t.pf + t.k + 10000 * (t.lpw - 1);
 }
 
 // Dispatch on (nvar, precision) for the fused kernels: 2..16 instantiated
@@ -388,10 +392,10 @@ int launch_pack(const tbs_plan *p, hipStream_t pst, const double *jacD, const do
 }
 
 int k_residual(int nv, int prec, hipStream_t st, const void *sys, const int2 *gtab, int ng,
-               int want_R, int force, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+               int want_R, int force, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, int rvariant = 0) {
     switch (nv) {
-#define X(v) case v: return prec == TBS_FP32 ? launch_residual<v, float>(st, sys, gtab, ng, want_R, force, e0, e1) \
-                                         : launch_residual<v, double>(st, sys, gtab, ng, want_R, force, e0, e1);
+#define X(v) case v: return prec == TBS_FP32 ? launch_residual<v, float>(st, sys, gtab, ng, want_R, force, e0, e1, rvariant) \
+                                         : launch_residual<v, double>(st, sys, gtab, ng, want_R, force, e0, e1, rvariant);
         TBS_PLAN_NVARS(X)
 #undef X
     }
@@ -435,9 +439,10 @@ int batch_sweeps(tbs_batch *b, hipStream_t st, int nsweeps, int max_sweeps, doub
 // One sweep of the systems the tables ltab[0, nl) and gtab[0, ng) cover
 // (they index sys by system); the norm step runs over sys_final[0, nsys).
 int enqueue_sweep(int nv, int prec, hipStream_t st, const void *sys, const void *sys_final, int nsys,
-                  const int4 *ltab, int nl, const int2 *gtab, int ng, int max_sweeps, double rtol, int variant) {
+                  const int4 *ltab, int nl, const int2 *gtab, int ng, int max_sweeps, double rtol, int variant,
+                  int rvariant) {
     TBS_TRY(k_thomas(nv, prec, st, sys, ltab, nl, variant));
-    TBS_TRY(k_residual(nv, prec, st, sys, gtab, ng, 0, 0));
+    TBS_TRY(k_residual(nv, prec, st, sys, gtab, ng, 0, 0, nullptr, nullptr, rvariant));
     TBS_TRY(k_final(prec, st, sys_final, nsys, max_sweeps, rtol, 0));
     return TBS_OK;
 }
@@ -486,7 +491,8 @@ int plan_vecs(tbs_plan *p, PlanVecs *out) {
 int plan_activate(tbs_plan *p, hipStream_t st) { return k_reset(p->prec, st, p->d_sys, 1); }
 
 int plan_residual(tbs_plan *p, hipStream_t st, int want_R, bool u_zero) {
-    return k_residual(p->nv, p->prec, st, p->d_sys, p->d_gtab, p->ngroups, want_R, u_zero ? 3 : 1);
+    return k_residual(p->nv, p->prec, st, p->d_sys, p->d_gtab, p->ngroups, want_R, u_zero ? 3 : 1, nullptr, nullptr,
+                      p->rvariant);
 }
 
 int plan_thomas(tbs_plan *p, hipStream_t st) {
@@ -787,16 +793,22 @@ int tbs_plan_begin(tbs_plan *p, const double *rhs, const double *U0) {
     TBS_REQUIRE(rhs, TBS_ERR_ARG, "null rhs");
     hipStream_t st = p->ctx->stream;
     const long total = (long)p->nle * p->nv;
+    TBS_TRY(k_reset(p->prec, st, p->d_sys, 1));
     hipLaunchKernelGGL(gather_vec_kernel, dim3(gs_grid(total)), dim3(256), 0, st, total, p->nv,
                        p->d_lines, rhs, p->d_rhs);
     TBS_LAUNCHED("gather rhs");
-    hipLaunchKernelGGL(gather_vec_kernel, dim3(gs_grid(total)), dim3(256), 0, st, total, p->nv,
-                       p->d_lines, U0, p->d_U);
+    if (U0) {
+        hipLaunchKernelGGL(gather_u0_kernel, dim3(gs_grid(total)), dim3(256), 0, st, total, p->nv,
+                           p->d_lines, U0, p->d_U, p->d_state + 3);
+    } else {
+        hipLaunchKernelGGL(gather_vec_kernel, dim3(gs_grid(total)), dim3(256), 0, st, total, p->nv,
+                           p->d_lines, U0, p->d_U);
+    }
     TBS_LAUNCHED("gather U0");
-    TBS_TRY(k_reset(p->prec, st, p->d_sys, 1));
-    // R_0 = rhs + J U_0 (= rhs when U_0 = 0, the reference's start; then J U
-    // is skipped) and ||R_0||
-    TBS_TRY(k_residual(p->nv, p->prec, st, p->d_sys, p->d_gtab, p->ngroups, 0, U0 ? 1 : 3));
+    // R_0 = rhs + J U_0 (= rhs when U_0 = 0, the reference's start: then J U
+    // is skipped, also for a given U_0 whose values are all zero) and ||R_0||
+    TBS_TRY(k_residual(p->nv, p->prec, st, p->d_sys, p->d_gtab, p->ngroups, 0, U0 ? 1 | 4 : 3, nullptr, nullptr,
+                       p->rvariant));
     TBS_TRY(k_final(p->prec, st, p->d_sys, 1, p->max_sweeps, p->rtol, 1));
     return TBS_OK;
 }
@@ -820,7 +832,8 @@ int tbs_plan_launch_kernel(tbs_plan *p, int step, void *ev_start, void *ev_stop)
     hipEvent_t e0 = (hipEvent_t)ev_start, e1 = (hipEvent_t)ev_stop;
     switch (step) {
         case TBS_STEP_THOMAS: return k_thomas(p->nv, p->prec, st, p->d_sys, p->d_ltab, p->nline, p->variant, e0, e1);
-        case TBS_STEP_RESIDUAL: return k_residual(p->nv, p->prec, st, p->d_sys, p->d_gtab, p->ngroups, 0, 0, e0, e1);
+        case TBS_STEP_RESIDUAL:
+            return k_residual(p->nv, p->prec, st, p->d_sys, p->d_gtab, p->ngroups, 0, 0, e0, e1, p->rvariant);
         case TBS_STEP_NORM: return k_final(p->prec, st, p->d_sys, 1, p->max_sweeps, p->rtol, 0, e0, e1);
     }
     tbs::set_error("unknown sweep step %d", step);
@@ -833,7 +846,7 @@ int tbs_plan_launch_sweeps(tbs_plan *p, int nsweeps, int use_graph) {
     hipStream_t st = p->ctx->stream;
     auto one = [p, st]() {
         return enqueue_sweep(p->nv, p->prec, st, p->d_sys, p->d_sys, 1, p->d_ltab, p->nline, p->d_gtab,
-                             p->ngroups, p->max_sweeps, p->rtol, p->variant);
+                             p->ngroups, p->max_sweeps, p->rtol, p->variant, p->rvariant);
     };
     if (!use_graph) {
         for (int i = 0; i < nsweeps; ++i) TBS_TRY(one());
@@ -861,7 +874,7 @@ int tbs_plan_end(tbs_plan *p, double *U, double *R_final) {
         TBS_LAUNCHED("scatter U");
     }
     if (R_final) {
-        TBS_TRY(k_residual(p->nv, p->prec, st, p->d_sys, p->d_gtab, p->ngroups, 1, 1));
+        TBS_TRY(k_residual(p->nv, p->prec, st, p->d_sys, p->d_gtab, p->ngroups, 1, 1, nullptr, nullptr, p->rvariant));
         hipLaunchKernelGGL(scatter_vec_kernel, dim3(gs_grid(total)), dim3(256), 0, st, total, p->nv,
                            p->d_lines, p->d_R, R_final);
         TBS_LAUNCHED("scatter R");
@@ -904,44 +917,61 @@ int tbs_plan_internal(tbs_plan *p, int which, void **dptr, size_t *bytes) {
 }  // extern "C"
 
 namespace {
-// Time each Thomas variant on zeroed vectors (the kernel's cost does not
-// depend on the values) and return the fastest.  Leaves the systems' state
+int residual_variants(int nv) { return nv == 9 ? ResidualV<9>::n : 1; }
+int residual_rw_of(int nv, int rvariant) { return nv == 9 ? ResidualV<9>::rw[rvariant % ResidualV<9>::n] : 4; }
+
+// Mean microseconds of one call of fn over reps calls on st (after one
+// warm-up call), from two timing events.
+int time_calls(hipStream_t st, int reps, const std::function<int()> &fn, double *us) {
+    TBS_TRY(fn());
+    hipEvent_t e0, e1;
+    TBS_HIP(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));   // timing only
+    TBS_HIP(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
+    int rc = TBS_OK;
+    if (hipEventRecord(e0, st)) rc = TBS_ERR_STATE;
+    for (int r = 0; r < reps && rc == TBS_OK; ++r) rc = fn();
+    if (rc == TBS_OK && (hipEventRecord(e1, st) || hipEventSynchronize(e1))) rc = TBS_ERR_STATE;
+    float ms = 0.f;
+    if (rc == TBS_OK) hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc == TBS_ERR_STATE) tbs::set_error("timing events failed");
+    *us = 1e3 * ms / reps;
+    return rc;
+}
+
+// Time every compiled Thomas variant and residual shape on zeroed vectors
+// (the kernels' cost does not depend on the values) over the systems the
+// tables cover, and return the fastest of each.  Leaves the systems' state
 // reset; callers start a solve with begin() afterwards.
-int autotune_thomas(tbs_ctx *ctx, int nv, int prec, const void *sys, int nsys, const int4 *ltab,
-                    int nl, const std::vector<std::pair<double *, long>> &zero_vecs, int reps,
-                    int *chosen, double *us, int cap) {
+int autotune_kernels(tbs_ctx *ctx, int nv, int prec, const void *sys, int nsys, const int4 *ltab, int nl,
+                     const int2 *gtab, int ng, const std::vector<std::pair<double *, long>> &zero_vecs, int reps,
+                     int *tbest, double *t_us, int *rbest, double *r_us) {
     hipStream_t st = ctx->stream;
     for (auto &zv : zero_vecs) {
         hipLaunchKernelGGL(zero_kernel, dim3(gs_grid(zv.second)), dim3(256), 0, st, zv.second, zv.first);
         TBS_LAUNCHED("zero_kernel");
     }
     TBS_TRY(k_reset(prec, st, sys, nsys));
-    const int nvar = thomas_variants(nv, prec);
-    hipEvent_t e0, e1;
-    TBS_HIP(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));   // timing only
-    TBS_HIP(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
-    int best = 0;
     double best_t = 1e300;
-    int rc = TBS_OK;
-    for (int v = 0; v < nvar && rc == TBS_OK; ++v) {
-        rc = k_thomas(nv, prec, st, sys, ltab, nl, v);     // warm-up
-        if (rc) break;
-        if (hipEventRecord(e0, st)) { rc = TBS_ERR_STATE; break; }
-        for (int r = 0; r < reps && rc == TBS_OK; ++r) rc = k_thomas(nv, prec, st, sys, ltab, nl, v);
-        if (rc) break;
-        if (hipEventRecord(e1, st) || hipEventSynchronize(e1)) { rc = TBS_ERR_STATE; break; }
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, e0, e1);
-        const double t = 1e3 * ms / reps;
-        if (us && v < cap) us[v] = t;
-        if (t < best_t) { best_t = t; best = v; }
+    *tbest = 0;
+    for (int v = 0; v < thomas_variants(nv, prec) && v < kTuneThomas; ++v) {
+        double us = 0.0;
+        TBS_TRY(time_calls(st, reps, [&]() { return k_thomas(nv, prec, st, sys, ltab, nl, v); }, &us));
+        t_us[v] = us;
+        if (us < best_t) { best_t = us; *tbest = v; }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    if (rc) return rc;
+    best_t = 1e300;
+    *rbest = 0;
+    for (int v = 0; v < residual_variants(nv) && v < kTuneResidual; ++v) {
+        double us = 0.0;
+        TBS_TRY(time_calls(st, reps, [&]() { return k_residual(nv, prec, st, sys, gtab, ng, 0, 1, nullptr, nullptr, v); },
+                           &us));
+        r_us[v] = us;
+        if (us < best_t) { best_t = us; *rbest = v; }
+    }
     TBS_TRY(k_reset(prec, st, sys, nsys));
     TBS_HIP(hipStreamSynchronize(st));
-    if (chosen) *chosen = best;
     return TBS_OK;
 }
 
@@ -957,7 +987,7 @@ int batch_sweeps(tbs_batch *b, hipStream_t st, int nsweeps, int max_sweeps, doub
     if (!split || b->nsA <= 0 || b->nsA >= ns) {
         for (int i = 0; i < nsweeps; ++i)
             TBS_TRY(enqueue_sweep(b->nv, b->prec, st, b->d_sys, b->d_sys, ns, b->d_ltab, b->nl, b->d_gtab, b->ng,
-                                  max_sweeps, rtol, b->variant));
+                                  max_sweeps, rtol, b->variant, b->rvariant));
         return TBS_OK;
     }
     hipStream_t aux;
@@ -969,15 +999,59 @@ int batch_sweeps(tbs_batch *b, hipStream_t st, int nsweeps, int max_sweeps, doub
     int rc = TBS_OK;
     for (int i = 0; i < nsweeps && rc == TBS_OK; ++i) {
         rc = enqueue_sweep(b->nv, b->prec, st, b->d_sys, b->d_sys, b->nsA, b->d_ltab, b->nlA, b->d_gtab, b->ngA,
-                           max_sweeps, rtol, b->variant);
+                           max_sweeps, rtol, b->variant, b->rvariant);
         if (rc == TBS_OK)
             rc = enqueue_sweep(b->nv, b->prec, aux, b->d_sys, sysB, ns - b->nsA, b->d_ltab + b->nlA, b->nl - b->nlA,
-                               b->d_gtab + b->ngA, b->ng - b->ngA, max_sweeps, rtol, b->variant);
+                               b->d_gtab + b->ngA, b->ng - b->ngA, max_sweeps, rtol, b->variant, b->rvariant);
     }
     // join even after a failed launch, so that a capture on st can end
     TBS_HIP(hipEventRecord(b->ctx->ev_join, aux));
     TBS_HIP(hipStreamWaitEvent(st, b->ctx->ev_join, 0));
     return rc;
+}
+// reps Thomas + residual passes of a batch (no norm step: the systems stay
+// active), on one stream or as the two halves of batch_sweeps, for timing.
+int batch_pass(tbs_batch *b, hipStream_t st, int reps, bool split) {
+    const int ns = (int)b->plans.size();
+    auto pass = [b](hipStream_t s2, const int4 *lt, int nl, const int2 *gt, int ng) {
+        TBS_TRY(k_thomas(b->nv, b->prec, s2, b->d_sys, lt, nl, b->variant));
+        return k_residual(b->nv, b->prec, s2, b->d_sys, gt, ng, 0, 1, nullptr, nullptr, b->rvariant);
+    };
+    if (!split || b->nsA <= 0 || b->nsA >= ns) {
+        for (int r = 0; r < reps; ++r) TBS_TRY(pass(st, b->d_ltab, b->nl, b->d_gtab, b->ng));
+        return TBS_OK;
+    }
+    hipStream_t aux;
+    TBS_TRY(tbs::aux_stream(b->ctx, &aux));
+    TBS_HIP(hipEventRecord(b->ctx->ev_fork, st));
+    TBS_HIP(hipStreamWaitEvent(aux, b->ctx->ev_fork, 0));
+    int rc = TBS_OK;
+    for (int r = 0; r < reps && rc == TBS_OK; ++r) {
+        rc = pass(st, b->d_ltab, b->nlA, b->d_gtab, b->ngA);
+        if (rc == TBS_OK) rc = pass(aux, b->d_ltab + b->nlA, b->nl - b->nlA, b->d_gtab + b->ngA, b->ng - b->ngA);
+    }
+    TBS_HIP(hipEventRecord(b->ctx->ev_join, aux));
+    TBS_HIP(hipStreamWaitEvent(st, b->ctx->ev_join, 0));
+    return rc;
+}
+
+void fill_tuning(int nv, int prec, int variant, int rvariant, int split, const double *t_us, const double *r_us,
+                 const double *pass_us, tbs_tuning *o) {
+    *o = tbs_tuning{};
+    const int code = thomas_pf_of(nv, prec, variant);
+    o->thomas_variant = variant;
+    o->thomas_prefetch = (code % 10000) / 100;
+    o->thomas_lds_windows = code % 100;
+    o->thomas_lines_per_wg = code / 10000 + 1;
+    o->residual_variant = rvariant;
+    o->residual_waves_per_wg = residual_rw_of(nv, rvariant);
+    o->two_stream = split;
+    o->n_thomas = std::min(thomas_variants(nv, prec), kTuneThomas);
+    o->n_residual = std::min(residual_variants(nv), kTuneResidual);
+    for (int v = 0; v < kTuneThomas; ++v) o->thomas_us[v] = t_us[v];
+    for (int v = 0; v < kTuneResidual; ++v) o->residual_us[v] = r_us[v];
+    for (int v = 0; v < o->n_residual; ++v) o->residual_shapes[v] = residual_rw_of(nv, v);
+    if (pass_us) { o->pass_us[0] = pass_us[0]; o->pass_us[1] = pass_us[1]; }
 }
 }  // namespace
 
@@ -991,14 +1065,38 @@ int tbs_plan_autotune(tbs_plan *p, int reps, int *chosen, double *us, int cap) {
     const size_t tf = tf_size(p->prec);
     std::vector<std::pair<double *, long>> zv = {{p->d_U, nvec}, {p->d_rhs, nvec},
                                                  {(double *)p->d_g, (long)(nvec * tf / 8)}};
-    int best = 0;
-    TBS_TRY(autotune_thomas(p->ctx, p->nv, p->prec, p->d_sys, 1, p->d_ltab, p->nline, zv, reps, &best, us, cap));
-    if (best != p->variant) {
+    int tb = 0, rb = 0;
+    TBS_TRY(autotune_kernels(p->ctx, p->nv, p->prec, p->d_sys, 1, p->d_ltab, p->nline, p->d_gtab, p->ngroups, zv,
+                             reps, &tb, p->t_us, &rb, p->r_us));
+    if (tb != p->variant || rb != p->rvariant) {
         for (auto &kv : p->graphs) hipGraphExecDestroy(kv.second);
         p->graphs.clear();
-        p->variant = best;
+        p->variant = tb;
+        p->rvariant = rb;
     }
-    if (chosen) *chosen = best;
+    if (chosen) *chosen = tb;
+    for (int v = 0; us && v < cap && v < kTuneThomas; ++v) us[v] = p->t_us[v];
+    return TBS_OK;
+}
+
+int tbs_plan_tuning(tbs_plan *p, tbs_tuning *out) {
+    TBS_REQUIRE(p && out, TBS_ERR_ARG, "null argument");
+    fill_tuning(p->nv, p->prec, p->variant, p->rvariant, 0, p->t_us, p->r_us, nullptr, out);
+    return TBS_OK;
+}
+
+int tbs_plan_set_tuning(tbs_plan *p, const tbs_tuning *t) {
+    TBS_REQUIRE(p && t, TBS_ERR_ARG, "null argument");
+    TBS_REQUIRE(t->thomas_variant >= 0 && t->thomas_variant < thomas_variants(p->nv, p->prec), TBS_ERR_ARG,
+                "Thomas variant %d not built", t->thomas_variant);
+    TBS_REQUIRE(t->residual_variant >= 0 && t->residual_variant < residual_variants(p->nv), TBS_ERR_ARG,
+                "residual variant %d not built", t->residual_variant);
+    if (t->thomas_variant != p->variant || t->residual_variant != p->rvariant) {
+        for (auto &kv : p->graphs) hipGraphExecDestroy(kv.second);
+        p->graphs.clear();
+        p->variant = t->thomas_variant;
+        p->rvariant = t->residual_variant;
+    }
     return TBS_OK;
 }
 
@@ -1073,6 +1171,7 @@ int tbs_batch_create(tbs_ctx *ctx, tbs_plan *const *plans, int nplans, tbs_batch
     b->ctx = ctx;
     b->nv = plans[0]->nv;
     b->prec = plans[0]->prec;
+    b->split = b->prec == TBS_FP64;   // until autotuned: FP64 halves overlap, FP32 ones gain nothing
     TBS_HIP(hipEventCreateWithFlags(&b->staged, hipEventDisableTiming));
     TBS_TRY(tbs_batch_set_plans(b.get(), plans, nplans));
     *out = b.release();
@@ -1095,11 +1194,11 @@ int tbs_batch_set_plans(tbs_batch *b, tbs_plan *const *plans, int nplans) {
         for (auto &kv : b->graphs) hipGraphExecDestroy(kv.second);
         b->graphs.clear();
     }
-    // FP64 factors: the systems in two halves of about equal element count,
-    // run on two streams (each half's Thomas pass overlaps the other's
-    // residual; with FP32 factors one stream is as fast)
+    // the systems in two halves of about equal element count, which eager
+    // sweeps run on two streams when b->split is set (each half's Thomas pass
+    // overlaps the other's residual)
     int nsA = 0;
-    if (b->prec == TBS_FP64 && nplans >= 2) {
+    if (nplans >= 2) {
         long tot = 0, acc = 0;
         for (int s2 = 0; s2 < nplans; ++s2) tot += plans[s2]->nle;
         while (nsA < nplans - 1 && 2 * (acc + plans[nsA]->nle) <= tot) acc += plans[nsA++]->nle;
@@ -1198,15 +1297,55 @@ int tbs_batch_autotune(tbs_batch *b, int reps, int *chosen, double *us, int cap)
         zv.push_back({p->d_rhs, nvec});
         zv.push_back({(double *)p->d_g, (long)(nvec * tf / 8)});
     }
-    int best = 0;
-    TBS_TRY(autotune_thomas(b->ctx, b->nv, b->prec, b->d_sys, (int)b->plans.size(), b->d_ltab, b->nl, zv,
-                            reps, &best, us, cap));
-    if (best != b->variant) {
+    const int ns = (int)b->plans.size();
+    int tb = 0, rb = 0;
+    TBS_TRY(autotune_kernels(b->ctx, b->nv, b->prec, b->d_sys, ns, b->d_ltab, b->nl, b->d_gtab, b->ng, zv, reps,
+                             &tb, b->t_us, &rb, b->r_us));
+    if (tb != b->variant || rb != b->rvariant) {
         for (auto &kv : b->graphs) hipGraphExecDestroy(kv.second);
         b->graphs.clear();
-        b->variant = best;
+        b->variant = tb;
+        b->rvariant = rb;
     }
-    if (chosen) *chosen = best;
+    // the two-stream split of eager sweeps, timed both ways with the chosen
+    // kernels (a batch of one system has no halves)
+    hipStream_t st = b->ctx->stream;
+    b->pass_us[0] = b->pass_us[1] = 0.0;
+    if (b->nsA > 0 && b->nsA < ns) {
+        for (int sp2 = 0; sp2 < 2; ++sp2)
+            TBS_TRY(time_calls(st, 1, [&]() { return batch_pass(b, st, reps, sp2 == 1); }, &b->pass_us[sp2]));
+        b->pass_us[0] /= reps;
+        b->pass_us[1] /= reps;
+        b->split = b->pass_us[1] < b->pass_us[0];
+    } else {
+        b->split = false;
+    }
+    TBS_TRY(k_reset(b->prec, st, b->d_sys, ns));
+    TBS_HIP(hipStreamSynchronize(st));
+    if (chosen) *chosen = tb;
+    for (int v = 0; us && v < cap && v < kTuneThomas; ++v) us[v] = b->t_us[v];
+    return TBS_OK;
+}
+
+int tbs_batch_tuning(tbs_batch *b, tbs_tuning *out) {
+    TBS_REQUIRE(b && out, TBS_ERR_ARG, "null argument");
+    fill_tuning(b->nv, b->prec, b->variant, b->rvariant, b->split ? 1 : 0, b->t_us, b->r_us, b->pass_us, out);
+    return TBS_OK;
+}
+
+int tbs_batch_set_tuning(tbs_batch *b, const tbs_tuning *t) {
+    TBS_REQUIRE(b && t, TBS_ERR_ARG, "null argument");
+    TBS_REQUIRE(t->thomas_variant >= 0 && t->thomas_variant < thomas_variants(b->nv, b->prec), TBS_ERR_ARG,
+                "Thomas variant %d not built", t->thomas_variant);
+    TBS_REQUIRE(t->residual_variant >= 0 && t->residual_variant < residual_variants(b->nv), TBS_ERR_ARG,
+                "residual variant %d not built", t->residual_variant);
+    if (t->thomas_variant != b->variant || t->residual_variant != b->rvariant) {
+        for (auto &kv : b->graphs) hipGraphExecDestroy(kv.second);
+        b->graphs.clear();
+        b->variant = t->thomas_variant;
+        b->rvariant = t->residual_variant;
+    }
+    b->split = t->two_stream != 0;
     return TBS_OK;
 }
 
@@ -1214,7 +1353,7 @@ int tbs_batch_launch_sweeps(tbs_batch *b, int nsweeps, int max_sweeps, double rt
     TBS_REQUIRE(b && nsweeps >= 0 && max_sweeps >= 0 && max_sweeps < kHistCap, TBS_ERR_ARG, "bad arguments");
     TBS_TRY(tbs::use_ctx(b->ctx));
     if (b->plans.empty()) return TBS_OK;
-    return batch_sweeps(b, b->ctx->stream, nsweeps, max_sweeps, rtol, true);
+    return batch_sweeps(b, b->ctx->stream, nsweeps, max_sweeps, rtol, b->split);
 }
 
 int tbs_batch_states(tbs_batch *b, int *states) {
@@ -1261,18 +1400,19 @@ int tbs_batch_solve(tbs_batch *b, const double *const *rhs, double *const *U, in
     const int ns = (int)b->plans.size();
     hipStream_t st = b->ctx->stream;
     if (sweeps_per_check < 1) sweeps_per_check = 1;
+    for (int s = 0; s < ns; ++s) TBS_REQUIRE(rhs[s] && U[s], TBS_ERR_ARG, "null vector for system %d", s);
+    TBS_TRY(k_reset(b->prec, st, b->d_sys, ns));
     for (int s = 0; s < ns; ++s) {
         tbs_plan *p = b->plans[s];
-        TBS_REQUIRE(rhs[s] && U[s], TBS_ERR_ARG, "null vector for system %d", s);
         const long total = (long)p->nle * p->nv;
         hipLaunchKernelGGL(gather_vec_kernel, dim3(gs_grid(total)), dim3(256), 0, st, total, p->nv,
                            p->d_lines, rhs[s], p->d_rhs);
-        hipLaunchKernelGGL(gather_vec_kernel, dim3(gs_grid(total)), dim3(256), 0, st, total, p->nv,
-                           p->d_lines, (const double *)U[s], p->d_U);
+        hipLaunchKernelGGL(gather_u0_kernel, dim3(gs_grid(total)), dim3(256), 0, st, total, p->nv,
+                           p->d_lines, (const double *)U[s], p->d_U, p->d_state + 3);
         TBS_LAUNCHED("batch gather");
     }
-    TBS_TRY(k_reset(b->prec, st, b->d_sys, ns));
-    TBS_TRY(k_residual(b->nv, b->prec, st, b->d_sys, b->d_gtab, b->ng, 0, 1));
+    // J U skipped for the systems whose initial U is all zeros
+    TBS_TRY(k_residual(b->nv, b->prec, st, b->d_sys, b->d_gtab, b->ng, 0, 1 | 4, nullptr, nullptr, b->rvariant));
     TBS_TRY(k_final(b->prec, st, b->d_sys, ns, max_sweeps, rtol, 1));
     auto key = std::make_pair(sweeps_per_check, std::make_pair(max_sweeps, rtol));
     auto it = b->graphs.find(key);

@@ -73,6 +73,7 @@ using tbs::rdlane;
 
 constexpr int kGroup = 64;      // elements per residual group (one wave)
 constexpr int kHistCap = 4096;  // sweeps of history kept per system
+constexpr int kTuneThomas = TBS_TUNE_THOMAS_MAX, kTuneResidual = TBS_TUNE_RESIDUAL_MAX;
 constexpr int kPad = 64;                  // zero elements before and after U, rhs, R
 constexpr int kSlotNone = INT_MIN;        // no lane of the group has this face
 constexpr int kSlotGather = INT_MIN + 1;  // irregular neighbours: per-lane gather
@@ -181,29 +182,32 @@ __device__ __forceinline__ void read_tile(const double *tile, int lane, double (
     __builtin_amdgcn_wave_barrier();
 }
 
-template <int NV, typename TF>
-__global__ __launch_bounds__(256) void fres_kernel(const SysDev<TF> *__restrict__ sys,
-                                                   const int2 *__restrict__ gtab, int ng,
-                                                   int want_R, int force) {
+// RW waves (64-element groups) per workgroup.
+template <int NV, typename TF, int RW>
+__global__ __launch_bounds__(64 * RW) void fres_kernel(const SysDev<TF> *__restrict__ sys,
+                                                       const int2 *__restrict__ gtab, int ng,
+                                                       int want_R, int force) {
     constexpr int BS = NV * NV;
-    __shared__ double tiles[4][kGroup * NV];
+    __shared__ double tiles[RW][kGroup * NV];
     // XCD-aware order: hardware block b runs on XCD b % 8; give each XCD a
     // contiguous run of groups so neighbouring lines share an L2.
-    const int nblk = (ng + 3) / 4;
+    const int nblk = (ng + RW - 1) / RW;
     const int per = (nblk + 7) / 8;
     const int blk = (int)(blockIdx.x % 8u) * per + (int)(blockIdx.x / 8u);
     if (blk >= nblk) return;
     const int wave = threadIdx.x >> 6;
-    const int wid = __builtin_amdgcn_readfirstlane(blk * 4 + wave);
+    const int wid = __builtin_amdgcn_readfirstlane(blk * RW + wave);
     if (wid >= ng) return;
     double *tile = tiles[wave];
     const int lane = threadIdx.x & 63;
     const int2 sg = gtab[wid];
     const SysDev<TF> &S = sys[sg.x];
     // force bit 0: run even if the system is inactive; bit 1: U is zero
-    // (the first residual of a solve from zero is R = rhs: skip J U)
+    // (the first residual of a solve from zero is R = rhs: skip J U); bit 2:
+    // U is zero unless the gather of the initial U flagged a nonzero value
+    // (state[3])
     if (!(force & 1) && !S.state[1]) return;
-    const bool u_zero = (force & 2) != 0;
+    const bool u_zero = (force & 2) != 0 || ((force & 4) != 0 && S.state[3] == 0);
     const int g = sg.y;
     const int F = S.F;
     const int m = g * kGroup + lane;
@@ -355,9 +359,12 @@ __device__ __forceinline__ void bulk_store(tbs::rsrc_t r, uint32_t base, int lan
 // sides and both waves substitute back outwards from it.  The bottom wave
 // runs the same loops over its own windows; only the place of a step's
 // values in memory differs (element m0+n-1-s for its step s, less pad).
-template <int NV, typename TF, int PF, int K>
-__global__ __launch_bounds__(128) void fthomas_kernel(const SysDev<TF> *__restrict__ sys,
-                                                      const int4 *__restrict__ ltab, int nl) {
+// A workgroup holds LPW lines (2 LPW waves, consecutive table lines); a line
+// past the table's end or of an inactive system only takes part in the
+// workgroup's two barriers.
+template <int NV, typename TF, int PF, int K, int LPW>
+__global__ __launch_bounds__(128 * LPW) void fthomas_kernel(const SysDev<TF> *__restrict__ sys,
+                                                            const int4 *__restrict__ ltab, int nl) {
     constexpr int SPW = 64 / NV;      // steps per window
     constexpr int LW = SPW * NV;      // lanes carrying data
     constexpr int NB = PF + 1;        // register ring depth
@@ -365,24 +372,27 @@ __global__ __launch_bounds__(128) void fthomas_kernel(const SysDev<TF> *__restri
     constexpr int VG = G * LW;        // vector values per group
     using IOT = GroupIO<VG, TF>;
     using IOD = GroupIO<VG, double>;
-    __shared__ TF s_vin[2][2][IOT::LDS];    // [wave] forward: g; backward: d'
-    __shared__ TF s_out[2][IOT::LDS];       // [wave] forward: d' staging
-    __shared__ double s_u[2][2][IOD::LDS];  // [wave] backward: U (updated in place)
-    __shared__ TF s_x[3][NV];               // d' at step m-1, d' at step m+1, d at step m
-    const int l = blockIdx.x;
-    if (l >= nl) return;
+    __shared__ TF s_vin[2 * LPW][2][IOT::LDS];    // [wave] forward: g; backward: d'
+    __shared__ TF s_out[2 * LPW][IOT::LDS];       // [wave] forward: d' staging
+    __shared__ double s_u[2 * LPW][2][IOD::LDS];  // [wave] backward: U (updated in place)
+    __shared__ TF s_xall[LPW][3][NV];             // d' at step m-1, d' at step m+1, d at step m
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);   // wave of the workgroup
+    const int l = (int)blockIdx.x * LPW + (wave >> 1);
+    const bool bot = (wave & 1) == 1;
+    auto s_x = s_xall[wave >> 1];
     // {system, m0, n, window offset}, wave-uniform by construction: the
     // group loops below must be scalar loops (the chain broadcasts read
     // lanes that every step writes)
-    int4 lt = ltab[l];
+    int4 lt = l < nl ? ltab[l] : make_int4(0, 0, 0, 0);
     lt.x = __builtin_amdgcn_readfirstlane(lt.x);
     lt.y = __builtin_amdgcn_readfirstlane(lt.y);
     lt.z = __builtin_amdgcn_readfirstlane(lt.z);
     lt.w = __builtin_amdgcn_readfirstlane(lt.w);
     const SysDev<TF> &S = sys[lt.x];
-    if (!S.state[1]) return;                                            // the whole workgroup
-    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-    const bool bot = wave == 1;
+    // (for LPW = 1 the whole workgroup; else the line's two waves, which still
+    // meet the workgroup's barriers)
+    const bool live = l < nl && __builtin_amdgcn_readfirstlane(S.state[1]) != 0;
+    if (LPW == 1 && !live) return;
     const int lane = threadIdx.x & 63;
     const int st_of_lane = lane / NV;
     const int row = lane - st_of_lane * NV;
@@ -433,9 +443,10 @@ __global__ __launch_bounds__(128) void fthomas_kernel(const SysDev<TF> *__restri
             e[WL::NC * WL::VW + k] = tbs::bload_nt<TF>(r, b + (uint32_t)((WL::NC * WL::VW * LW + k * LW + lane) * sizeof(TF)), 0);
     };
 
-    if (lane < NV) s_x[wave][lane] = TF(0);                             // an empty side contributes 0
+    if (lane < NV) s_x[bot ? 1 : 0][lane] = TF(0);                      // an empty side contributes 0
     TF d[NV];
     TF eb[NB][NV];
+    if (live) {
     // ---- forward: d_k = g_k - E_k d_{k-1} (bottom wave: d_{k+1})
 #pragma unroll
     for (int j = 0; j < NV; ++j) d[j] = TF(0);
@@ -476,14 +487,15 @@ __global__ __launch_bounds__(128) void fthomas_kernel(const SysDev<TF> *__restri
             }
             if (lane < LW) s_out[wave][slot] = out;
             // the wave's last step (step m-1, or m+1 for the bottom wave)
-            if (w == nw - 1 && st_of_lane == SPW - 1 && lane < LW) s_x[wave][row] = out;
+            if (w == nw - 1 && st_of_lane == SPW - 1 && lane < LW) s_x[bot ? 1 : 0][row] = out;
         }
         bulk_store<VG, TF>(rd, vbytes(gbase(jg), sizeof(TF)), lane, s_out[wave]);
         if (jg + 1 < ngrp) bulk_commit<VG, TF>(vr, lane, s_vin[wave][cur ^ 1]);
     }
+    }
     __syncthreads();
     // ---- step m: d_m = g_m - E_m d'_{m-1} - F_m d'_{m+1}, U_m += d_m
-    if (!bot && lane < NV) {
+    if (live && !bot && lane < NV) {
         const tbs::rsrc_t rME = tbs::make_rsrc(S.EF + ((long)lt.w + sp.q) * WSTRIDE, wbytes);
         const tbs::rsrc_t rMF = tbs::make_rsrc(S.GB + ((long)lt.w + sp.q) * WSTRIDE, wbytes);
         TF em[NV], fm[NV];
@@ -501,6 +513,7 @@ __global__ __launch_bounds__(128) void fthomas_kernel(const SysDev<TF> *__restri
         S.U[at] += (double)v;
     }
     __syncthreads();
+    if (!live) return;
 
     // ---- backward: d_k = d'_k - Gamma_k d_{k+1} (bottom wave: d_{k-1}); U_k += d_k
     // The groups in reverse, each from its last window down.  The top wave's
@@ -619,6 +632,7 @@ __global__ void reset_state_kernel(const SysDev<TF> *__restrict__ sys, int nsys)
         sys[i].state[0] = 0;
         sys[i].state[1] = 1;
         sys[i].state[2] = 0;
+        sys[i].state[3] = 0;
     }
 }
 
@@ -1416,6 +1430,21 @@ __global__ void gather_vec_kernel(long total, int nv, const int *__restrict__ li
         dst[t] = src ? src[(size_t)nv * lines[m] + i] : 0.0;
     }
 }
+// The gather of an initial U, which also sets *nonzero when any value is
+// not +-0 (NaN included): a solve from U = 0 then starts with R = rhs.
+__global__ void gather_u0_kernel(long total, int nv, const int *__restrict__ lines,
+                                 const double *__restrict__ src, double *__restrict__ dst, int *__restrict__ nonzero) {
+    int any = 0;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long)gridDim.x * blockDim.x) {
+        const long m = t / nv;
+        const int i = (int)(t % nv);
+        const double v = src[(size_t)nv * lines[m] + i];
+        dst[t] = v;
+        any |= !(v == 0.0);
+    }
+    if (__syncthreads_or(any) && threadIdx.x == 0) *nonzero = 1;
+}
 __global__ void scatter_vec_kernel(long total, int nv, const int *__restrict__ lines,
                                    const double *__restrict__ src, double *__restrict__ dst) {
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
@@ -1445,41 +1474,81 @@ void launch_k(K kernel, dim3 grid, dim3 block, hipStream_t st, hipEvent_t e0, hi
     else hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
 }
 
-template <int NV, typename TF>
-int launch_residual(hipStream_t st, const void *sys, const int2 *gtab, int ng, int want_R, int force,
-                    hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
-    if (ng == 0) return TBS_OK;
-    const int nblk = (ng + 3) / 4;
-    launch_k(fres_kernel<NV, TF>, dim3(8 * ((nblk + 7) / 8)), dim3(256), st, e0, e1,
+// Residual workgroup shapes the autotuner picks from: waves (64-element
+// groups) per workgroup; index 0 is the default.  Only the headline block
+// size has more than one.
+template <int NV> struct ResidualV {
+    static constexpr int n = 1;
+    static constexpr int rw[1] = {4};
+};
+template <> struct ResidualV<9> {
+    static constexpr int n = 4;
+    static constexpr int rw[4] = {4, 2, 8, 1};
+};
+
+template <int NV, typename TF, int RW>
+int launch_residual_rw(hipStream_t st, const void *sys, const int2 *gtab, int ng, int want_R, int force,
+                       hipEvent_t e0, hipEvent_t e1) {
+    const int nblk = (ng + RW - 1) / RW;
+    launch_k(fres_kernel<NV, TF, RW>, dim3(8 * ((nblk + 7) / 8)), dim3(64 * RW), st, e0, e1,
              (const SysDev<TF> *)sys, gtab, ng, want_R, force);
     TBS_LAUNCHED("fres_kernel");
     return TBS_OK;
 }
 
+template <int NV, typename TF>
+int launch_residual(hipStream_t st, const void *sys, const int2 *gtab, int ng, int want_R, int force,
+                    hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, int rvariant = 0) {
+    if (ng == 0) return TBS_OK;
+    using P = ResidualV<NV>;
+    if (rvariant < 0 || rvariant >= P::n) rvariant = 0;
+    if constexpr (P::n == 1) {
+        return launch_residual_rw<NV, TF, P::rw[0]>(st, sys, gtab, ng, want_R, force, e0, e1);
+    } else {
+        switch (rvariant) {
+            case 1: return launch_residual_rw<NV, TF, P::rw[1]>(st, sys, gtab, ng, want_R, force, e0, e1);
+            case 2: return launch_residual_rw<NV, TF, P::rw[2]>(st, sys, gtab, ng, want_R, force, e0, e1);
+            case 3: return launch_residual_rw<NV, TF, P::rw[3]>(st, sys, gtab, ng, want_R, force, e0, e1);
+            default: return launch_residual_rw<NV, TF, P::rw[0]>(st, sys, gtab, ng, want_R, force, e0, e1);
+        }
+    }
+}
+
 // Thomas variants the autotuner picks from: {prefetch windows PF, ring
-// cycles K per loop iteration}; index 0 is the default.
-struct TV { int pf, k; };
+// cycles K per loop iteration (the group of windows staged in LDS), lines per
+// workgroup}; index 0 is the default.
+struct TV { int pf, k, lpw; };
 // (blocks above 16x16: one window prefetched, so the ring of NV-value rows
 // stays within the registers)
 template <int NV, typename TF> struct ThomasV {
     static constexpr int n = 1;
-    static constexpr TV v[1] = {{NV > 16 ? 1 : 2, 2}};
+    static constexpr TV v[1] = {{NV > 16 ? 1 : 2, 2, 1}};
 };
-// (LDS per two-wave workgroup stays <= 32 KiB so five lines fit a CU.)
+// (LDS per two-wave line stays <= 32 KiB so five lines fit a CU.)
 template <> struct ThomasV<9, double> {
-    static constexpr int n = 4;
-    static constexpr TV v[4] = {{2, 2}, {3, 1}, {1, 3}, {1, 2}};
+    static constexpr int n = 8;
+    static constexpr TV v[8] = {{2, 2, 1}, {3, 1, 1}, {1, 3, 1}, {1, 2, 1},
+                                {2, 2, 2}, {3, 1, 2}, {1, 3, 2}, {1, 2, 2}};
 };
 template <> struct ThomasV<9, float> {
-    static constexpr int n = 4;
-    static constexpr TV v[4] = {{3, 2}, {2, 2}, {5, 1}, {3, 1}};
+    static constexpr int n = 8;
+    static constexpr TV v[8] = {{3, 2, 1}, {2, 2, 1}, {5, 1, 1}, {3, 1, 1},
+                                {3, 2, 2}, {2, 2, 2}, {5, 1, 2}, {3, 1, 2}};
 };
 
-template <int NV, typename TF, int PF, int K>
+template <int NV, typename TF, int PF, int K, int LPW>
 int launch_thomas_pf(hipStream_t st, const void *sys, const int4 *ltab, int nl, hipEvent_t e0, hipEvent_t e1) {
-    launch_k(fthomas_kernel<NV, TF, PF, K>, dim3(nl), dim3(128), st, e0, e1, (const SysDev<TF> *)sys, ltab, nl);
+    launch_k(fthomas_kernel<NV, TF, PF, K, LPW>, dim3((nl + LPW - 1) / LPW), dim3(128 * LPW), st, e0, e1,
+             (const SysDev<TF> *)sys, ltab, nl);
     TBS_LAUNCHED("fthomas_kernel");
     return TBS_OK;
+}
+
+template <int NV, typename TF, int I>
+int launch_thomas_v(hipStream_t st, const void *sys, const int4 *ltab, int nl, hipEvent_t e0, hipEvent_t e1) {
+    using P = ThomasV<NV, TF>;
+    constexpr TV t = P::v[I < P::n ? I : 0];
+    return launch_thomas_pf<NV, TF, t.pf, t.k, t.lpw>(st, sys, ltab, nl, e0, e1);
 }
 
 template <int NV, typename TF>
@@ -1488,15 +1557,19 @@ int launch_thomas(hipStream_t st, const void *sys, const int4 *ltab, int nl, int
     if (nl == 0) return TBS_OK;
     using P = ThomasV<NV, TF>;
     if (variant < 0 || variant >= P::n) variant = 0;
-    constexpr TV a = P::v[0];
-    constexpr TV b = P::v[P::n > 1 ? 1 : 0];
-    constexpr TV c = P::v[P::n > 2 ? 2 : 0];
-    constexpr TV d = P::v[P::n > 3 ? 3 : 0];
-    switch (variant) {
-        case 1: return launch_thomas_pf<NV, TF, b.pf, b.k>(st, sys, ltab, nl, e0, e1);
-        case 2: return launch_thomas_pf<NV, TF, c.pf, c.k>(st, sys, ltab, nl, e0, e1);
-        case 3: return launch_thomas_pf<NV, TF, d.pf, d.k>(st, sys, ltab, nl, e0, e1);
-        default: return launch_thomas_pf<NV, TF, a.pf, a.k>(st, sys, ltab, nl, e0, e1);
+    if constexpr (P::n == 1) {
+        return launch_thomas_v<NV, TF, 0>(st, sys, ltab, nl, e0, e1);
+    } else {
+        switch (variant) {
+            case 1: return launch_thomas_v<NV, TF, 1>(st, sys, ltab, nl, e0, e1);
+            case 2: return launch_thomas_v<NV, TF, 2>(st, sys, ltab, nl, e0, e1);
+            case 3: return launch_thomas_v<NV, TF, 3>(st, sys, ltab, nl, e0, e1);
+            case 4: return launch_thomas_v<NV, TF, 4>(st, sys, ltab, nl, e0, e1);
+            case 5: return launch_thomas_v<NV, TF, 5>(st, sys, ltab, nl, e0, e1);
+            case 6: return launch_thomas_v<NV, TF, 6>(st, sys, ltab, nl, e0, e1);
+            case 7: return launch_thomas_v<NV, TF, 7>(st, sys, ltab, nl, e0, e1);
+            default: return launch_thomas_v<NV, TF, 0>(st, sys, ltab, nl, e0, e1);
+        }
     }
 }
 

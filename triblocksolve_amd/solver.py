@@ -28,9 +28,33 @@ def factor_plans(ctx, plans, jacD, jacO1, jacO2):
 
 
 def variant_name(code):
-    """Name of a Thomas variant from its code (100 * prefetch windows + ring cycles)."""
+    """Name of a Thomas variant from its code (100 * prefetch windows + ring
+    cycles + 10000 * (lines per workgroup - 1)); the lines per workgroup are
+    named when more than one."""
     code = int(code)
-    return f"pf{code // 100}_k{code % 100}"
+    name = f"pf{(code % 10000) // 100}_k{code % 100}"
+    return name + (f"_l{code // 10000 + 1}" if code >= 10000 else "")
+
+
+def _tuning_dict(t):
+    return {"thomas_variant": t.thomas_variant, "thomas_prefetch": t.thomas_prefetch,
+            "thomas_lds_windows": t.thomas_lds_windows, "thomas_lines_per_wg": t.thomas_lines_per_wg,
+            "residual_variant": t.residual_variant, "residual_waves_per_wg": t.residual_waves_per_wg,
+            "two_stream": bool(t.two_stream),
+            "thomas_us": [float(x) for x in t.thomas_us[: t.n_thomas]],
+            "residual_us": [float(x) for x in t.residual_us[: t.n_residual]],
+            "residual_waves_per_wg_by_variant": [int(x) for x in t.residual_shapes[: t.n_residual]],
+            "pass_us": [float(x) for x in t.pass_us]}
+
+
+def _set_tuning(fn_get, fn_set, handle, kw):
+    t = A.Tuning()
+    A.check(fn_get(handle, C.byref(t)))
+    for k, v in kw.items():
+        if k not in ("thomas_variant", "residual_variant", "two_stream"):
+            raise KeyError(f"{k} is not settable")
+        setattr(t, k, int(v))
+    A.check(fn_set(handle, C.byref(t)))
 
 
 def _ptr(x):
@@ -443,6 +467,18 @@ class Plan:
     def set_variant(self, v):
         A.check(self.ctx._lib.tbs_plan_set_variant(self.handle, v))
 
+    def tuning(self):
+        """The chosen tuple (Thomas variant and its prefetch / LDS windows /
+        lines per workgroup, residual waves per workgroup) and the last
+        autotune's timings."""
+        t = A.Tuning()
+        A.check(self.ctx._lib.tbs_plan_tuning(self.handle, C.byref(t)))
+        return _tuning_dict(t)
+
+    def set_tuning(self, **kw):
+        """Set thomas_variant and/or residual_variant."""
+        _set_tuning(self.ctx._lib.tbs_plan_tuning, self.ctx._lib.tbs_plan_set_tuning, self.handle, kw)
+
     def factor(self, jacD, jacO1, jacO2):
         """Factorise; returns after the device has finished reading the Jacobian
         (the C call is asynchronous, and the caller's buffers may be temporaries)."""
@@ -507,10 +543,21 @@ class Batch:
             pass
 
     def autotune(self, reps=3, nvariants=8):
+        """Tune Thomas variant, residual shape and the two-stream split over all
+        the batch's systems; returns (Thomas variant, its timings per variant)."""
         us = np.zeros(nvariants)
         ch = C.c_int()
         A.check(self.ctx._lib.tbs_batch_autotune(self.handle, reps, C.byref(ch), _ptr(us), nvariants))
         return ch.value, us
+
+    def tuning(self):
+        t = A.Tuning()
+        A.check(self.ctx._lib.tbs_batch_tuning(self.handle, C.byref(t)))
+        return _tuning_dict(t)
+
+    def set_tuning(self, **kw):
+        """Set thomas_variant, residual_variant and/or two_stream."""
+        _set_tuning(self.ctx._lib.tbs_batch_tuning, self.ctx._lib.tbs_batch_set_tuning, self.handle, kw)
 
     def set_plans(self, plans):
         """Replace the batch's systems (tables rewritten in place, stream-ordered)."""
