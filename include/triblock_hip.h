@@ -404,8 +404,10 @@ int tbs_server_timeline(tbs_server *srv, long id, double *t, int n);
 int tbs_server_info(tbs_server *srv, size_t *budget, size_t *used, double *now);
 /* Host time per engine phase, cumulative seconds: [0] waiting for the batch's
  * sweeps, [1] membership update, [2] sweep launch, [3] retirement, [4]
- * admission, [5] the part of [4] spent building plans; counts: [0] steps,
- * [1] admitted, [2] retired, [3] batch membership updates. */
+ * admission on the serving thread (slab and hand-off), [5] plan building and
+ * [6] whole set-ups on the admission thread (which runs beside the steps);
+ * counts: [0] steps, [1] admitted, [2] retired, [3] batch membership
+ * updates. */
 int tbs_server_stats(tbs_server *srv, double *seconds, int nseconds, long *counts, int ncounts);
 
 /* ---------------------------------------------------------------------- */

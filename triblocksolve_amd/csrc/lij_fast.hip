@@ -170,10 +170,12 @@ struct Layout {
     std::vector<int> woff;
 };
 
-Layout layout_of(const tbs_mesh_host *m) {
+// (F_known > 0: the faces per element are known without epoint, as for a
+// structured grid whose tables the device generates)
+Layout layout_of(const tbs_mesh_host *m, int F_known = 0) {
     Layout L;
-    int F = 0;
-    for (int e = 0; e < m->nelem; ++e) F = std::max(F, m->epoint[e + 1] - m->epoint[e]);
+    int F = F_known;
+    for (int e = 0; F_known == 0 && e < m->nelem; ++e) F = std::max(F, m->epoint[e + 1] - m->epoint[e]);
     L.F = F;
     L.S = F + 1;
     L.ngroups = (m->nlineelem + kGroup - 1) / kGroup;
@@ -538,21 +540,25 @@ int tbs_plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pla
 
 }  // extern "C"
 
-int tbs::plan_footprint(const tbs_mesh_host *mesh, int precision, size_t *bytes, bool validate) {
+int tbs::plan_footprint(const tbs_mesh_host *mesh, int precision, size_t *bytes, bool validate, bool structured) {
     TBS_REQUIRE(bytes, TBS_ERR_ARG, "null output");
     if (validate) TBS_TRY(check_mesh(mesh));
     else TBS_REQUIRE(mesh != nullptr && mesh->nelem >= 1 && mesh->nline >= 1, TBS_ERR_ARG, "bad mesh sizes");
     TBS_REQUIRE(precision == TBS_FP64 || precision == TBS_FP32, TBS_ERR_ARG, "bad precision");
     TBS_REQUIRE(plan_nvar_ok(mesh->nvar), TBS_ERR_UNSUPPORTED, "nvar %d not built for the fused path", mesh->nvar);
     TBS_TRY(check_ranges(mesh, precision));
-    Layout L = layout_of(mesh);
+    Layout L = layout_of(mesh, structured ? 4 : 0);
     *bytes = footprint_of(mesh, L, precision);
     return TBS_OK;
 }
 
 // Plan creation; validate = false skips the O(mesh) consistency checks for
-// meshes the library built itself (a served request's structured grid).
-int tbs::plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_plan **out, bool validate) {
+// meshes the library built itself (a served request's structured grid);
+// structured_nk > 0: the mesh is tbs_structured_mesh's NL x NK grid, given
+// by its sizes and line arrays only, and its element/face tables are
+// generated on the device (no O(mesh) host work).
+int tbs::plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_plan **out, bool validate,
+                     int structured_nk) {
     TBS_TRY(tbs::use_ctx(ctx));
     TBS_REQUIRE(out, TBS_ERR_ARG, "null output");
     *out = nullptr;
@@ -564,7 +570,10 @@ int tbs::plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pl
     TBS_REQUIRE(precision == TBS_FP64 || precision == TBS_FP32, TBS_ERR_ARG, "bad precision");
     TBS_TRY(check_ranges(m, precision));
     const int nv = m->nvar;
-    Layout L = layout_of(m);
+    const bool gen = structured_nk > 0;
+    TBS_REQUIRE(!gen || (m->nline >= 1 && (long)m->nline * structured_nk == m->nelem && m->nlineelem == m->nelem),
+                TBS_ERR_ARG, "structured grid sizes do not match");
+    Layout L = layout_of(m, gen ? 4 : 0);
     std::unique_ptr<tbs_plan, void (*)(tbs_plan *)> p(new tbs_plan(), free_plan);
     p->ctx = ctx;
     p->nv = nv;
@@ -625,7 +634,7 @@ int tbs::plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pl
     // reused from plan to plan, so the upload is one DMA; large ones through
     // a pageable image kept by the plan so the upload stays asynchronous
     constexpr size_t kStageMax = (size_t)64 << 20;
-    const bool staged = T.host_end <= kStageMax;
+    const bool staged = T.host_end <= kStageMax || gen;
     char *h = nullptr;
     if (staged) {
         TBS_TRY(tbs::stage_acquire(ctx, T.host_end, &h));
@@ -635,14 +644,16 @@ int tbs::plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pl
     }
     const int nle = P->nle, F = L.F, Fm = std::max(F, 1);
     auto put = [h](size_t off, const void *src_, size_t bytes) { if (bytes) std::memcpy(h + off, src_, bytes); };
-    put(T.lines, m->lines, (size_t)nle * 4);
+    if (!gen) put(T.lines, m->lines, (size_t)nle * 4);
     put(T.linepoint, m->linepoint, (size_t)(P->nline + 1) * 4);
     put(T.linesize, m->linesize, (size_t)P->nline * 4);
     put(T.woff, L.woff.data(), (size_t)(P->nline + 1) * 4);
-    put(T.fc, m->fc, (size_t)2 * m->nintface * 4);
-    put(T.lineface, m->lineface, (size_t)nle * 4);
-    put(T.epoint, m->epoint, (size_t)(P->nelem + 1) * 4);
-    put(T.ef, m->ef, (size_t)m->eftot * 4);
+    if (!gen) {
+        put(T.fc, m->fc, (size_t)2 * m->nintface * 4);
+        put(T.lineface, m->lineface, (size_t)nle * 4);
+        put(T.epoint, m->epoint, (size_t)(P->nelem + 1) * 4);
+        put(T.ef, m->ef, (size_t)m->eftot * 4);
+    }
     if (precision == TBS_FP32) {
         SysDev<float> sd = make_sysdev<float>(P);
         put(T.sys, &sd, sizeof(sd));
@@ -651,7 +662,17 @@ int tbs::plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pl
         put(T.sys, &sd, sizeof(sd));
     }
     hipStream_t st = ctx->stream;
-    TBS_HIP(hipMemcpyAsync(tb, h, T.host_end, hipMemcpyHostToDevice, st));
+    if (gen) {
+        // the line arrays and the system record from the host; the element
+        // and face tables from the generator kernel
+        TBS_HIP(hipMemcpyAsync(tb + T.linepoint, h + T.linepoint, T.fc - T.linepoint, hipMemcpyHostToDevice, st));
+        TBS_HIP(hipMemcpyAsync(tb + T.sys, h + T.sys, T.host_end - T.sys, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(structured_tables_kernel, dim3(gs_grid(nle)), dim3(256), 0, st, m->nline, structured_nk,
+                           P->d_lines, P->d_fc, P->d_lineface, (int *)(tb + T.epoint), (int *)(tb + T.ef));
+        TBS_LAUNCHED("structured_tables_kernel");
+    } else {
+        TBS_HIP(hipMemcpyAsync(tb, h, T.host_end, hipMemcpyHostToDevice, st));
+    }
     if (staged) TBS_TRY(tbs::stage_release(ctx));
     // derived tables on the device; the R vector (zeroed below) holds the
     // inverse permutation meanwhile
@@ -1178,6 +1199,74 @@ int tbs_batch_create(tbs_ctx *ctx, tbs_plan *const *plans, int nplans, tbs_batch
     return TBS_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// Device tables and pinned staging of a batch, grown (after the work that
+// reads them has finished) to hold ns systems, nl lines and ng groups.
+int batch_grow(tbs_batch *b, size_t ns, size_t nl, size_t ng) {
+    hipStream_t st = b->ctx->stream;
+    const size_t ssz = b->prec == TBS_FP32 ? sizeof(SysDev<float>) : sizeof(SysDev<double>);
+    if (ns > b->cap_sys || nl > b->cap_l || ng > b->cap_g) {
+        TBS_HIP(hipStreamSynchronize(st));
+        if (ns > b->cap_sys) {
+            const size_t c = std::max<size_t>(2 * ns, 16);
+            if (b->d_sys) TBS_HIP(hipFree(b->d_sys));
+            if (b->d_states) TBS_HIP(hipFree(b->d_states));
+            if (b->h_states) TBS_HIP(hipHostFree(b->h_states));
+            b->d_sys = nullptr; b->d_states = nullptr; b->h_states = nullptr; b->cap_sys = 0;
+            TBS_HIP(hipMalloc(&b->d_sys, c * ssz));
+            TBS_HIP(hipMalloc((void **)&b->d_states, c * 4 * sizeof(int)));
+            TBS_HIP(hipHostMalloc((void **)&b->h_states, c * 4 * sizeof(int), hipHostMallocDefault));
+            b->cap_sys = c;
+        }
+        if (nl > b->cap_l) {
+            const size_t c = std::max<size_t>(2 * nl, 256);
+            if (b->d_ltab) TBS_HIP(hipFree(b->d_ltab));
+            b->d_ltab = nullptr; b->cap_l = 0;
+            TBS_HIP(hipMalloc((void **)&b->d_ltab, c * sizeof(int4)));
+            b->cap_l = c;
+        }
+        if (ng > b->cap_g) {
+            const size_t c = std::max<size_t>(2 * ng, 256);
+            if (b->d_gtab) TBS_HIP(hipFree(b->d_gtab));
+            b->d_gtab = nullptr; b->cap_g = 0;
+            TBS_HIP(hipMalloc((void **)&b->d_gtab, c * sizeof(int2)));
+            b->cap_g = c;
+        }
+    }
+    const size_t total = (ns * ssz + 15) / 16 * 16 + nl * sizeof(int4) + ng * sizeof(int2);
+    if (total > b->cap_stage) {
+        TBS_HIP(hipEventSynchronize(b->staged));   // the previous upload must have left it
+        if (b->h_stage) TBS_HIP(hipHostFree(b->h_stage));
+        b->h_stage = nullptr; b->cap_stage = 0;
+        const size_t c = std::max<size_t>(2 * total, 4096);
+        TBS_HIP(hipHostMalloc((void **)&b->h_stage, c, hipHostMallocDefault));
+        b->cap_stage = c;
+    }
+    return TBS_OK;
+}
+}  // namespace
+
+int tbs::batch_create_reserved(tbs_ctx *ctx, int nvar, int prec, size_t nsys, size_t nlines, size_t ngroups,
+                               tbs_batch **out) {
+    TBS_TRY(tbs::use_ctx(ctx));
+    TBS_REQUIRE(out && (prec == TBS_FP64 || prec == TBS_FP32), TBS_ERR_ARG, "bad batch arguments");
+    *out = nullptr;
+    std::unique_ptr<tbs_batch, int (*)(tbs_batch *)> b(new tbs_batch(), tbs_batch_destroy);
+    b->ctx = ctx;
+    b->nv = nvar;
+    b->prec = prec;
+    b->split = prec == TBS_FP64;
+    TBS_HIP(hipEventCreateWithFlags(&b->staged, hipEventDisableTiming));
+    TBS_HIP(hipEventRecord(b->staged, ctx->stream));
+    TBS_TRY(batch_grow(b.get(), nsys, nlines, ngroups));
+    *out = b.release();
+    return TBS_OK;
+}
+
+extern "C" {
+
 int tbs_batch_set_plans(tbs_batch *b, tbs_plan *const *plans, int nplans) {
     TBS_REQUIRE(b && nplans >= 0 && (plans || nplans == 0), TBS_ERR_ARG, "bad batch arguments");
     TBS_TRY(tbs::use_ctx(b->ctx));
@@ -1228,44 +1317,10 @@ int tbs_batch_set_plans(tbs_batch *b, tbs_plan *const *plans, int nplans) {
     // on any failure below the batch is left empty, so launches are no-ops
     b->plans.clear();
     b->nl = b->ng = 0;
-    if (ns > b->cap_sys || nl > b->cap_l || ng > b->cap_g) {
-        TBS_HIP(hipStreamSynchronize(st));
-        if (ns > b->cap_sys) {
-            const size_t c = std::max<size_t>(2 * ns, 16);
-            if (b->d_sys) TBS_HIP(hipFree(b->d_sys));
-            if (b->d_states) TBS_HIP(hipFree(b->d_states));
-            if (b->h_states) TBS_HIP(hipHostFree(b->h_states));
-            b->d_sys = nullptr; b->d_states = nullptr; b->h_states = nullptr; b->cap_sys = 0;
-            TBS_HIP(hipMalloc(&b->d_sys, c * ssz));
-            TBS_HIP(hipMalloc((void **)&b->d_states, c * 4 * sizeof(int)));
-            TBS_HIP(hipHostMalloc((void **)&b->h_states, c * 4 * sizeof(int), hipHostMallocDefault));
-            b->cap_sys = c;
-        }
-        if (nl > b->cap_l) {
-            const size_t c = std::max<size_t>(2 * nl, 256);
-            if (b->d_ltab) TBS_HIP(hipFree(b->d_ltab));
-            b->d_ltab = nullptr; b->cap_l = 0;
-            TBS_HIP(hipMalloc((void **)&b->d_ltab, c * sizeof(int4)));
-            b->cap_l = c;
-        }
-        if (ng > b->cap_g) {
-            const size_t c = std::max<size_t>(2 * ng, 256);
-            if (b->d_gtab) TBS_HIP(hipFree(b->d_gtab));
-            b->d_gtab = nullptr; b->cap_g = 0;
-            TBS_HIP(hipMalloc((void **)&b->d_gtab, c * sizeof(int2)));
-            b->cap_g = c;
-        }
-    }
+    TBS_TRY(batch_grow(b, ns, nl, ng));
     // stage through pinned memory; the previous upload must have left it
-    const size_t o_l = (ns * ssz + 15) / 16 * 16, o_g = o_l + nl * sizeof(int4), total = o_g + ng * sizeof(int2);
+    const size_t o_l = (ns * ssz + 15) / 16 * 16, o_g = o_l + nl * sizeof(int4);
     TBS_HIP(hipEventSynchronize(b->staged));
-    if (total > b->cap_stage) {
-        if (b->h_stage) TBS_HIP(hipHostFree(b->h_stage));
-        b->h_stage = nullptr; b->cap_stage = 0;
-        const size_t c = std::max<size_t>(2 * total, 4096);
-        TBS_HIP(hipHostMalloc((void **)&b->h_stage, c, hipHostMallocDefault));
-        b->cap_stage = c;
-    }
     for (size_t s2 = 0; s2 < ns; ++s2) {
         if (b->prec == TBS_FP32) { SysDev<float> d = make_sysdev<float>(plans[s2]); std::memcpy(b->h_stage + ssz * s2, &d, ssz); }
         else { SysDev<double> d = make_sysdev<double>(plans[s2]); std::memcpy(b->h_stage + ssz * s2, &d, ssz); }

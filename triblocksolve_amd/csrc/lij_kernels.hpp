@@ -1365,6 +1365,46 @@ __global__ void tab_lines_kernel(const int *__restrict__ linepoint, const int *_
     }
 }
 
+// The element and face tables of tbs_structured_mesh's NL x NK grid (same
+// numbering: element e = l NK + k creates its in-line face (e+1, e), then its
+// cross face (e+NK, e); boundary ids -(b+2) for S, E, N, W boundary entry b),
+// one thread per element, from the closed form of the face count before e:
+// F(l, k) = l (2 NK - 1) + k (1 + [l < NL-1]).
+__global__ void structured_tables_kernel(int NL, int NK, int *__restrict__ lines, int *__restrict__ fc,
+                                         int *__restrict__ lineface, int *__restrict__ epoint, int *__restrict__ ef) {
+    const long n = (long)NL * NK;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int e = (int)t, l = e / NK, k = e - l * NK;
+        auto f_in = [NL, NK](int l2, int k2) {        // face (e+1, e) of element (l2, k2), k2 < NK-1
+            return l2 * (2 * NK - 1) + k2 * (1 + (l2 < NL - 1 ? 1 : 0));
+        };
+        auto f_cross = [NL, NK, &f_in](int l2, int k2) {   // face (e+NK, e), l2 < NL-1
+            return f_in(l2, k2) + (k2 < NK - 1 ? 1 : 0);
+        };
+        auto bnd = [](int b) { return -b - 2; };
+        const int bS = 0, bE = NK, bN = NK + NL, bW = NK + NL + NK;
+        if (k < NK - 1) {
+            const int f = f_in(l, k);
+            fc[2 * f] = e + 1;
+            fc[2 * f + 1] = e;
+        }
+        if (l < NL - 1) {
+            const int f = f_cross(l, k);
+            fc[2 * f] = e + NK;
+            fc[2 * f + 1] = e;
+        }
+        const int w = k > 0 ? f_in(l, k - 1) : bnd(bW + l);
+        ef[4 * t + 0] = l > 0 ? f_cross(l - 1, k) : bnd(bS + k);
+        ef[4 * t + 1] = k < NK - 1 ? f_in(l, k) : bnd(bE + l);
+        ef[4 * t + 2] = l < NL - 1 ? f_cross(l, k) : bnd(bN + k);
+        ef[4 * t + 3] = w;
+        epoint[t] = 4 * e;
+        if (t == n - 1) epoint[n] = 4 * (int)n;
+        lines[t] = e;
+        lineface[t] = w;
+    }
+}
+
 // inv[lines[i]] = i: internal position of every element.
 __global__ void tab_inv_kernel(int nle, const int *__restrict__ lines, int *__restrict__ inv) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nle; i += gridDim.x * blockDim.x) inv[lines[i]] = i;

@@ -14,21 +14,28 @@
 //   3. enqueues K sweeps of the batch,
 //   4. while those run, retires the finished systems on a third stream (U
 //      scattered to element order and copied to the host, slab returned), and
-//   5. admits queued requests that fit: mesh tables, the Jacobian (copied
-//      from the caller's arrays or generated), factorisation and the initial
-//      residual are enqueued on the admission stream and overlap the batch's
-//      sweeps on the device.
+//   5. admits queued requests that fit: each gets its slab and goes to the
+//      admission thread, which builds its plan tables and enqueues the
+//      Jacobian (copied from the caller's arrays or generated), the
+//      factorisation and the initial residual on the admission stream, so
+//      neither that host work nor the device work holds up the steps.
+// Everything a step could otherwise allocate (pinned result buffers, table
+// staging, the batch's tables) is allocated when the server is created.
 // A request's arithmetic depends on its own data alone (per-line Thomas,
 // per-element residual, per-system fixed-order norm), so its result is
 // bitwise the same whatever it shares the batch with.
 #include "tbs_common.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -89,11 +96,15 @@ struct Request {
     int where = TBS_HOST;
     std::vector<double> hD, hO1, hO2, hrhs;
     const double *uD = nullptr, *uO1 = nullptr, *uO2 = nullptr, *urhs = nullptr;
-    int nelem = 0, nintface = 0;
+    int nelem = 0, nintface = 0, eftot = 0, nlineelem = 0;
     size_t footprint = 0;   // slab bytes while admitting
     size_t persist = 0;     // slab bytes kept once admitted (plan, rhs, U)
     size_t slab_off = 0, slab_len = 0;
+    char *slab_ptr = nullptr;   // device address of the slab (set before the hand-off)
     int skipped = 0;
+    // admission thread -> serving thread: 0 set-up pending, 1 set-up enqueued
+    // (ready recorded), -1 set-up failed (slab no longer in use)
+    std::atomic<int> setup{0};
     std::vector<int> epoint, ef, fc, linesize, linepoint, lines, lineface;
     tbs_plan *plan = nullptr;
     double *d_rhs = nullptr, *d_U = nullptr;   // element order, in the slab
@@ -101,7 +112,8 @@ struct Request {
     hipEvent_t ready = nullptr;
     double *hU = nullptr;           // result U (pinned, from the server's pool)
     size_t nU = 0, hU_cls = 0;
-    std::vector<double> hist;
+    double *hH = nullptr;           // residual history (pinned, from the pool)
+    size_t nH = 0, hH_cls = 0;
     int nsweeps = 0, status = 0;
     // timeline (seconds on the server's clock): submitted, admitted (set-up
     // enqueued), set-up seen complete (joins the batch), convergence seen by
@@ -109,21 +121,20 @@ struct Request {
     double t_submit = 0, t_admit = 0, t_ready = 0, t_conv = 0, t_retire = 0, t_done = 0;
 };
 
+// A seeded request's grid: its sizes and line arrays only; the plan
+// generates the element and face tables on the device.
 int build_mesh(Request *r) {
     const int NL = r->spec.NL, NK = r->spec.NK;
     int nif = 0, n = 0, eft = 0, nl = 0, nle = 0;
     TBS_TRY(tbs_structured_sizes(NL, NK, &n, &nif, &eft, &nl, &nle));
     r->nelem = n;
     r->nintface = nif;
-    r->epoint.resize(n + 1);
-    r->ef.resize(eft);
-    r->fc.resize(2 * (size_t)nif);
-    r->linesize.resize(nl);
+    r->eftot = eft;
+    r->nlineelem = nle;
+    r->linesize.assign(nl, NK);
     r->linepoint.resize(nl + 1);
-    r->lines.resize(nle);
-    r->lineface.resize(nle);
-    return tbs_structured_mesh(NL, NK, r->epoint.data(), r->ef.data(), r->fc.data(), r->linesize.data(),
-                               r->linepoint.data(), r->lines.data(), r->lineface.data());
+    for (int l = 0; l <= nl; ++l) r->linepoint[l] = l * NK;
+    return TBS_OK;
 }
 
 tbs_mesh_host mesh_view(const Request &r, int nvar) {
@@ -131,16 +142,18 @@ tbs_mesh_host mesh_view(const Request &r, int nvar) {
     mh.nvar = nvar;
     mh.nelem = r.nelem;
     mh.nintface = r.nintface;
-    mh.eftot = (int)r.ef.size();
+    mh.eftot = r.eftot;
     mh.nline = (int)r.linesize.size();
-    mh.nlineelem = (int)r.lines.size();
-    mh.epoint = r.epoint.data();
-    mh.ef = r.ef.data();
-    mh.fc = r.fc.data();
+    mh.nlineelem = r.nlineelem;
     mh.linesize = r.linesize.data();
     mh.linepoint = r.linepoint.data();
-    mh.lines = r.lines.data();
-    mh.lineface = r.lineface.data();
+    if (r.caller) {   // a seeded request's grid has no host element/face tables
+        mh.epoint = r.epoint.data();
+        mh.ef = r.ef.data();
+        mh.fc = r.fc.data();
+        mh.lines = r.lines.data();
+        mh.lineface = r.lineface.data();
+    }
     return mh;
 }
 
@@ -149,7 +162,7 @@ tbs_mesh_host mesh_view(const Request &r, int nvar) {
 int request_sizes(const Request &r, int prec, int nvar, size_t *persist, size_t *total) {
     const tbs_mesh_host mh = mesh_view(r, nvar);
     size_t plan = 0;
-    TBS_TRY(tbs::plan_footprint(&mh, prec, &plan, false));   // the server's own mesh
+    TBS_TRY(tbs::plan_footprint(&mh, prec, &plan, false, !r.caller));   // checked at submit
     const size_t bs = (size_t)nvar * nvar * 8, vec = (size_t)nvar * r.nelem * 8;
     *persist = align_up(plan + 2 * align_up(vec) + kSlack);
     *total = *persist + align_up(bs * r.nelem) + 2 * align_up(bs * std::max(r.nintface, 1)) +
@@ -157,7 +170,7 @@ int request_sizes(const Request &r, int prec, int nvar, size_t *persist, size_t 
     return TBS_OK;
 }
 
-enum { T_WAIT, T_MEMBERS, T_LAUNCH, T_RETIRE, T_ADMIT, T_PLAN, T_NPHASE };
+enum { T_WAIT, T_MEMBERS, T_LAUNCH, T_RETIRE, T_ADMIT, T_PLAN, T_SETUP, T_NPHASE };
 enum { C_STEPS, C_ADMITTED, C_RETIRED, C_REBUILDS, C_NCOUNT };
 
 }  // namespace
@@ -184,6 +197,17 @@ struct PinnedPool {
     }
     void put(double *p, size_t cls) {
         if (p) free[cls].push_back(p);
+    }
+    // n buffers of every class from 4 KiB up to max_bytes, allocated now
+    int prefill(size_t max_bytes, int n) {
+        for (size_t c = 4096; c <= max_bytes; c <<= 1)
+            for (int i = 0; i < n; ++i) {
+                double *p = nullptr;
+                TBS_HIP(hipHostMalloc((void **)&p, c, hipHostMallocDefault));
+                all.push_back(p);
+                free[c].push_back(p);
+            }
+        return TBS_OK;
     }
     void clear() {
         for (double *p : all) hipHostFree(p);
@@ -222,6 +246,14 @@ struct tbs_server {
     double t_phase[T_NPHASE] = {};
     long count[C_NCOUNT] = {};
     double now() const { return std::chrono::duration<double>(clk::now() - t0).count(); }
+    // admission thread: requests handed over by the steps (owned by
+    // `admitting`), the thread's own host-time counters (nanoseconds)
+    std::thread worker;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Request *> handoff;
+    bool stop = false;
+    std::atomic<long long> ns_plan{0}, ns_setup{0};
 };
 
 namespace {
@@ -245,14 +277,16 @@ int admit_setup(tbs_server *s, Request *r) {
     const int nv = mh.nvar;
     const size_t bs = (size_t)nv * nv;
     tbs_ctx *a = s->adm;
-    a->region = s->addr(r->slab_off);
+    a->region = r->slab_ptr;
     a->region_size = r->slab_len;
     a->region_used = 0;
     int rc = TBS_OK;
     do {
-        const double tp0 = s->now();
-        rc = tbs::plan_create(a, &mh, s->prec, &r->plan, false);   // the server's own mesh
-        s->t_phase[T_PLAN] += s->now() - tp0;
+        const auto tp0 = clk::now();
+        // (a caller's mesh was checked at submit; a seeded grid's tables are
+        // generated on the device)
+        rc = tbs::plan_create(a, &mh, s->prec, &r->plan, false, r->caller ? 0 : r->spec.NK);
+        s->ns_plan += std::chrono::duration_cast<std::chrono::nanoseconds>(clk::now() - tp0).count();
         if (rc) break;
         if ((rc = tbs::dev_alloc(a, (size_t)nv * r->nelem * 8, (void **)&r->d_rhs, nullptr))) break;
         if ((rc = tbs::dev_alloc(a, (size_t)nv * r->nelem * 8, (void **)&r->d_U, nullptr))) break;
@@ -309,6 +343,32 @@ int admit_finish(tbs_server *s, const std::vector<Request *> &rs) {
     }
     if (rc) hipStreamSynchronize(a->stream);   // nothing of the slabs may still be in use
     return rc;
+}
+
+// The admission thread: takes the requests the steps hand over, all those
+// waiting at once, sets each up on the admission stream and factorises them
+// in one launch, then publishes the outcome through Request::setup.
+void admission_loop(tbs_server *s) {
+    if (tbs::use_ctx(s->adm) != TBS_OK) return;
+    for (;;) {
+        std::vector<Request *> got;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv.wait(lk, [s] { return s->stop || !s->handoff.empty(); });
+            if (s->handoff.empty()) return;   // stop, nothing left
+            got.assign(s->handoff.begin(), s->handoff.end());
+            s->handoff.clear();
+        }
+        const auto t0 = clk::now();
+        std::vector<Request *> ok;
+        for (Request *r : got) {
+            if (admit_setup(s, r) == TBS_OK) ok.push_back(r);
+            else r->setup.store(-1, std::memory_order_release);
+        }
+        const int rc = ok.empty() ? TBS_OK : admit_finish(s, ok);
+        s->ns_setup += std::chrono::duration_cast<std::chrono::nanoseconds>(clk::now() - t0).count();
+        for (Request *r : ok) r->setup.store(rc == TBS_OK ? 1 : -1, std::memory_order_release);
+    }
 }
 
 // Add a segment able to hold `need` bytes, within the budget.
@@ -378,6 +438,8 @@ int caller_request(tbs_server *s, const tbs_mesh_host *m, std::unique_ptr<Reques
     r->caller = true;
     r->nelem = m->nelem;
     r->nintface = m->nintface;
+    r->eftot = m->eftot;
+    r->nlineelem = m->nlineelem;
     r->epoint.assign(m->epoint, m->epoint + m->nelem + 1);
     r->ef.assign(m->ef, m->ef + m->eftot);
     r->fc.assign(m->fc, m->fc + 2 * (size_t)m->nintface);
@@ -421,13 +483,84 @@ int tbs_server_create(tbs_ctx *ctx, size_t hbm_budget, int precision, double rto
     for (size_t want = s->budget; want >= std::min(kSegment, s->budget) && want > 0; want /= 2)
         if (grow_arena(s.get(), want) == TBS_OK) break;
     hipGetLastError();
+    // the rest of what serving would otherwise allocate on first use, inside
+    // some request's latency: pinned result buffers (two of every size class
+    // up to 32 MiB, a 460,000-element result at nvar 9), the admission
+    // context's table staging ring and device stage, and the batch's tables
+    TBS_TRY(s->pinned.prefill((size_t)32 << 20, 2));
+    for (int i = 0; i < tbs_ctx::kStages; ++i) {
+        char *h = nullptr;
+        TBS_TRY(tbs::stage_acquire(s->adm, (size_t)32 << 20, &h));
+        TBS_TRY(tbs::stage_release(s->adm));
+    }
+    TBS_TRY(tbs::dev_stage(s->adm, (size_t)1 << 20));
+    TBS_TRY(tbs::batch_create_reserved(ctx, ctx->nvar, precision, 256, (size_t)1 << 16, (size_t)1 << 18, &s->batch));
+    hipStream_t aux = nullptr;
+    TBS_TRY(tbs::aux_stream(s->adm, &aux));   // the factorisation's packing stream
+    TBS_TRY(tbs::aux_stream(ctx, &aux));      // the batch's second half
+    // copies of every size a request moves (the runtime picks its copy path
+    // by size, and a path's first use is slow): host -> device on the batch
+    // and admission streams, device -> host on the retirement stream, into the
+    // arena and pinned buffers of the pool
+    if (!s->seg_base.empty()) {
+        double *hbuf = nullptr;
+        size_t cls = 0;
+        const size_t big = std::min<size_t>((size_t)32 << 20, s->seg_len[0]);
+        TBS_TRY(s->pinned.get(big, &hbuf, &cls));
+        for (size_t b = 4096; b <= big; b <<= 2) {
+            TBS_HIP(hipMemcpyAsync(s->seg_base[0], hbuf, b, hipMemcpyHostToDevice, ctx->stream));
+            TBS_HIP(hipMemcpyAsync(s->seg_base[0], hbuf, b, hipMemcpyHostToDevice, s->adm->stream));
+            TBS_HIP(hipMemcpyAsync(hbuf, s->seg_base[0], b, hipMemcpyDeviceToHost, s->ret->stream));
+            TBS_HIP(hipDeviceSynchronize());
+        }
+        s->pinned.put(hbuf, cls);
+    }
+    TBS_HIP(hipDeviceSynchronize());
     s->t0 = clk::now();
+    s->worker = std::thread(admission_loop, s.get());
+    // one small request through the whole pipeline (admission thread, batch,
+    // retirement), so that no first-use cost of a stream, kernel or buffer
+    // lands on a caller's request; then the counters start from zero
+    {
+        tbs_server_request rq{};
+        rq.NL = 2;
+        rq.NK = 64;
+        rq.seed = 1;
+        rq.shift = 4.0 * ctx->nvar;
+        long id = 0;
+        size_t need = 0;
+        TBS_TRY(tbs_server_footprint(s.get(), &rq, &need));
+        bool done = need > s->budget;   // (a budget too small for it: no warm-up)
+        if (!done) TBS_TRY(tbs_server_submit(s.get(), &rq, &id));
+        for (int i = 0; i < 1000000 && !done; ++i) {
+            TBS_TRY(tbs_server_step(s.get(), nullptr, nullptr, nullptr));
+            done = s->finished.count(id) != 0;
+            if (!done) std::this_thread::yield();
+        }
+        TBS_REQUIRE(done, TBS_ERR_STATE, "server warm-up request did not finish");
+        if (id) TBS_TRY(tbs_server_release(s.get(), id));
+        s->completed_unpolled.clear();
+        s->next_id = 1;
+        for (double &t : s->t_phase) t = 0.0;
+        for (long &c : s->count) c = 0;
+        s->ns_plan = 0;
+        s->ns_setup = 0;
+        s->t0 = clk::now();
+    }
     *out = s.release();
     return TBS_OK;
 }
 
 int tbs_server_destroy(tbs_server *s) {
     if (!s) return TBS_OK;
+    if (s->worker.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stop = true;
+        }
+        s->cv.notify_all();
+        s->worker.join();   // finishes the set-ups already handed over
+    }
     hipSetDevice(s->ctx->device);
     hipDeviceSynchronize();
     if (s->batch) tbs_batch_destroy(s->batch);
@@ -545,7 +678,18 @@ int tbs_server_step(tbs_server *s, int *nactive, int *nqueued, int *nadmitting) 
     bool changed = !done.empty();
     for (auto it = s->admitting.begin(); it != s->admitting.end();) {
         Request *r = it->get();
-        if (hipEventQuery(r->ready) != hipSuccess) { ++it; continue; }
+        const int su = r->setup.load(std::memory_order_acquire);
+        if (su < 0) {   // set-up failed: the admission thread is done with it
+            std::unique_ptr<Request> own = std::move(*it);
+            it = s->admitting.erase(it);
+            s->used -= own->slab_len;
+            drop_device(s, own.get());
+            own->status = -1;
+            s->count[C_ADMITTED]--;
+            finish(s, std::move(own));
+            continue;
+        }
+        if (su == 0 || hipEventQuery(r->ready) != hipSuccess) { ++it; continue; }
         // set-up complete: the temporaries of the slab go back to the arena,
         // and a caller's host copies are no longer needed
         r->t_ready = s->now();
@@ -565,8 +709,7 @@ int tbs_server_step(tbs_server *s, int *nactive, int *nqueued, int *nadmitting) 
     if (changed) {
         std::vector<tbs_plan *> plans;
         for (auto &r : s->active) plans.push_back(r->plan);
-        if (s->batch) TBS_TRY(tbs_batch_set_plans(s->batch, plans.data(), (int)plans.size()));
-        else if (!plans.empty()) TBS_TRY(tbs_batch_create(s->ctx, plans.data(), (int)plans.size(), &s->batch));
+        TBS_TRY(tbs_batch_set_plans(s->batch, plans.data(), (int)plans.size()));
         s->count[C_REBUILDS]++;
     }
     lap(T_MEMBERS);
@@ -583,10 +726,11 @@ int tbs_server_step(tbs_server *s, int *nactive, int *nqueued, int *nadmitting) 
         for (auto &r : done) {
             r->nU = (size_t)nv * r->nelem;
             TBS_TRY(s->pinned.get(r->nU * 8, &r->hU, &r->hU_cls));
-            r->hist.resize(r->nsweeps + 1);
+            r->nH = (size_t)r->nsweeps + 1;
+            TBS_TRY(s->pinned.get(r->nH * 8, &r->hH, &r->hH_cls));
             TBS_TRY(tbs::plan_scatter_U(r->plan, rs, r->d_U));
             TBS_HIP(hipMemcpyAsync(r->hU, r->d_U, r->nU * 8, hipMemcpyDeviceToHost, rs));
-            TBS_HIP(hipMemcpyAsync(r->hist.data(), tbs::plan_hist(r->plan), r->hist.size() * 8,
+            TBS_HIP(hipMemcpyAsync(r->hH, tbs::plan_hist(r->plan), r->nH * 8,
                                    hipMemcpyDeviceToHost, rs));
             TBS_HIP(hipEventRecord(r->ready, rs));
             r->t_retire = s->now();
@@ -598,7 +742,8 @@ int tbs_server_step(tbs_server *s, int *nactive, int *nqueued, int *nadmitting) 
     lap(T_RETIRE);
 
     // 5. admission: FIFO; later requests may pass a head that does not fit
-    //    until the head has been passed over kMaxSkips times
+    //    until the head has been passed over kMaxSkips times.  An admitted
+    //    request's set-up goes to the admission thread.
     std::vector<Request *> fresh;
     for (auto it = s->queue.begin(); it != s->queue.end();) {
         Request *r = it->get();
@@ -614,34 +759,20 @@ int tbs_server_step(tbs_server *s, int *nactive, int *nqueued, int *nadmitting) 
             continue;
         }
         r->slab_len = r->footprint;
+        r->slab_ptr = s->addr(r->slab_off);
         s->used += r->footprint;
         r->t_admit = s->now();
-        const int rc = admit_setup(s, r);
-        std::unique_ptr<Request> own = std::move(*it);
-        it = s->queue.erase(it);
-        if (rc) {
-            s->used -= own->slab_len;
-            drop_device(s, own.get());
-            own->status = -1;
-            finish(s, std::move(own));
-            continue;
-        }
         s->count[C_ADMITTED]++;
-        fresh.push_back(own.get());
-        s->admitting.push_back(std::move(own));
+        fresh.push_back(r);
+        s->admitting.push_back(std::move(*it));
+        it = s->queue.erase(it);
     }
-    if (!fresh.empty() && admit_finish(s, fresh) != TBS_OK) {
-        // none of this step's admissions is usable: fail them all
-        for (auto it = s->admitting.begin(); it != s->admitting.end();) {
-            if (std::find(fresh.begin(), fresh.end(), it->get()) == fresh.end()) { ++it; continue; }
-            std::unique_ptr<Request> own = std::move(*it);
-            it = s->admitting.erase(it);
-            s->used -= own->slab_len;
-            drop_device(s, own.get());
-            own->status = -1;
-            s->count[C_ADMITTED]--;
-            finish(s, std::move(own));
+    if (!fresh.empty()) {
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->handoff.insert(s->handoff.end(), fresh.begin(), fresh.end());
         }
+        s->cv.notify_one();
     }
     lap(T_ADMIT);
 
@@ -674,7 +805,7 @@ int tbs_server_result(tbs_server *s, long id, double *U, size_t ucap, double *hi
         TBS_REQUIRE(ucap >= r->nU, TBS_ERR_ARG, "U capacity %zu < %zu", ucap, r->nU);
         if (r->nU) std::memcpy(U, r->hU, r->nU * 8);
     }
-    if (hist && hcap > 0) std::memcpy(hist, r->hist.data(), std::min<size_t>(hcap, r->hist.size()) * 8);
+    if (hist && hcap > 0 && r->nH) std::memcpy(hist, r->hH, std::min<size_t>(hcap, r->nH) * 8);
     if (nsweeps) *nsweeps = r->nsweeps;
     if (status) *status = r->status;
     if (times) {
@@ -690,6 +821,7 @@ int tbs_server_release(tbs_server *s, long id) {
     auto it = s->finished.find(id);
     if (it != s->finished.end()) {
         s->pinned.put(it->second->hU, it->second->hU_cls);
+        s->pinned.put(it->second->hH, it->second->hH_cls);
         s->finished.erase(it);
     }
     return TBS_OK;
@@ -705,6 +837,8 @@ int tbs_server_info(tbs_server *s, size_t *budget, size_t *used, double *now) {
 
 int tbs_server_stats(tbs_server *s, double *seconds, int nseconds, long *counts, int ncounts) {
     TBS_REQUIRE(s, TBS_ERR_ARG, "null server");
+    s->t_phase[T_PLAN] = 1e-9 * (double)s->ns_plan.load();
+    s->t_phase[T_SETUP] = 1e-9 * (double)s->ns_setup.load();
     for (int i = 0; seconds && i < nseconds && i < T_NPHASE; ++i) seconds[i] = s->t_phase[i];
     for (int i = 0; counts && i < ncounts && i < C_NCOUNT; ++i) counts[i] = s->count[i];
     return TBS_OK;

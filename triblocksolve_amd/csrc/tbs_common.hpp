@@ -98,15 +98,26 @@ constexpr size_t kRegionAlign = 256;
 // Internal access to a plan for the serving engine (same library).
 // tbs_plan_create, with the mesh consistency checks skipped when !validate
 // (for meshes the library built itself).
-int plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_plan **out, bool validate);
-// tbs_plan_footprint, with the same choice.
-int plan_footprint(const tbs_mesh_host *mesh, int precision, size_t *bytes, bool validate);
+// structured_nk > 0: m is tbs_structured_mesh's grid of m->nline lines of
+// structured_nk elements given by its sizes and line arrays only (epoint, ef,
+// fc, lines, lineface may be null); the plan generates those on the device.
+int plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_plan **out, bool validate,
+                int structured_nk = 0);
+// tbs_plan_footprint, with the same choices (structured: such a grid).
+int plan_footprint(const tbs_mesh_host *mesh, int precision, size_t *bytes, bool validate,
+                   bool structured = false);
 // Scatter the plan's solution into U (device, element order) on stream st.
 int plan_scatter_U(tbs_plan *p, hipStream_t st, double *U);
 // Device residual history [kHistCap] of a plan.
 const double *plan_hist(const tbs_plan *p);
 // Bytes the plan's factorisation draws from a context region.
 size_t plan_factor_workspace(int nvar, int nelem);
+// An empty batch (block size nvar, precision prec) whose device tables and
+// pinned staging already hold nsys systems, nlines lines and ngroups
+// residual groups, so that tbs_batch_set_plans within those sizes allocates
+// nothing (the serving engine creates its batch this way).
+int batch_create_reserved(tbs_ctx *ctx, int nvar, int prec, size_t nsys, size_t nlines, size_t ngroups,
+                          tbs_batch **out);
 
 // Internal-order vectors of a factored plan (n elements x nvar values each,
 // line-major order) for solvers built on the plan's kernels.

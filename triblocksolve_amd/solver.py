@@ -725,8 +725,9 @@ class Server:
 
     def stats(self):
         """Cumulative host seconds per engine phase and event counts."""
-        t = np.zeros(6)
+        t = np.zeros(7)
         c = np.zeros(4, np.int64)
-        A.check(self.ctx._lib.tbs_server_stats(self.handle, _ptr(t), 6, _ptr(c), 4))
-        return dict(zip(("wait", "membership", "launch", "retire", "admit", "admit_plan_build"), t.tolist()),
+        A.check(self.ctx._lib.tbs_server_stats(self.handle, _ptr(t), 7, _ptr(c), 4))
+        return dict(zip(("wait", "membership", "launch", "retire", "admit", "admit_plan_build", "admit_thread"),
+                        t.tolist()),
                     **dict(zip(("steps", "admitted", "retired", "membership_updates"), c.tolist())))
