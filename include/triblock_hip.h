@@ -354,7 +354,11 @@ typedef struct tbs_server_request {
 } tbs_server_request;
 /* hbm_budget 0 = 80% of the device's free memory at creation.  The server's
  * HBM arena is allocated at the full budget here (in smaller pieces later
- * only if the device refuses that), so admissions allocate nothing. */
+ * only if the device refuses that), so admissions allocate nothing; so are
+ * its pinned result buffers, staging and batch tables, and one small
+ * request runs through it before the call returns.  The server starts one
+ * host thread of its own (admission set-up); the tbs_server_* calls on a
+ * server must come from one thread at a time. */
 int tbs_server_create(tbs_ctx *ctx, size_t hbm_budget, int precision, double rtol,
                       int max_sweeps, int sweeps_per_step, tbs_server **out);
 int tbs_server_destroy(tbs_server *srv);
