@@ -349,7 +349,7 @@ int admit_finish(tbs_server *s, const std::vector<Request *> &rs) {
 // waiting at once, sets each up on the admission stream and factorises them
 // in one launch, then publishes the outcome through Request::setup.
 void admission_loop(tbs_server *s) {
-    if (tbs::use_ctx(s->adm) != TBS_OK) return;
+    const bool usable = tbs::use_ctx(s->adm) == TBS_OK;   // (if not, every set-up fails)
     for (;;) {
         std::vector<Request *> got;
         {
@@ -362,7 +362,7 @@ void admission_loop(tbs_server *s) {
         const auto t0 = clk::now();
         std::vector<Request *> ok;
         for (Request *r : got) {
-            if (admit_setup(s, r) == TBS_OK) ok.push_back(r);
+            if (usable && admit_setup(s, r) == TBS_OK) ok.push_back(r);
             else r->setup.store(-1, std::memory_order_release);
         }
         const int rc = ok.empty() ? TBS_OK : admit_finish(s, ok);
