@@ -725,12 +725,16 @@ int tbs_plan_info_get(tbs_plan *p, tbs_plan_info *info) {
     info->device_bytes = p->bytes;
     info->alg_bytes_per_sweep = model_bytes(&mh, p->prec);
     info->ref_bytes_per_sweep = 0.0;
-    // per-kernel algorithmic bytes of the fused form:
-    //   Thomas: E + Gamma factor blocks (tf), g in, d' out + in, U in + out
+    // per-kernel algorithmic bytes of the fused form, the sweep's model bytes
+    // split between its two kernels (they add up to alg_bytes_per_sweep at
+    // FP64):
+    //   Thomas: E + Gamma factor blocks (tf), g in, U in + out (the forward
+    //   intermediate d' it writes and reads back is the implementation's, not
+    //   the algorithm's: 2 tf nv n more bytes moved)
     //   residual: D + Of1 + Of2 (FP64), W (tf), U, rhs in, g out
     const double bs = (double)p->nv * p->nv, nv = p->nv, tf = (double)tf_size(p->prec);
     const double n = p->nle;
-    info->thomas_bytes = tf * bs * 2.0 * (n - p->nline) + nv * n * (tf * 3.0 + 16.0);
+    info->thomas_bytes = tf * bs * 2.0 * (n - p->nline) + nv * n * (tf + 16.0);
     info->residual_bytes = 8.0 * bs * (n + 2.0 * p->nintface) + tf * bs * n + nv * n * (16.0 + tf);
     return TBS_OK;
 }
