@@ -635,15 +635,23 @@ int tbs::plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pl
     // a pageable image kept by the plan so the upload stays asynchronous
     constexpr size_t kStageMax = (size_t)64 << 20;
     const bool staged = T.host_end <= kStageMax || gen;
+    // (a generated grid stages only its line arrays and its system record:
+    // the image holds [linepoint, fc) and then [sys, host_end), and put()
+    // shifts the offsets accordingly)
+    const size_t lo_end = T.fc - T.linepoint;
+    const size_t img = gen ? lo_end + (T.host_end - T.sys) : T.host_end;
     char *h = nullptr;
     if (staged) {
-        TBS_TRY(tbs::stage_acquire(ctx, T.host_end, &h));
+        TBS_TRY(tbs::stage_acquire(ctx, img, &h));
     } else {
         P->h_tables.reset(new char[T.host_end]);
         h = P->h_tables.get();
     }
     const int nle = P->nle, F = L.F, Fm = std::max(F, 1);
-    auto put = [h](size_t off, const void *src_, size_t bytes) { if (bytes) std::memcpy(h + off, src_, bytes); };
+    auto at = [&](size_t off) -> size_t {
+        return !gen ? off : (off >= T.sys ? lo_end + (off - T.sys) : off - T.linepoint);
+    };
+    auto put = [&](size_t off, const void *src_, size_t bytes) { if (bytes) std::memcpy(h + at(off), src_, bytes); };
     if (!gen) put(T.lines, m->lines, (size_t)nle * 4);
     put(T.linepoint, m->linepoint, (size_t)(P->nline + 1) * 4);
     put(T.linesize, m->linesize, (size_t)P->nline * 4);
@@ -665,8 +673,8 @@ int tbs::plan_create(tbs_ctx *ctx, const tbs_mesh_host *m, int precision, tbs_pl
     if (gen) {
         // the line arrays and the system record from the host; the element
         // and face tables from the generator kernel
-        TBS_HIP(hipMemcpyAsync(tb + T.linepoint, h + T.linepoint, T.fc - T.linepoint, hipMemcpyHostToDevice, st));
-        TBS_HIP(hipMemcpyAsync(tb + T.sys, h + T.sys, T.host_end - T.sys, hipMemcpyHostToDevice, st));
+        TBS_HIP(hipMemcpyAsync(tb + T.linepoint, h, lo_end, hipMemcpyHostToDevice, st));
+        TBS_HIP(hipMemcpyAsync(tb + T.sys, h + lo_end, T.host_end - T.sys, hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(structured_tables_kernel, dim3(gs_grid(nle)), dim3(256), 0, st, m->nline, structured_nk,
                            P->d_lines, P->d_fc, P->d_lineface, (int *)(tb + T.epoint), (int *)(tb + T.ef));
         TBS_LAUNCHED("structured_tables_kernel");
