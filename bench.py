@@ -42,6 +42,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="only the FP64 timed region")
     ap.add_argument("--no-oracle", action="store_true", help="skip the errors against the FP64 oracle")
+    ap.add_argument("--section", choices=["batched", "serving"], default=None,
+                    help="run only this extra section (cfg3 batch or cfg5 serving, without the oracle) and "
+                         "print its JSON; for profiling one workload")
     ap.add_argument("--cpu-baseline-only", action="store_true",
                     help="(internal) run only the CPU baseline leg and print its JSON; no GPU use")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -526,6 +529,10 @@ def main():
     import triblocksolve_amd as tbs
 
     ctx = tbs.Context(0, 9)
+    if args.section:
+        fn = bench_batched if args.section == "batched" else bench_serving
+        print(json.dumps({args.section: fn(tbs, ctx, args)}))
+        return
     m = tbs.structured_mesh(args.nl, args.nk)
     n, nif = m.nelem, m.nintface
     z = lambda k: ctx.array(size=k, zero=False)
