@@ -364,10 +364,28 @@ int launch_factor(int nv, int prec, hipStream_t st, const FactorRec &one, int nl
     return TBS_OK;
 }
 
+// Workgroups (lines) of the factor kernel resident at once on the context's
+// device; 0 when not known (block sizes of the wide factor kernel)
+int factor_round(tbs_ctx *ctx, int nv, int prec) {
+    if (nv < 2 || nv > 16) return 0;
+    int &c = ctx->factor_round[prec == TBS_FP32 ? 1 : 0][nv];
+    if (c == 0) {
+        int per = 0, cus = 0;
+        switch (nv) {
+#define X(v) case v: per = prec == TBS_FP32 ? factor_blocks_per_cu<v, float>() : factor_blocks_per_cu<v, double>(); break;
+            TBS_PLAN_NVARS(X)
+#undef X
+        }
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) cus = 0;
+        c = per > 0 && cus > 0 ? per * cus : -1;
+    }
+    return c > 0 ? c : 0;
+}
+
 int launch_pack(const tbs_plan *p, hipStream_t pst, const double *jacD, const double *jacO1, const double *jacO2) {
     const int nv = p->nv, BS = nv * nv;
     const long nws = (long)p->ngroups * p->S;
-    if (2 * nws < INT_MAX && nv <= 16) {
+    if (nv <= 16) {
         switch (nv) {
 #define X(v) case v: launch_pack_lds<v>(pst, nws, p->S, p->d_src, jacD, jacO1, jacO2, p->d_blk); break;
             TBS_PLAN_NVARS(X)
@@ -778,11 +796,32 @@ int tbs_plans_factor(tbs_ctx *ctx, tbs_plan *const *plans, int n, const double *
     }
     hipStream_t pst;
     TBS_TRY(tbs::aux_stream(ctx, &pst));
-    TBS_HIP(hipEventRecord(ctx->ev_fork, st));
-    TBS_HIP(hipStreamWaitEvent(pst, ctx->ev_fork, 0));
+    // The factor kernel and the packing slow each other down (both wait on
+    // memory; together they take about what they take one after the other).
+    // When the lines take between one and two rounds of resident factor
+    // workgroups, the first round runs alone and the packing runs beside the
+    // second, short one: at 1,200 lines of 1,280 (1,024 resident) 5.1 ms
+    // against 5.6 ms with the packing beside the whole factorisation.
+    const int R = factor_round(ctx, nv, prec);
+    const long nA = R > 0 && nl > R && nl < 2L * R ? R : 0;
+    if (nA == 0) {
+        TBS_HIP(hipEventRecord(ctx->ev_fork, st));
+        TBS_HIP(hipStreamWaitEvent(pst, ctx->ev_fork, 0));
+    }
     if (n == 1) {
-        TBS_TRY(launch_factor(nv, prec, st, factor_rec(plans[0], jacD[0], jacO1[0], jacO2[0]), plans[0]->nline,
-                              nullptr, nullptr));
+        const FactorRec fr = factor_rec(plans[0], jacD[0], jacO1[0], jacO2[0]);
+        if (nA > 0) {
+            TBS_TRY(launch_factor(nv, prec, st, fr, (int)nA, nullptr, nullptr));
+            TBS_HIP(hipEventRecord(ctx->ev_fork, st));
+            TBS_HIP(hipStreamWaitEvent(pst, ctx->ev_fork, 0));
+            FactorRec fb = fr;
+            fb.linepoint += nA;
+            fb.linesize += nA;
+            fb.woff += nA;
+            TBS_TRY(launch_factor(nv, prec, st, fb, (int)(nl - nA), nullptr, nullptr));
+        } else {
+            TBS_TRY(launch_factor(nv, prec, st, fr, (int)nl, nullptr, nullptr));
+        }
     } else {
         // one launch over every plan's lines, longest first; the records and
         // the line table go up through the staging ring into a device buffer
@@ -804,14 +843,22 @@ int tbs_plans_factor(tbs_ctx *ctx, tbs_plan *const *plans, int n, const double *
         TBS_TRY(tbs::dev_stage(ctx, bytes));
         TBS_HIP(hipMemcpyAsync(ctx->dstage, h, bytes, hipMemcpyHostToDevice, st));
         TBS_TRY(tbs::stage_release(ctx));
-        TBS_TRY(launch_factor(nv, prec, st, FactorRec{}, (int)nl, (const FactorRec *)ctx->dstage,
-                              (const int2 *)(ctx->dstage + lt_off)));
+        const FactorRec *drec = (const FactorRec *)ctx->dstage;
+        const int2 *dlt = (const int2 *)(ctx->dstage + lt_off);
+        if (nA > 0) {
+            TBS_TRY(launch_factor(nv, prec, st, FactorRec{}, (int)nA, drec, dlt));
+            TBS_HIP(hipEventRecord(ctx->ev_fork, st));
+            TBS_HIP(hipStreamWaitEvent(pst, ctx->ev_fork, 0));
+            TBS_TRY(launch_factor(nv, prec, st, FactorRec{}, (int)(nl - nA), drec, dlt + nA));
+        } else {
+            TBS_TRY(launch_factor(nv, prec, st, FactorRec{}, (int)nl, drec, dlt));
+        }
     }
     // residual blocks, packed on the context's second stream while the
-    // factor kernel (latency-bound, two waves per line) runs on the main one:
-    // queued after the factor kernel (its waves take their CUs first) and
-    // forked after the memsets; the main stream waits for the packing before
-    // anything after the call
+    // factor kernel (two waves per line) runs on the main one: queued after
+    // the factor kernel (its waves take their CUs first) and forked after the
+    // memsets, or after the first round; the main stream waits for the
+    // packing before anything after the call
     for (int i = 0; i < n; ++i) TBS_TRY(launch_pack(plans[i], pst, jacD[i], jacO1[i], jacO2[i]));
     TBS_HIP(hipEventRecord(ctx->ev_join, pst));
     TBS_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));

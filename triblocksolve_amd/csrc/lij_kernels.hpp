@@ -678,63 +678,95 @@ __global__ void pack_blocks_kernel(long total, int S, const int2 *__restrict__ s
     }
 }
 
-// Same packing through LDS: one 256-thread workgroup per (group, slot) reads
-// the group's 64 source blocks whole (consecutive threads read consecutive
-// entries of a block, so every load is contiguous), then writes the slot in
-// SoA-64 order (consecutive threads write consecutive addresses).
-// One 128-thread workgroup per half of a (group, slot): the 32 source blocks
-// of lanes h*32 .. h*32+31 come into LDS whole, in 16-byte pieces (consecutive
-// threads read consecutive pieces of a block; a block is contiguous in the
-// source array, 8-byte aligned), and leave as those lanes' part of the SoA-64
-// slot: 512-byte runs of 16-byte pieces.  (Half groups keep the LDS of a
-// workgroup at 21 KB, so seven fit a CU.)
+// Same packing through LDS, one wave per quarter of a (group, slot): the 16
+// source blocks of lanes q*16 .. q*16+15 come into LDS whole in 16-byte
+// pieces (consecutive threads read consecutive pieces of a block, which is
+// contiguous in the source array and 8-byte aligned; the last piece of an odd
+// block is read as the 16 bytes that end the block) and leave as those lanes'
+// part of the SoA-64 slot: 256-byte runs of 16-byte pieces.  Each thread
+// issues its loads NB at a time before writing any to LDS, so a wave keeps NB
+// KiB in flight; 10.5 KB of LDS and 41 VGPRs (9x9 blocks) let the waves sit
+// beside the factor kernel's.  The grid strides over the units (any grid size
+// is right).
 typedef double f64x2u __attribute__((ext_vector_type(2), aligned(8)));
-constexpr int kPackLanes = 32;
-template <int NV>
-__global__ __launch_bounds__(128) void pack_lds_kernel(int S, const int2 *__restrict__ src,
-                                                       const double *__restrict__ D,
-                                                       const double *__restrict__ O1,
-                                                       const double *__restrict__ O2,
-                                                       double *__restrict__ blk) {
+constexpr int kPackLanes = 16;
+template <int NV, int NB = 6>
+__global__ __launch_bounds__(64) void pack_lds_kernel(long nunits, int S, const int2 *__restrict__ src,
+                                                      const double *__restrict__ D,
+                                                      const double *__restrict__ O1,
+                                                      const double *__restrict__ O2,
+                                                      double *__restrict__ blk) {
     constexpr int BS = NV * NV;
-    constexpr int PB = (BS + 1) / 2;           // 16-byte pieces per block (the last may be 8 bytes)
     constexpr int EL = kPackLanes;
+    constexpr int PB = (BS + 1) / 2;           // 16-byte pieces per block
+    constexpr int NP = EL * PB;                // pieces per unit
+    constexpr int NIT = (NP + 63) / 64;        // pieces per thread
+    constexpr int NU = kGroup / EL;            // units per (group, slot)
     using L = tbs::Soa64<BS, double>;
     static_assert(L::VW == 2, "SoA-64 slots of doubles hold 2 entries per 16-byte chunk");
-    __shared__ double s[EL * BS];              // [element lane - h*EL][entry]
-    __shared__ const double *s_src[EL];
-    const long w = blockIdx.x >> 1;            // (group, slot)
-    const int h = (int)(blockIdx.x & 1u);      // half of the group's lanes
-    const long g = w / S;
-    const int slot = (int)(w - g * S);
+    using gsrc = const __attribute__((address_space(1))) f64x2u *;
+    using gdst = __attribute__((address_space(1))) f64x2u *;
+    __shared__ double s[EL * BS];              // [element lane - q*EL][entry]
+    __shared__ uintptr_t s_src[EL];            // block addresses (0: the zero block)
     const int t = threadIdx.x;
-    if (t < EL) {
-        const int2 c = src[(g * kGroup + h * EL + t) * S + slot];
-        const double *a = c.x == 1 ? D : (c.x == 2 ? O1 : (c.x == 3 ? O2 : nullptr));
-        s_src[t] = a ? a + (size_t)BS * c.y : nullptr;
-    }
-    __syncthreads();
-    for (int i = t; i < EL * PB; i += 128) {
-        const int e = i / PB, k = i - e * PB;
-        const double *p = s_src[e];
-        double *d = s + e * BS + 2 * k;
-        if (2 * k + 1 < BS) {
-            const f64x2u v = p ? *reinterpret_cast<const f64x2u *>(p + 2 * k) : f64x2u{0.0, 0.0};
-            d[0] = v[0];
-            d[1] = v[1];
-        } else {
-            d[0] = p ? p[2 * k] : 0.0;
+    for (long unit = blockIdx.x; unit < nunits; unit += gridDim.x) {
+        const long w = unit / NU;              // (group, slot)
+        const int q = (int)(unit - w * NU);
+        const long g = w / S;
+        const int slot = (int)(w - g * S);
+        if (t < EL) {
+            const int2 c = src[(g * kGroup + q * EL + t) * S + slot];
+            const double *a = c.x == 1 ? D : (c.x == 2 ? O1 : (c.x == 3 ? O2 : nullptr));
+            s_src[t] = a ? (uintptr_t)(a + (size_t)BS * c.y) : 0;
         }
+        __syncthreads();
+#pragma unroll
+        for (int b0 = 0; b0 < NIT; b0 += NB) {
+            f64x2u v[NB];
+#pragma unroll
+            for (int r = 0; r < NB; ++r) {
+                const int i = t + 64 * (b0 + r);
+                if (b0 + r < NIT) {
+                    const int e = i < NP ? i / PB : 0, k = i < NP ? i - e * PB : 0;
+                    const uintptr_t a = s_src[e];
+                    const bool odd_last = (BS & 1) && k == PB - 1;
+                    // a missing block reads D's first block and keeps zeros
+                    const uintptr_t at = (a ? a : (uintptr_t)D) + (uintptr_t)(odd_last ? 8 * (BS - 2) : 16 * k);
+                    v[r] = *(gsrc)at;
+                    if (!a) v[r] = f64x2u{0.0, 0.0};
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < NB; ++r) {
+                const int i = t + 64 * (b0 + r);
+                if (b0 + r < NIT && i < NP) {
+                    const int e = i / PB, k = i - e * PB;
+                    double *d = s + e * BS + 2 * k;
+                    if ((BS & 1) && k == PB - 1) {
+                        d[0] = v[r][1];
+                    } else {
+                        d[0] = v[r][0];
+                        d[1] = v[r][1];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        double *out = blk + w * L::ELEMS;
+        constexpr int NO = (L::NC * EL + 63) / 64;
+#pragma unroll
+        for (int r = 0; r < NO; ++r) {         // chunk c, lane q*EL + e: entries 2c, 2c+1
+            const int j = t + 64 * r;
+            if (j < L::NC * EL) {
+                const int c = j / EL, e = j - c * EL;
+                const double *x = s + e * BS + 2 * c;
+                *(gdst)(out + c * 128 + (q * EL + e) * 2) = f64x2u{x[0], x[1]};
+            }
+        }
+        for (int j = t; j < L::REM * EL; j += 64)   // odd BS: the last entry, [lane]
+            out[L::NC * 128 + q * EL + j] = s[j * BS + BS - 1];
+        __syncthreads();                        // LDS reused by the next unit
     }
-    __syncthreads();
-    double *out = blk + w * L::ELEMS;
-    for (int i = t; i < L::NC * EL; i += 128) {              // chunk c, lane h*EL + e: entries 2c, 2c+1
-        const int c = i / EL, e = i - c * EL;
-        const double *q = s + e * BS + 2 * c;
-        *reinterpret_cast<f64x2u *>(out + c * 128 + (h * EL + e) * 2) = f64x2u{q[0], q[1]};
-    }
-    for (int i = t; i < L::REM * EL; i += 128)               // odd BS: the last entry, [lane]
-        out[L::NC * 128 + h * EL + i] = s[i * BS + BS - 1];
 }
 
 // ------------------------------------------------------------- factor --
@@ -1378,7 +1410,7 @@ __global__ void structured_tables_kernel(int NL, int NK, int *__restrict__ lines
         auto f_in = [NL, NK](int l2, int k2) {        // face (e+1, e) of element (l2, k2), k2 < NK-1
             return l2 * (2 * NK - 1) + k2 * (1 + (l2 < NL - 1 ? 1 : 0));
         };
-        auto f_cross = [NL, NK, &f_in](int l2, int k2) {   // face (e+NK, e), l2 < NL-1
+        auto f_cross = [NK, &f_in](int l2, int k2) {   // face (e+NK, e), l2 < NL-1
             return f_in(l2, k2) + (k2 < NK - 1 ? 1 : 0);
         };
         auto bnd = [](int b) { return -b - 2; };
@@ -1631,14 +1663,29 @@ void launch_factor_nv(hipStream_t st, const FactorRec &one, int nline, const Fac
         hipLaunchKernelGGL((ffactor_wide_kernel<NV, TF>), dim3(nline), dim3(128), 0, st, one, nline, recs, ltab);
 }
 
-// The LDS packing holds 32 whole blocks per workgroup (up to 64 KiB at
-// NV = 16); larger blocks take the direct packing.
+// Workgroups of ffactor_kernel<NV, TF> resident on one CU (0: not known, or
+// a block size the wide factor kernel takes)
+template <int NV, typename TF>
+int factor_blocks_per_cu() {
+    int nb = 0;
+    if constexpr (4 * NV <= 64) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void *>(&ffactor_kernel<NV, TF>),
+                                                         128, 0) != hipSuccess)
+            nb = 0;
+    }
+    return nb;
+}
+
+// The LDS packing holds 16 whole blocks per workgroup (32 KiB at NV = 16);
+// larger blocks take the direct packing.
 template <int NV>
 void launch_pack_lds(hipStream_t pst, long nws, int S, const int2 *src, const double *jacD, const double *jacO1,
                      const double *jacO2, double *blk) {
-    if constexpr (NV <= 16)
-        hipLaunchKernelGGL(pack_lds_kernel<NV>, dim3((unsigned)(2 * nws)), dim3(128), 0, pst, S, src, jacD, jacO1,
-                           jacO2, blk);
+    if constexpr (NV <= 16) {
+        const long units = (long)(kGroup / kPackLanes) * nws;
+        hipLaunchKernelGGL(pack_lds_kernel<NV>, dim3((unsigned)std::min<long>(units, 1L << 24)), dim3(64), 0, pst, units,
+                           S, src, jacD, jacO1, jacO2, blk);
+    }
 }
 
 }  // namespace

@@ -44,6 +44,9 @@ struct tbs_ctx {
     // stream order on the context's stream).
     char *dstage = nullptr;
     size_t dstage_cap = 0;
+    // Workgroups of the factor kernel resident at once on the device, per
+    // (FP32 factors?, nvar); 0 until asked, -1 not known
+    int factor_round[2][17] = {};
 };
 
 namespace tbs {
