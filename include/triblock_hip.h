@@ -398,8 +398,10 @@ int tbs_server_step(tbs_server *srv, int *nactive, int *nqueued, int *nadmitting
 /* ids of requests finished since the last poll */
 int tbs_server_poll(tbs_server *srv, long *ids, int cap, int *n);
 /* U (host, element order), residual history, sweeps, status (1 converged,
- * 2 non-finite, 3 sweep cap, 4 diverging, -1 admission failed) and
- * times[3] = submit, admit, done (seconds on the server's clock).          */
+ * 2 non-finite, 3 sweep cap, 4 diverging, -1 admission failed: the call
+ * still returns TBS_OK and tbs_last_error() then gives the admission's
+ * message) and times[3] = submit, admit, done (seconds on the server's
+ * clock).                                                                   */
 int tbs_server_result(tbs_server *srv, long id, double *U, size_t ucap, double *hist,
                       int hcap, int *nsweeps, int *status, double *times);
 int tbs_server_release(tbs_server *srv, long id);

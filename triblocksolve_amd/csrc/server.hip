@@ -35,6 +35,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -103,8 +104,10 @@ struct Request {
     char *slab_ptr = nullptr;   // device address of the slab (set before the hand-off)
     int skipped = 0;
     // admission thread -> serving thread: 0 set-up pending, 1 set-up enqueued
-    // (ready recorded), -1 set-up failed (slab no longer in use)
+    // (ready recorded), -1 set-up failed (slab no longer in use; err holds
+    // the admission thread's error message)
     std::atomic<int> setup{0};
+    std::string err;
     std::vector<int> epoint, ef, fc, linesize, linepoint, lines, lineface;
     tbs_plan *plan = nullptr;
     double *d_rhs = nullptr, *d_U = nullptr;   // element order, in the slab
@@ -362,12 +365,19 @@ void admission_loop(tbs_server *s) {
         const auto t0 = clk::now();
         std::vector<Request *> ok;
         for (Request *r : got) {
-            if (usable && admit_setup(s, r) == TBS_OK) ok.push_back(r);
-            else r->setup.store(-1, std::memory_order_release);
+            if (usable && admit_setup(s, r) == TBS_OK) {
+                ok.push_back(r);
+            } else {
+                r->err = usable ? tbs_last_error() : "admission device could not be selected";
+                r->setup.store(-1, std::memory_order_release);
+            }
         }
         const int rc = ok.empty() ? TBS_OK : admit_finish(s, ok);
         s->ns_setup += std::chrono::duration_cast<std::chrono::nanoseconds>(clk::now() - t0).count();
-        for (Request *r : ok) r->setup.store(rc == TBS_OK ? 1 : -1, std::memory_order_release);
+        for (Request *r : ok) {
+            if (rc != TBS_OK) r->err = tbs_last_error();
+            r->setup.store(rc == TBS_OK ? 1 : -1, std::memory_order_release);
+        }
     }
 }
 
@@ -808,6 +818,7 @@ int tbs_server_result(tbs_server *s, long id, double *U, size_t ucap, double *hi
     if (hist && hcap > 0 && r->nH) std::memcpy(hist, r->hH, std::min<size_t>(hcap, r->nH) * 8);
     if (nsweeps) *nsweeps = r->nsweeps;
     if (status) *status = r->status;
+    if (r->status == -1 && !r->err.empty()) tbs::set_error("admission failed: %s", r->err.c_str());
     if (times) {
         times[0] = r->t_submit;
         times[1] = r->t_admit;

@@ -710,8 +710,11 @@ class Server:
         times = np.zeros(3)
         A.check(self.ctx._lib.tbs_server_result(self.handle, rid, _ptr(U), 0 if U is None else U.size, _ptr(hist),
                                                 hist.size, C.byref(ns), C.byref(st), _ptr(times)))
-        return dict(U=U, hist=hist[: ns.value + 1], nsweeps=ns.value, status=st.value, t_submit=times[0],
-                    t_admit=times[1], t_done=times[2])
+        out = dict(U=U, hist=hist[: ns.value + 1], nsweeps=ns.value, status=st.value, t_submit=times[0],
+                   t_admit=times[1], t_done=times[2])
+        if st.value == -1:                 # admission failed: the library's message says why
+            out["error"] = self.ctx._lib.tbs_last_error().decode()
+        return out
 
     def release(self, rid):
         A.check(self.ctx._lib.tbs_server_release(self.handle, rid))
