@@ -6,6 +6,8 @@ Tolerances (stated per result):
   FP32 factors: converge to h_k/h_0 <= 1e-10 and ||U - U*|| <= 1e-9 ||U*||,
         U* = FP64 oracle solution run to the round-off floor.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -85,6 +87,31 @@ def test_variants_fp64(tbs, ctxs, oracle, name, make):
     ok, worst = cf.history_close(out["hist"], ref["hist"])
     assert ok, worst
     assert relerr(out["U"], ref["U"]) < 1e-12
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_factor_in_two_rounds_ragged(tbs, ctxs, oracle, prec):
+    """1,100 ragged lines (seeded lengths 1..90): more lines than the factor
+    kernel's resident workgroups (1,024 for 9x9 blocks on a 256-CU MI355X),
+    so the factorisation runs its first round alone and the rest beside the
+    block packing.  FP64 against the oracle's same sweeps; FP32 factors to
+    1e-9 of the FP64 oracle's converged solution."""
+    rng = np.random.default_rng(77)
+    lens = rng.integers(1, 91, size=1100)
+    m = tbs.ragged_mesh(lens, nvar=9)
+    print(f"case ragged 1100 lines, {m.nelem} elements, prec {prec}", flush=True)
+    jac = synth_np.generate(78, 9, m.nelem, m.nintface, 12.0)
+    if prec == 0:
+        ref = oracle.lij_run(m, jac, 6, nthreads=min(16, os.cpu_count() or 1))
+        out = solve_plan(tbs, ctxs(9), m, jac, 6)
+        ok, worst = cf.history_close(out["hist"], ref["hist"])
+        assert ok, worst
+        assert relerr(out["U"], ref["U"]) < 1e-12
+    else:
+        ref = oracle.lij_run(m, jac, 14, nthreads=min(16, os.cpu_count() or 1))
+        out = solve_plan(tbs, ctxs(9), m, jac, 40, rtol=1e-10, prec=1)
+        assert out["hist"][-1] <= 1e-10 * out["hist"][0]
+        assert relerr(out["U"], ref["U"]) < 1e-9
 
 
 @pytest.mark.parametrize("prec", [0, 1])
