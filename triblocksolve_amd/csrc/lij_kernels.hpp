@@ -808,8 +808,8 @@ __device__ __forceinline__ double recip(double x) {
 // each lane reads its new column through a per-lane LDS pointer (D, C or A
 // of the step, or a resident identity block), every lane applies the
 // S_k update with its old column scaled by 1 (pivot lanes) or 0 (the rest),
-// and the outputs go out through per-lane destination offsets
-// u_i + s * v_i precomputed for the lane's role.  The D, A, C blocks of 4
+// and the outputs are staged in the step's LDS slot and stored a chunk later
+// in 16-byte pieces (see the output descriptors below).  The D, A, C blocks of 4
 // steps at a time (2 for blocks larger than 10x10) are prefetched into
 // double-buffered LDS one chunk ahead;
 // their sources come from the per-element table fidx = {e, A source, C
@@ -827,7 +827,6 @@ __global__ __launch_bounds__(128) void ffactor_kernel(FactorRec one, int nline, 
     constexpr int NLD = (CH * SB + 63) / 64;    // loads per lane per chunk
     constexpr int SPW = 64 / NV, LW = SPW * NV;
     using WL = tbs::WinLayout<NV, LW, TF>;
-    using WS = tbs::Soa64<BS, TF>;
     static_assert(4 * NV <= 64, "lane roles need 4*NV lanes");
     static_assert(BS <= 256, "entry index kept in 8 bits");
     __shared__ double sblk_all[2][2][CH * SB];  // [wave][buffer]
@@ -887,19 +886,54 @@ __global__ __launch_bounds__(128) void ffactor_kernel(FactorRec one, int nline, 
         const int which = rem / BS;
         code[r] = t < CH * SB ? ((st * 3 + which) << 8) | (rem - which * BS) : -1;
     }
-    // outputs: lanes of group 0/1 (X, parity-dependent) and 2 write window
-    // entries WL::index(j, st * NV + i) = u_i + st * v_i; group 3 writes W
-    // entries WS::index(i + NV j, lane of the element) = u_i + elane * v_i
-    int u[NV], v[NV];
+    // outputs: a step's Gamma (X lanes), E (A lanes) and W (W lanes) are
+    // staged in the step's own LDS slot once its D, A, C have been read -- W
+    // column-major at [0, BS), E and Gamma row-major at [BS, 2 BS) and
+    // [2 BS, 3 BS), in TF -- and a chunk's slots are stored at the start of
+    // the next chunk, before that chunk's block loads are issued.  The stores
+    // and the loads share the wave's memory counter, which completes in
+    // order, and the chunk start waits for the loads it needs: stores issued
+    // in the middle of a chunk would make that wait include their write
+    // latency (stored right after each step, the same outputs took the
+    // kernel from 2.33 to 2.66 ms at 1,024 lines of 1,280, 9x9 FP64).  A
+    // slot leaves as 16-byte pieces of VW values: W as
+    // its element's SoA-64 chunks (q = i + NV j, VW consecutive entries), E
+    // and Gamma as the VW-entry chunks of the step's NV window lanes; then
+    // the leftover single values.  Lane t's pieces are the same for every
+    // step: descriptors kind << 30 | LDS offset << 20 | destination offset,
+    // kind 0 W, 1 E, 2 Gamma, 3 none.
+    constexpr int VW = 16 / (int)sizeof(TF);
+    constexpr int NCW = BS / VW, REMW = BS - NCW * VW;   // W: pieces, leftover values
+    constexpr int NCE = NV / VW, REME = NV - NCE * VW;   // one row of E or Gamma
+    constexpr int P16 = NCW + 2 * NV * NCE, PR = REMW + 2 * NV * REME;
+    constexpr int NI = (P16 + 63) / 64, NR = (PR + 63) / 64;
+    static_assert(3 * BS < 1024 && NCW * 64 * VW < (1 << 20), "output descriptor fields");
+    uint32_t pdesc[NI + NR];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        if (grp == 3) {
-            u[i] = (int)WS::index(i + NV * j, 0);
-            v[i] = (int)(WS::index(i + NV * j, 1) - u[i]);
-        } else {
-            u[i] = (int)WL::index(j < NV ? j : 0, i);
-            v[i] = (int)(WL::index(j < NV ? j : 0, NV) - WL::index(j < NV ? j : 0, 0));
+    for (int t = 0; t < NI; ++t) {
+        const int r = lane + 64 * t;
+        uint32_t d = 3u << 30;
+        if (r < NCW) {
+            d = (uint32_t)(r * VW) << 20 | (uint32_t)(r * 64 * VW);
+        } else if (r < P16) {
+            const int r1 = r - NCW, h = r1 / (NV * NCE), r2 = r1 - h * NV * NCE, i = r2 / NCE, c = r2 - i * NCE;
+            d = (uint32_t)(1 + h) << 30 | (uint32_t)((1 + h) * BS + i * NV + c * VW) << 20 |
+                (uint32_t)(c * LW * VW + i * VW);
         }
+        pdesc[t] = d;
+    }
+#pragma unroll
+    for (int t = 0; t < NR; ++t) {
+        const int r = lane + 64 * t;
+        uint32_t d = 3u << 30;
+        if (r < REMW) {
+            d = (uint32_t)(NCW * VW + r) << 20 | (uint32_t)(r * 64);
+        } else if (r < PR) {
+            const int r1 = r - REMW, h = r1 / (NV * REME), r2 = r1 - h * NV * REME, i = r2 / REME, e = r2 - i * REME;
+            d = (uint32_t)(1 + h) << 30 | (uint32_t)((1 + h) * BS + i * NV + NCE * VW + e) << 20 |
+                (uint32_t)(e * LW + i);
+        }
+        pdesc[NI + t] = d;
     }
     const double fP[2] = {grp == 0 ? 1.0 : 0.0, grp == 1 ? 1.0 : 0.0};   // pivot lanes at parity 0 / 1
 
@@ -966,7 +1000,7 @@ __global__ __launch_bounds__(128) void ffactor_kernel(FactorRec one, int nline, 
 #pragma unroll
     for (int i = 0; i < NV; ++i) col[i] = 0.0;
 
-    const uintptr_t aW = (uintptr_t)W, aEF = (uintptr_t)EF, dGB = (uintptr_t)GB - (uintptr_t)EF;
+    const uintptr_t aEF = (uintptr_t)EF, dGB = (uintptr_t)GB - (uintptr_t)EF;
     auto step = [&](auto par, int b, int s, int sw) {
         constexpr int PAR = decltype(par)::value;
         constexpr int P = PAR ? NV : 0;
@@ -1003,28 +1037,65 @@ __global__ __launch_bounds__(128) void ffactor_kernel(FactorRec one, int nline, 
                 if (i != p) col[i] -= cpv[i] * x;
             col[p] = x;
         }
-        // outputs: Gamma_k (X lanes), E_k (A lanes), W_k, at the wave's
-        // window position of the step (the bottom wave's windows start with
-        // pad empty steps)
-        const int pos = bot ? sw + sp.pad : sw;
-        const int w = pos / SPW, st = pos - w * SPW;
-        const int mm = m0 + k;
-        const bool isW = grp == 3;
-        // every window entry of a real step is written: E_0, E''_{n-1} and
-        // Gamma_{n-1} (no neighbour) as zeros, so the factor arrays need no
-        // zero fill
-        const bool act = grp == xg || grp == 2 || isW;
+        // outputs: Gamma_k (X lanes), E_k (A lanes), W_k into the step's LDS
+        // slot (its D, A, C have been read); every window entry of a real
+        // step is written: E_0, E''_{n-1} and Gamma_{n-1} (no neighbour) as
+        // zeros, so the factor arrays need no zero fill
+        const bool act = grp == xg || grp == 2 || grp == 3;
         const bool zero = bot ? (grp == 2 && sw == 0) : ((grp == xg && k == n - 1) || (grp == 2 && k == 0));
-        const uintptr_t wdst = aW + (uintptr_t)((size_t)(mm / kGroup) * BS * kGroup * sizeof(TF));
-        const uintptr_t fdst = aEF + (uintptr_t)(grp == xg) * dGB +
-                               (uintptr_t)((wbase + (long)w * WL::STRIDE) * (long)sizeof(TF));
-        using gdst = __attribute__((address_space(1))) TF *;
-        const gdst dst = (gdst)(TF *)(isW ? wdst : fdst);
-        const int sc = isW ? mm % kGroup : st;
+        TF *stg = (TF *)&sblk[b][s * SB];
+        wsync();
         if (act) {
+            const int o = grp == 3 ? NV * j : (grp == 2 ? BS : 2 * BS) + j;
+            const int os = grp == 3 ? 1 : NV;
 #pragma unroll
-            for (int i = 0; i < NV; ++i) dst[u[i] + sc * v[i]] = zero ? TF(0) : (TF)col[i];
+            for (int i = 0; i < NV; ++i) stg[o + i * os] = zero ? TF(0) : (TF)col[i];
         }
+        wsync();
+    };
+
+    // store the staged outputs of the first nst steps of chunk c (buffer
+    // c & 1) at the wave's window positions of those steps (the bottom
+    // wave's windows start with pad empty steps)
+    auto flush = [&](int c, int nst) {
+        using gdst = __attribute__((address_space(1))) TF *;
+        typedef TF tf16 __attribute__((ext_vector_type(VW)));
+        using gdst16 = __attribute__((address_space(1))) tf16 *;
+        wsync();
+        for (int s = 0; s < nst; ++s) {
+            const int sw = c * CH + s;
+            const int k = bot ? n - 1 - sw : sw;
+            const int pos = bot ? sw + sp.pad : sw;
+            const int w = pos / SPW, st = pos - w * SPW;
+            const int mm = m0 + k;
+            const TF *stg = (const TF *)&sblk[c & 1][s * SB];
+            const size_t wg = (size_t)(mm / kGroup) * BS * kGroup;
+            const long wofs = wbase + (long)w * WL::STRIDE;
+            TF *const b16[3] = {W + wg + (mm % kGroup) * VW, EF + wofs + st * NV * VW, GB + wofs + st * NV * VW};
+            TF *const b1[3] = {W + wg + NCW * VW * kGroup + mm % kGroup, EF + wofs + NCE * VW * LW + st * NV,
+                               GB + wofs + NCE * VW * LW + st * NV};
+#pragma unroll
+            for (int t = 0; t < NI; ++t) {
+                const uint32_t d = pdesc[t], kind = d >> 30;
+                if (kind != 3) {
+                    const TF *src = stg + ((d >> 20) & 1023u);
+                    tf16 x;
+#pragma unroll
+                    for (int q = 0; q < VW; ++q) x[q] = src[q];
+                    TF *base = kind == 0 ? b16[0] : (kind == 1 ? b16[1] : b16[2]);
+                    *(gdst16)(base + (d & 0xFFFFFu)) = x;
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < NR; ++t) {
+                const uint32_t d = pdesc[NI + t], kind = d >> 30;
+                if (kind != 3) {
+                    TF *base = kind == 0 ? b1[0] : (kind == 1 ? b1[1] : b1[2]);
+                    *(gdst)(base + (d & 0xFFFFFu)) = stg[(d >> 20) & 1023u];
+                }
+            }
+        }
+        wsync();
     };
 
     // one uniform loop over step pairs (CH is even); at the first step of
@@ -1052,6 +1123,7 @@ __global__ __launch_bounds__(128) void ffactor_kernel(FactorRec one, int nline, 
         if (s == 0 && c > 0) {
             wsync();                                    // LDS of chunk c-1 read
             store_blocks(c & 1, R);                     // chunk c
+            flush(c - 1, CH);                           // outputs of chunk c-1
             set_ptrs((c + 1) & 1, xnext);               // sources of chunk c+1
             wsync();
             load_blocks((c + 1) & 1, R);
@@ -1064,9 +1136,11 @@ __global__ __launch_bounds__(128) void ffactor_kernel(FactorRec one, int nline, 
             step(std::integral_constant<int, 1>(), c & 1, s + 1, k + 1);
         }
     }
+    if (nsteps > 0) flush((nsteps - 1) / CH, nsteps - (nsteps - 1) / CH * CH);   // the last chunk
     if (bot) {
         // Gamma''_{m+1}: the X lanes of the last step (group 1 after an even
         // step, 0 after an odd one); zeros when the bottom side is empty
+        // (written after the last chunk's outputs have left its buffer)
         wsync();
         const int xl = sp.B == 0 ? -1 : (((sp.B - 1) & 1) ? 0 : 1);
         if (xl < 0) {
@@ -1091,8 +1165,9 @@ __global__ __launch_bounds__(128) void ffactor_kernel(FactorRec one, int nline, 
             const gdst e = (gdst)(TF *)(aEF + base), g = (gdst)(TF *)(aEF + dGB + base);
 #pragma unroll
             for (int i = 0; i < NV; ++i) {
-                e[u[i] + st * v[i]] = TF(0);
-                g[u[i] + st * v[i]] = TF(0);
+                const long at = WL::index(j, st * NV + i);
+                e[at] = TF(0);
+                g[at] = TF(0);
             }
         }
     }
