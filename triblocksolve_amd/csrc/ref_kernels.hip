@@ -86,6 +86,7 @@ __global__ __launch_bounds__(64) void lineLU_kernel(int nlines, const int *__res
     __shared__ double sA[BS];
     __shared__ double sG[BS];    // Gamma = D'_{k-1}^{-1} C
     __shared__ double sDe[BS];   // raw D_k
+    __shared__ int sIe[64], sIf[64], sIo[64];   // 64 steps' element, face, A from Of2
     const int l = blockIdx.x;
     if (l >= nlines) return;
     const int t = threadIdx.x;
@@ -94,9 +95,49 @@ __global__ __launch_bounds__(64) void lineLU_kernel(int nlines, const int *__res
     const int m0 = linepoint[l];
     if (n <= 0) return;
 
+    // Software pipeline: step k+1's blocks are loaded into registers while
+    // step k computes, issued before step k's stores, and the line's indices
+    // come 64 steps at a time through LDS.  The wave's loads and stores
+    // complete in order on one memory counter, so a load issued after a
+    // step's stores would make the next step wait for that store traffic as
+    // well as for its own load latency.
+    auto idx_chunk = [&](int c) {
+        const int kk = c * 64 + t;
+        if (kk < n) {
+            const int e = lines[m0 + kk];
+            const int f = kk > 0 ? lineface[m0 + kk] : 0;   // (step 0 has no face before it)
+            sIe[t] = e;
+            sIf[t] = f;
+            sIo[t] = kk > 0 && e == fc[2 * f];             // :116-117
+        }
+        __syncthreads();
+    };
+    constexpr int NQ = (BS + 63) / 64;   // block entries per lane
+    double pC[NQ], pA[NQ], pD[NQ];
+    auto blocks = [&](int e, int f, bool o2) {
+        const double *Ap = o2 ? Of2 : Of1;                           // :116
+        const double *Cp = o2 ? Of1 : Of2;                           // :117
+#pragma unroll
+        for (int u = 0; u < NQ; ++u) {
+            const int q = t + 64 * u;
+            if (q < BS) {
+                pC[u] = Cp[(size_t)BS * f + q];
+                pA[u] = Ap[(size_t)BS * f + q];
+                pD[u] = Dia[(size_t)BS * e + q];
+            }
+        }
+    };
+    idx_chunk(0);
+    int ne = 0;                  // element of the step whose blocks are in pC, pA, pD
+    int elast = sIe[0];
+    if (n > 1) {
+        ne = sIe[1];
+        blocks(ne, sIf[1], sIo[1] != 0);
+    }
+
     double r[NV];
     {
-        const int e0 = lines[m0];
+        const int e0 = elast;
 #pragma unroll
         for (int k = 0; k < NV; ++k) r[k] = (i < NV) ? Dia[(size_t)BS * e0 + i + NV * k] : 1.0;
         lu_rows<NV>(r, i);                                           // :91
@@ -110,16 +151,21 @@ __global__ __launch_bounds__(64) void lineLU_kernel(int nlines, const int *__res
     }
     __syncthreads();
     for (int k = 1; k < n; ++k) {
-        const int e = lines[m0 + k];
-        const int f = lineface[m0 + k];
-        const int e1 = fc[2 * f];
-        const int elast = lines[m0 + k - 1];
-        const double *Ap = (e == e1) ? Of2 : Of1;                    // :116
-        const double *Cp = (e == e1) ? Of1 : Of2;                    // :117
-        for (int q = t; q < BS; q += 64) {                           // :119-139
-            sC[q] = Cp[(size_t)BS * f + q];
-            sA[q] = Ap[(size_t)BS * f + q];
-            sDe[q] = Dia[(size_t)BS * e + q];
+        const int e = ne;
+#pragma unroll
+        for (int u = 0; u < NQ; ++u) {                               // :119-139
+            const int q = t + 64 * u;
+            if (q < BS) {
+                sC[q] = pC[u];
+                sA[q] = pA[u];
+                sDe[q] = pD[u];
+            }
+        }
+        if (k + 1 < n) {                                             // step k+1's blocks
+            if ((k + 1) % 64 == 0) idx_chunk((k + 1) / 64);
+            const int kn = (k + 1) % 64;
+            ne = sIe[kn];
+            blocks(ne, sIf[kn], sIo[kn] != 0);
         }
         __syncthreads();
         if (t < NV) {                                                // :142-144
@@ -165,6 +211,7 @@ __global__ __launch_bounds__(64) void lineLU_kernel(int nlines, const int *__res
             }
         }
         for (int q = t; q < BS; q += 64) DinvC[(size_t)BS * elast + q] = sG[q];
+        elast = e;
         __syncthreads();
     }
 }
