@@ -23,6 +23,7 @@ import json
 import os
 import sys
 import time
+import traceback
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -699,43 +700,54 @@ def main():
                                  "seeded system, 8 sweeps from zero, outside every timed region",
                          "oracle_s": time.perf_counter() - t0, "fp64_same_sweeps": par,
                          "tolerance": "history |h_k/h_0 - h'_k/h'_0| <= 1e-12 max(h'_k/h'_0, 1e-4); U <= 1e-12"}
+    def section(name, fn):
+        """An extra section: its failure is recorded in the JSON line (with the
+        traceback on stderr) instead of ending the run before the line."""
+        log(name)
+        try:
+            return fn()
+        except Exception as e:                      # noqa: BLE001
+            traceback.print_exc(file=sys.stderr)
+            return {"error": f"{type(e).__name__}: {e}"}
+
     if not args.no_extras:
-        log("fp32-factor timed sweeps")
-        r32 = timed_sweeps(tbs.TBS_FP32, args.steps, args.warmup)
-        i32 = r32["info"]
-        out["fp32_factor"] = {
-            "sweeps_per_s": args.steps / r32["t_graph"],
-            "gbps_model": i32["alg_bytes_per_sweep"] / (r32["t_graph"] / args.steps) / 1e9,
-            "thomas_ms": float(r32["thomas"].mean()) * 1e3,
-            "residual_ms": float(r32["residual"].mean()) * 1e3,
-            "eager_sweeps_per_s": args.steps / r32["wall"],
-        }
-        log("time to 1e-10")
-        a = time_to_tol(tbs.TBS_FP64)
-        b = time_to_tol(tbs.TBS_FP32)
-        ua, ub = a["U"].numpy(), b["U"].numpy()
-        err = float(np.linalg.norm(ub - ua) / np.linalg.norm(ua))
-        for d_, u_ in ((a, ua), (b, ub)):
-            d_.pop("U").close()
-            if ref is not None:
-                # against the oracle's solution at the round-off floor (8 sweeps)
-                d_["oracle_U_relerr_vs_converged"] = relerr(u_, ref["U"])
-        out["time_to_1e-10"] = {"fp64": a, "fp32_factor": b, "fp32_vs_fp64_U_relerr": err}
-        log("drop-in reference loop")
-        out["dropin"] = bench_dropin(tbs, ctx, m, D, O1, O2, rhs)
-        log("gmres")
-        out["gmres"] = bench_gmres(tbs, ctx, m, args)
-        log("batched")
-        out["batched"] = bench_batched(tbs, ctx, args, oracle=oracle)
-        log("serving")
-        out["serving"] = bench_serving(tbs, ctx, args, oracle=oracle)
-        log("block sizes")
-        out["block_sizes"] = bench_block_sizes(tbs, args)
+        def fp32_section():
+            r32 = timed_sweeps(tbs.TBS_FP32, args.steps, args.warmup)
+            i32 = r32["info"]
+            return {
+                "sweeps_per_s": args.steps / r32["t_graph"],
+                "gbps_model": i32["alg_bytes_per_sweep"] / (r32["t_graph"] / args.steps) / 1e9,
+                "thomas_ms": float(r32["thomas"].mean()) * 1e3,
+                "residual_ms": float(r32["residual"].mean()) * 1e3,
+                "eager_sweeps_per_s": args.steps / r32["wall"],
+            }
+
+        def tol_section():
+            a = time_to_tol(tbs.TBS_FP64)
+            b = time_to_tol(tbs.TBS_FP32)
+            ua, ub = a["U"].numpy(), b["U"].numpy()
+            err = float(np.linalg.norm(ub - ua) / np.linalg.norm(ua))
+            for d_, u_ in ((a, ua), (b, ub)):
+                d_.pop("U").close()
+                if ref is not None:
+                    # against the oracle's solution at the round-off floor (8 sweeps)
+                    d_["oracle_U_relerr_vs_converged"] = relerr(u_, ref["U"])
+            return {"fp64": a, "fp32_factor": b, "fp32_vs_fp64_U_relerr": err}
+
+        out["fp32_factor"] = section("fp32-factor timed sweeps", fp32_section)
+        out["time_to_1e-10"] = section("time to 1e-10", tol_section)
+        out["dropin"] = section("drop-in reference loop", lambda: bench_dropin(tbs, ctx, m, D, O1, O2, rhs))
+        out["gmres"] = section("gmres", lambda: bench_gmres(tbs, ctx, m, args))
+        out["batched"] = section("batched", lambda: bench_batched(tbs, ctx, args, oracle=oracle))
+        out["serving"] = section("serving", lambda: bench_serving(tbs, ctx, args, oracle=oracle))
+        out["block_sizes"] = section("block sizes", lambda: bench_block_sizes(tbs, args))
     if not args.no_cpu:
-        log("cpu baseline (child process)")
-        cb = cpu_baseline(args)
-        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
-        out["cpu_baseline_detail"] = {k: v for k, v in cb.items() if k not in out["cpu_baseline"]}
+        cb = section("cpu baseline (child process)", lambda: cpu_baseline(args))
+        if "error" in cb:
+            out["cpu_baseline"] = cb
+        else:
+            out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+            out["cpu_baseline_detail"] = {k: v for k, v in cb.items() if k not in out["cpu_baseline"]}
     ctx.close()
     print(json.dumps(out))
 

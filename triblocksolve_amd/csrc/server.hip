@@ -542,12 +542,16 @@ int tbs_server_create(tbs_ctx *ctx, size_t hbm_budget, int precision, double rto
         TBS_TRY(tbs_server_footprint(s.get(), &rq, &need));
         bool done = need > s->budget;   // (a budget too small for it: no warm-up)
         if (!done) TBS_TRY(tbs_server_submit(s.get(), &rq, &id));
-        for (int i = 0; i < 1000000 && !done; ++i) {
+        // (bounded by time, not by steps: in a fresh process the request's
+        // set-up includes the first launch of every kernel it uses, and an
+        // idle step takes microseconds)
+        const auto t_end = clk::now() + std::chrono::seconds(120);
+        while (!done && clk::now() < t_end) {
             TBS_TRY(tbs_server_step(s.get(), nullptr, nullptr, nullptr));
             done = s->finished.count(id) != 0;
             if (!done) std::this_thread::yield();
         }
-        TBS_REQUIRE(done, TBS_ERR_STATE, "server warm-up request did not finish");
+        TBS_REQUIRE(done, TBS_ERR_STATE, "server warm-up request did not finish within 120 s");
         if (id) TBS_TRY(tbs_server_release(s.get(), id));
         s->completed_unpolled.clear();
         s->next_id = 1;
