@@ -1202,8 +1202,8 @@ __global__ __launch_bounds__(128) void ffactor_wide_kernel(FactorRec one, int nl
     constexpr int NLD = (SB + 63) / 64;         // loads per lane per step
     constexpr int SPW = 64 / NV, LW = SPW * NV;
     using WL = tbs::WinLayout<NV, LW, TF>;
-    using WS = tbs::Soa64<BS, TF>;
     static_assert(2 * NV <= 64 && 4 * NV > 64, "two halves of NV lanes; smaller blocks use ffactor_kernel");
+    constexpr bool kDefer = NV <= 24;           // outputs staged in LDS and stored a step later
     __shared__ double sblk_all[2][2][SB];       // [wave][buffer]
     __shared__ double sI[BS];                   // identity: the second column of the non-pivot half
     __shared__ const double *sptr_all[2][2][3];  // [wave][buffer]: sources of a step's three blocks
@@ -1340,13 +1340,36 @@ __global__ __launch_bounds__(128) void ffactor_wide_kernel(FactorRec one, int nl
             col1[p] = x1;
             col2[p] = x2;
         }
-        // outputs at the wave's window position of the step: E_k (pivot
-        // half, second column), Gamma_k and W_k (other half); E_0, E''_{n-1}
-        // and Gamma_{n-1} (no neighbour) written as zeros
-        const int pos = bot ? sw + sp.pad : sw;
-        const int w = pos / SPW, st = pos - w * SPW;
-        const int mm = m0 + k;
-        if (!idle) {
+        // outputs: E_k (pivot half, second column), Gamma_k (other half,
+        // first column), W_k (other half, second column); E_0, E''_{n-1} and
+        // Gamma_{n-1} (no neighbour) as zeros.  Up to 24x24 blocks they are
+        // staged in the step's LDS slot (its D, A, C have been read): W
+        // column-major at [0, BS), E and Gamma row-major at [BS, 2 BS) and
+        // [2 BS, 3 BS), in TF, and stored during the next step's set-up,
+        // after its block loads (flush below); larger blocks store them
+        // here, at the wave's window position of the step.
+        if constexpr (kDefer) {
+            TF *stg = (TF *)&sblk[b][0];
+            wsync();
+            if (!idle) {
+                if (piv) {
+                    const bool zero = bot ? sw == 0 : k == 0;
+#pragma unroll
+                    for (int i = 0; i < NV; ++i) stg[BS + i * NV + j] = zero ? TF(0) : (TF)col2[i];
+                } else {
+                    const bool zero = !bot && k == n - 1;
+#pragma unroll
+                    for (int i = 0; i < NV; ++i) {
+                        stg[2 * BS + i * NV + j] = zero ? TF(0) : (TF)col1[i];
+                        stg[NV * j + i] = (TF)col2[i];
+                    }
+                }
+            }
+            wsync();
+        } else if (!idle) {
+            const int pos = bot ? sw + sp.pad : sw;
+            const int w = pos / SPW, st = pos - w * SPW;
+            const int mm = m0 + k;
             const long wofs = wbase + (long)w * WL::STRIDE;
             if (piv) {
                 const bool zero = bot ? sw == 0 : k == 0;
@@ -1360,10 +1383,67 @@ __global__ __launch_bounds__(128) void ffactor_wide_kernel(FactorRec one, int nl
 #pragma unroll
                 for (int i = 0; i < NV; ++i) {
                     g[WL::index(j, st * NV + i)] = zero ? TF(0) : (TF)col1[i];
-                    wd[WS::index(i + NV * j, mm % kGroup)] = (TF)col2[i];
+                    wd[tbs::Soa64<BS, TF>::index(i + NV * j, mm % kGroup)] = (TF)col2[i];
                 }
             }
         }
+    };
+    // Store the staged outputs of wave step sw (slot b) at the wave's window
+    // position of the step, as 16-byte pieces of VW values (W: its element's
+    // SoA-64 chunks; E, Gamma: the VW-entry chunks of the step's NV window
+    // lanes) and then the leftover single values.  A step's stores are
+    // issued after the next step's block loads: the wave's loads and stores
+    // complete in order on one memory counter, and each step's set-up waits
+    // for the whole counter, so stores issued at the end of a step put their
+    // write latency on that wait (at 384,000 elements: 17x17 10.3 -> 8.2 ms,
+    // 22x22 13.9 -> 13.2 ms; at 32x32 a step is long enough that the
+    // staging's address arithmetic costs more than it saves, 27 -> 31 ms).
+    constexpr int VW = 16 / (int)sizeof(TF);
+    constexpr int NCW = BS / VW, REMW = BS - NCW * VW;   // W: pieces, leftover values
+    constexpr int NCE = NV / VW, REME = NV - NCE * VW;   // one row of E or Gamma
+    constexpr int P16 = NCW + 2 * NV * NCE, PR = REMW + 2 * NV * REME;
+    auto flush = [&](int b, int sw) {
+        typedef TF tf16 __attribute__((ext_vector_type(VW)));
+        using gdst16 = __attribute__((address_space(1))) tf16 *;
+        const int k = bot ? n - 1 - sw : sw;
+        const int pos = bot ? sw + sp.pad : sw;
+        const int w = pos / SPW, st = pos - w * SPW;
+        const int mm = m0 + k;
+        const TF *stg = (const TF *)&sblk[b][0];
+        const size_t wg = (size_t)(mm / kGroup) * BS * kGroup;
+        const long wofs = wbase + (long)w * WL::STRIDE;
+        TF *const Wt = (TF *)aW, *const Et = (TF *)aEF, *const Gt = (TF *)(aEF + dGB);
+        wsync();
+        for (int r = lane; r < P16; r += 64) {
+            int so;
+            TF *d;
+            if (r < NCW) {
+                so = r * VW;
+                d = Wt + wg + (size_t)r * 64 * VW + (mm % kGroup) * VW;
+            } else {
+                const int r1 = r - NCW, h = r1 / (NV * NCE), r2 = r1 - h * NV * NCE, i = r2 / NCE, c = r2 - i * NCE;
+                so = (1 + h) * BS + i * NV + c * VW;
+                d = (h ? Gt : Et) + wofs + (long)c * LW * VW + (long)(st * NV + i) * VW;
+            }
+            tf16 x;
+#pragma unroll
+            for (int q = 0; q < VW; ++q) x[q] = stg[so + q];
+            *(gdst16)d = x;
+        }
+        for (int r = lane; r < PR; r += 64) {
+            int so;
+            TF *d;
+            if (r < REMW) {
+                so = NCW * VW + r;
+                d = Wt + wg + (size_t)(NCW * VW + r) * 64 + mm % kGroup;
+            } else {
+                const int r1 = r - REMW, h = r1 / (NV * REME), r2 = r1 - h * NV * REME, i = r2 / REME, e = r2 - i * REME;
+                so = (1 + h) * BS + i * NV + NCE * VW + e;
+                d = (h ? Gt : Et) + wofs + (long)(NCE * VW + e) * LW + st * NV + i;
+            }
+            *(gdst)d = stg[so];
+        }
+        wsync();
     };
     // step m (top wave): wait for the bottom wave's last Gamma'' (the one
     // block-wide barrier of the two waves after the start), then D_m -=
@@ -1391,6 +1471,7 @@ __global__ __launch_bounds__(128) void ffactor_wide_kernel(FactorRec one, int nl
             wsync();
             load_blocks((s + 1) & 1, R);
             xnext = load_idx(s + 2);
+            if constexpr (kDefer) flush((s - 1) & 1, s - 1);   // outputs of step s-1
         }
     };
     for (int s = 0; s < nsteps; s += 2) {
@@ -1403,10 +1484,14 @@ __global__ __launch_bounds__(128) void ffactor_wide_kernel(FactorRec one, int nl
             step(std::integral_constant<int, 1>(), (s + 1) & 1, s + 1);
         }
     }
+    if constexpr (kDefer) {
+        if (nsteps > 0) flush((nsteps - 1) & 1, nsteps - 1);   // the last step's outputs
+    }
     if (bot) {
         // Gamma''_{m+1}: the first columns of the non-pivot half of the last
         // step (half 1 after an even step, half 0 after an odd one); zeros
-        // when the bottom side is empty
+        // when the bottom side is empty (after the last step's outputs have
+        // left its slot)
         wsync();
         const int xh = sp.B == 0 ? -1 : (((sp.B - 1) & 1) ? 0 : 1);
         if (xh < 0) {
