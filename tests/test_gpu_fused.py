@@ -182,7 +182,7 @@ def test_fp32_factors_reach_fp64_accuracy(tbs, ctxs, oracle):
     assert relerr(o32["U"], ref["U"]) < 1e-9
 
 
-@pytest.mark.parametrize("nv", [2, 3, 5, 9, 16, 17, 21, 22, 32])
+@pytest.mark.parametrize("nv", [2, 3, 5, 9, 16, 17, 21, 22, 24, 25, 32])
 def test_one_sweep_is_the_line_solve(tbs, ctxs, nv):
     """From U = 0 one sweep is U = -M^-1 rhs exactly (M: the lines' block
     tridiagonal, assembled and solved by scipy), whatever the line lengths:
