@@ -89,17 +89,20 @@ def test_variants_fp64(tbs, ctxs, oracle, name, make):
     assert relerr(out["U"], ref["U"]) < 1e-12
 
 
+@pytest.mark.parametrize("nlines", [1100, 1400])
 @pytest.mark.parametrize("prec", [0, 1])
-def test_factor_in_two_rounds_ragged(tbs, ctxs, oracle, prec):
-    """1,100 ragged lines (seeded lengths 1..90): more lines than the factor
-    kernel's resident workgroups (1,024 for 9x9 blocks on a 256-CU MI355X),
-    so the factorisation runs its first round alone and the rest beside the
-    block packing.  FP64 against the oracle's same sweeps; FP32 factors to
-    1e-9 of the FP64 oracle's converged solution."""
+def test_factor_in_two_rounds_ragged(tbs, ctxs, oracle, prec, nlines):
+    """1,100 and 1,400 ragged lines (seeded lengths 1..90) around the factor
+    kernel's resident workgroups for 9x9 blocks on a 256-CU MI355X (1,024
+    with FP64 factors; with FP32 factors 1,280, or 1,024 when held to four
+    per CU): FP64 runs its first round alone and the rest beside the block
+    packing at both counts, FP32 factors run all 1,100 lines at once and
+    1,400 in two held rounds.  FP64 against the oracle's same sweeps; FP32
+    factors to 1e-9 of the FP64 oracle's converged solution."""
     rng = np.random.default_rng(77)
-    lens = rng.integers(1, 91, size=1100)
+    lens = rng.integers(1, 91, size=nlines)
     m = tbs.ragged_mesh(lens, nvar=9)
-    print(f"case ragged 1100 lines, {m.nelem} elements, prec {prec}", flush=True)
+    print(f"case ragged {nlines} lines, {m.nelem} elements, prec {prec}", flush=True)
     jac = synth_np.generate(78, 9, m.nelem, m.nintface, 12.0)
     if prec == 0:
         ref = oracle.lij_run(m, jac, 6, nthreads=min(16, os.cpu_count() or 1))

@@ -826,6 +826,13 @@ __global__ __launch_bounds__(128) void ffactor_kernel(FactorRec one, int nline, 
     constexpr int SB = 3 * BS;                  // doubles per step
     constexpr int NLD = (CH * SB + 63) / 64;    // loads per lane per chunk
     constexpr int SPW = 64 / NV, LW = SPW * NV;
+    // A_k broadcast from lanes in the S_k update where that was measured
+    // faster (factorisation at 384,000 elements, the LDS form = 1: 12x12 to
+    // 15x15 0.85-0.96, up to 1,200 lines of 9x9 0.9-1.0; 2x2 to 8x8, 10x10,
+    // 11x11 1.02-1.10, 16x16 1.01): it frees the VGPRs that hold A_k in
+    // every lane (9x9: 232 -> 174), which at those sizes lets more waves
+    // share a SIMD or the step schedule tighter
+    constexpr bool kBcastA = NV == 9 || (NV >= 12 && NV <= 15);
     using WL = tbs::WinLayout<NV, LW, TF>;
     static_assert(4 * NV <= 64, "lane roles need 4*NV lanes");
     static_assert(BS <= 256, "entry index kept in 8 bits");
@@ -905,6 +912,7 @@ __global__ __launch_bounds__(128) void ffactor_kernel(FactorRec one, int nline, 
     constexpr int VW = 16 / (int)sizeof(TF);
     constexpr int NCW = BS / VW, REMW = BS - NCW * VW;   // W: pieces, leftover values
     constexpr int NCE = NV / VW, REME = NV - NCE * VW;   // one row of E or Gamma
+    constexpr int NCEd = NCE > 0 ? NCE : 1, REMEd = REME > 0 ? REME : 1;   // divisors where a branch is unreached
     constexpr int P16 = NCW + 2 * NV * NCE, PR = REMW + 2 * NV * REME;
     constexpr int NI = (P16 + 63) / 64, NR = (PR + 63) / 64;
     static_assert(3 * BS < 1024 && NCW * 64 * VW < (1 << 20), "output descriptor fields");
@@ -916,7 +924,7 @@ __global__ __launch_bounds__(128) void ffactor_kernel(FactorRec one, int nline, 
         if (r < NCW) {
             d = (uint32_t)(r * VW) << 20 | (uint32_t)(r * 64 * VW);
         } else if (r < P16) {
-            const int r1 = r - NCW, h = r1 / (NV * NCE), r2 = r1 - h * NV * NCE, i = r2 / NCE, c = r2 - i * NCE;
+            const int r1 = r - NCW, h = r1 / (NV * NCEd), r2 = r1 - h * NV * NCE, i = r2 / NCEd, c = r2 - i * NCE;
             d = (uint32_t)(1 + h) << 30 | (uint32_t)((1 + h) * BS + i * NV + c * VW) << 20 |
                 (uint32_t)(c * LW * VW + i * VW);
         }
@@ -929,7 +937,7 @@ __global__ __launch_bounds__(128) void ffactor_kernel(FactorRec one, int nline, 
         if (r < REMW) {
             d = (uint32_t)(NCW * VW + r) << 20 | (uint32_t)(r * 64);
         } else if (r < PR) {
-            const int r1 = r - REMW, h = r1 / (NV * REME), r2 = r1 - h * NV * REME, i = r2 / REME, e = r2 - i * REME;
+            const int r1 = r - REMW, h = r1 / (NV * REMEd), r2 = r1 - h * NV * REME, i = r2 / REMEd, e = r2 - i * REME;
             d = (uint32_t)(1 + h) << 30 | (uint32_t)((1 + h) * BS + i * NV + NCE * VW + e) << 20 |
                 (uint32_t)(e * LW + i);
         }
@@ -1015,14 +1023,38 @@ __global__ __launch_bounds__(128) void ffactor_kernel(FactorRec one, int nline, 
         double vold[NV];
 #pragma unroll
         for (int m = 0; m < NV; ++m) vold[m] = col[m] * fP[PAR];
-        // S_k = D_k - A_k Gamma_{k-1} on the pivot lanes (A_k uniform); the
-        // other lanes subtract exact zeros
+        // S_k = D_k - A_k Gamma_{k-1} on the pivot lanes; the other lanes
+        // subtract exact zeros.  Either A_k is spread over the wave (lane t
+        // holds entry t + 64 r) and each entry is broadcast with readlane,
+        // or every lane reads all of A_k from LDS at a uniform address (a
+        // VGPR pair per entry, and the reads wait for LDS about ten times a
+        // step); the sums are the same either way.
+        if constexpr (kBcastA) {
+            constexpr int NA = (BS + 63) / 64;
+            double av[NA];
 #pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            double tot = 0.0;
+            for (int r = 0; r < NA; ++r) av[r] = Ak[lane + 64 * r < BS ? lane + 64 * r : 0];
+            double bv[NV];
 #pragma unroll
-            for (int m = 0; m < NV; ++m) tot += Ak[i + NV * m] * vold[m];
-            col[i] = bcol[i] - tot;
+            for (int i = 0; i < NV; ++i) bv[i] = bcol[i];
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                double tot = 0.0;
+#pragma unroll
+                for (int m = 0; m < NV; ++m) {
+                    const int q = i + NV * m;
+                    tot += tbs::rdlane(av[q / 64], q % 64) * vold[m];
+                }
+                col[i] = bv[i] - tot;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                double tot = 0.0;
+#pragma unroll
+                for (int m = 0; m < NV; ++m) tot += Ak[i + NV * m] * vold[m];
+                col[i] = bcol[i] - tot;
+            }
         }
         // Gauss-Jordan on the pivot columns
 #pragma unroll
@@ -1401,6 +1433,7 @@ __global__ __launch_bounds__(128) void ffactor_wide_kernel(FactorRec one, int nl
     constexpr int VW = 16 / (int)sizeof(TF);
     constexpr int NCW = BS / VW, REMW = BS - NCW * VW;   // W: pieces, leftover values
     constexpr int NCE = NV / VW, REME = NV - NCE * VW;   // one row of E or Gamma
+    constexpr int NCEd = NCE > 0 ? NCE : 1, REMEd = REME > 0 ? REME : 1;   // divisors where a branch is unreached
     constexpr int P16 = NCW + 2 * NV * NCE, PR = REMW + 2 * NV * REME;
     auto flush = [&](int b, int sw) {
         typedef TF tf16 __attribute__((ext_vector_type(VW)));
@@ -1421,7 +1454,7 @@ __global__ __launch_bounds__(128) void ffactor_wide_kernel(FactorRec one, int nl
                 so = r * VW;
                 d = Wt + wg + (size_t)r * 64 * VW + (mm % kGroup) * VW;
             } else {
-                const int r1 = r - NCW, h = r1 / (NV * NCE), r2 = r1 - h * NV * NCE, i = r2 / NCE, c = r2 - i * NCE;
+                const int r1 = r - NCW, h = r1 / (NV * NCEd), r2 = r1 - h * NV * NCE, i = r2 / NCEd, c = r2 - i * NCE;
                 so = (1 + h) * BS + i * NV + c * VW;
                 d = (h ? Gt : Et) + wofs + (long)c * LW * VW + (long)(st * NV + i) * VW;
             }
@@ -1437,7 +1470,7 @@ __global__ __launch_bounds__(128) void ffactor_wide_kernel(FactorRec one, int nl
                 so = NCW * VW + r;
                 d = Wt + wg + (size_t)(NCW * VW + r) * 64 + mm % kGroup;
             } else {
-                const int r1 = r - REMW, h = r1 / (NV * REME), r2 = r1 - h * NV * REME, i = r2 / REME, e = r2 - i * REME;
+                const int r1 = r - REMW, h = r1 / (NV * REMEd), r2 = r1 - h * NV * REME, i = r2 / REMEd, e = r2 - i * REME;
                 so = (1 + h) * BS + i * NV + NCE * VW + e;
                 d = (h ? Gt : Et) + wofs + (long)(NCE * VW + e) * LW + st * NV + i;
             }
@@ -1815,25 +1848,44 @@ int launch_final(hipStream_t st, const void *sys, int nsys, int max_sweeps, doub
 }
 
 // One lane per column of [S | C | A | I] up to NV = 16, two columns per lane above.
+// lds: dynamic LDS reserved per workgroup (unused by the kernel; it limits
+// how many workgroups share a CU)
 template <int NV, typename TF>
-void launch_factor_nv(hipStream_t st, const FactorRec &one, int nline, const FactorRec *recs, const int2 *ltab) {
+void launch_factor_nv(hipStream_t st, const FactorRec &one, int nline, const FactorRec *recs, const int2 *ltab,
+                      size_t lds) {
     if constexpr (4 * NV <= 64)
-        hipLaunchKernelGGL((ffactor_kernel<NV, TF>), dim3(nline), dim3(128), 0, st, one, nline, recs, ltab);
+        hipLaunchKernelGGL((ffactor_kernel<NV, TF>), dim3(nline), dim3(128), lds, st, one, nline, recs, ltab);
     else
         hipLaunchKernelGGL((ffactor_wide_kernel<NV, TF>), dim3(nline), dim3(128), 0, st, one, nline, recs, ltab);
 }
 
-// Workgroups of ffactor_kernel<NV, TF> resident on one CU (0: not known, or
-// a block size the wide factor kernel takes)
+// Workgroups of ffactor_kernel<NV, TF> resident on one CU with lds bytes of
+// dynamic LDS each (0: not known, or a block size the wide factor kernel
+// takes)
 template <int NV, typename TF>
-int factor_blocks_per_cu() {
+int factor_blocks_per_cu(size_t lds) {
     int nb = 0;
     if constexpr (4 * NV <= 64) {
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void *>(&ffactor_kernel<NV, TF>),
-                                                         128, 0) != hipSuccess)
+                                                         128, lds) != hipSuccess)
             nb = 0;
     }
     return nb;
+}
+
+// Dynamic LDS that holds ffactor_kernel<NV, TF> to at most four workgroups
+// (two waves per SIMD) on a CU with lds_cu bytes of LDS; 0 when its own LDS
+// already does, or not known
+template <int NV, typename TF>
+size_t factor_limit_lds(int lds_cu) {
+    if constexpr (4 * NV <= 64) {
+        hipFuncAttributes fa;
+        if (lds_cu <= 0 || hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&ffactor_kernel<NV, TF>)) != hipSuccess)
+            return 0;
+        const size_t need = (size_t)lds_cu / 5 + 1;   // more than a fifth of the CU's LDS
+        return need > fa.sharedSizeBytes ? (need - fa.sharedSizeBytes + 255) / 256 * 256 : 0;
+    }
+    return 0;
 }
 
 // The LDS packing holds 16 whole blocks per workgroup (32 KiB at NV = 16);

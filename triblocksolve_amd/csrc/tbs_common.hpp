@@ -45,8 +45,11 @@ struct tbs_ctx {
     char *dstage = nullptr;
     size_t dstage_cap = 0;
     // Workgroups of the factor kernel resident at once on the device, per
-    // (FP32 factors?, nvar); 0 until asked, -1 not known
-    int factor_round[2][17] = {};
+    // (FP32 factors?, nvar): [0] as built, [1] held to four per CU by
+    // factor_lds bytes of dynamic LDS (factor_lds 0: no such limit needed);
+    // 0 until asked, -1 not known
+    int factor_round[2][17][2] = {};
+    int factor_lds[2][17] = {};
 };
 
 namespace tbs {
