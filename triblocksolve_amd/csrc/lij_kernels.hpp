@@ -817,6 +817,16 @@ __device__ __forceinline__ double recip(double x) {
 // Of1, -1 none: the zero block).
 using tbs::FactorRec;
 
+// t += (a of lane m of the lane's row of 16) * v[m] for m = 0..NV-1 in
+// order: a DPP row broadcast per term (NV <= 16)
+template <int NV, int M = 0>
+__device__ __forceinline__ void row_bcast_dot(double a, const double (&v)[NV], double &t) {
+    if constexpr (M < NV) {
+        t += __builtin_amdgcn_mov_dpp(a, 0x150 + M, 0xF, 0xF, true) * v[M];
+        row_bcast_dot<NV, M + 1>(a, v, t);
+    }
+}
+
 template <int NV, typename TF>
 __global__ __launch_bounds__(128) void ffactor_kernel(FactorRec one, int nline, const FactorRec *__restrict__ recs,
                                                       const int2 *__restrict__ ltab) {
@@ -826,13 +836,15 @@ __global__ __launch_bounds__(128) void ffactor_kernel(FactorRec one, int nline, 
     constexpr int SB = 3 * BS;                  // doubles per step
     constexpr int NLD = (CH * SB + 63) / 64;    // loads per lane per chunk
     constexpr int SPW = 64 / NV, LW = SPW * NV;
-    // A_k broadcast from lanes in the S_k update where that was measured
-    // faster (factorisation at 384,000 elements, the LDS form = 1: 12x12 to
-    // 15x15 0.85-0.96, up to 1,200 lines of 9x9 0.9-1.0; 2x2 to 8x8, 10x10,
-    // 11x11 1.02-1.10, 16x16 1.01): it frees the VGPRs that hold A_k in
-    // every lane (9x9: 232 -> 174), which at those sizes lets more waves
-    // share a SIMD or the step schedule tighter
-    constexpr bool kBcastA = NV == 9 || (NV >= 12 && NV <= 15);
+    // How the S_k update gets A_k's entries (below), by what was measured
+    // fastest (factorisation at 384,000 elements, and at 1,024-1,600 lines of
+    // 1,280 for 9x9; the LDS form = 1): 0 LDS reads at a uniform address
+    // (2x2 to 8x8, 10x10, 11x11, 16x16; the other forms 1.01-1.10 there);
+    // 2 DPP row broadcasts (9x9 FP64, 12x12 to 15x15: 0.80-0.96); 1 readlane
+    // broadcasts (9x9 FP32 factors: the fewest VGPRs, 168, so three waves
+    // share a SIMD and 1,200 lines fit one round, 0.90 against 0.97 with
+    // the DPP form)
+    constexpr int kAMode = NV == 9 && sizeof(TF) == 4 ? 1 : (NV == 9 || (NV >= 12 && NV <= 15) ? 2 : 0);
     using WL = tbs::WinLayout<NV, LW, TF>;
     static_assert(4 * NV <= 64, "lane roles need 4*NV lanes");
     static_assert(BS <= 256, "entry index kept in 8 bits");
@@ -1024,12 +1036,31 @@ __global__ __launch_bounds__(128) void ffactor_kernel(FactorRec one, int nline, 
 #pragma unroll
         for (int m = 0; m < NV; ++m) vold[m] = col[m] * fP[PAR];
         // S_k = D_k - A_k Gamma_{k-1} on the pivot lanes; the other lanes
-        // subtract exact zeros.  Either A_k is spread over the wave (lane t
-        // holds entry t + 64 r) and each entry is broadcast with readlane,
-        // or every lane reads all of A_k from LDS at a uniform address (a
-        // VGPR pair per entry, and the reads wait for LDS about ten times a
-        // step); the sums are the same either way.
-        if constexpr (kBcastA) {
+        // subtract exact zeros.  Every lane reads all of A_k from LDS at a
+        // uniform address (a VGPR pair per entry, and the reads wait for LDS
+        // about ten times a step), or A_k is held in the wave and each entry
+        // is broadcast: lane 16 r + m holds column m (every row of 16 lanes a
+        // copy) and a DPP row broadcast takes entry (i, m) from lane m of
+        // the lane's own row (one VGPR-to-VGPR move per entry), or lane t
+        // holds entry t + 64 r and readlane broadcasts it (two moves to
+        // SGPRs per entry, four VGPRs in all).  The sums are the same, in
+        // the same order, every way.
+        if constexpr (kAMode == 2) {
+            const int cm = lane & 15;
+            const double *acol = Ak + NV * (cm < NV ? cm : 0);
+            double ac[NV];
+#pragma unroll
+            for (int i = 0; i < NV; ++i) ac[i] = acol[i];
+            double bv[NV];
+#pragma unroll
+            for (int i = 0; i < NV; ++i) bv[i] = bcol[i];
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                double tot = 0.0;
+                row_bcast_dot<NV>(ac[i], vold, tot);
+                col[i] = bv[i] - tot;
+            }
+        } else if constexpr (kAMode == 1) {
             constexpr int NA = (BS + 63) / 64;
             double av[NA];
 #pragma unroll
