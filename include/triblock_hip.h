@@ -207,17 +207,14 @@ int tbs_plan_info_get(tbs_plan *plan, tbs_plan_info *info);
  * sweep-form factors W = D'^-1, E = D'^-1 A and Gamma = D'^-1 C directly.
  * Asynchronous on the context's stream: the Jacobian arrays must stay valid
  * until the stream has run it.  (The residual blocks are packed on a second
- * stream of the context beside the factor kernel, or beside the last of its
- * lines when they take more than one round of resident workgroups; the
- * context's stream waits for them, so work queued after the call sees the
- * whole factorisation.) */
+ * stream of the context beside the factor kernel; the context's stream waits
+ * for them, so work queued after the call sees the whole factorisation.) */
 int tbs_plan_factor(tbs_plan *plan, const double *jacD, const double *jacO1,
                     const double *jacO2);
 /* Factorise n plans of one context (same block size and precision) in one
- * pass of the factor kernel over all their lines, longest first (two
- * launches when the lines take between one and two rounds of resident
- * workgroups, the packing beside the second), plus one packing launch per
- * plan: jacD[i], jacO1[i], jacO2[i] are plan i's device
+ * launch of the factor kernel over all their lines, longest first, with one
+ * packing launch per plan beside it: jacD[i], jacO1[i], jacO2[i] are plan i's
+ * device
  * arrays.  The same factors, bitwise, as n calls of tbs_plan_factor, in the
  * time of the longest line rather than the sum over plans.  No counterpart
  * in the reference (one system per run); for batches of systems. */

@@ -94,11 +94,10 @@ def test_variants_fp64(tbs, ctxs, oracle, name, make):
 def test_factor_in_two_rounds_ragged(tbs, ctxs, oracle, prec, nlines):
     """1,100 and 1,400 ragged lines (seeded lengths 1..90) around the factor
     kernel's resident workgroups for 9x9 blocks on a 256-CU MI355X (1,024
-    with FP64 factors; with FP32 factors 1,280, or 1,024 when held to four
-    per CU): FP64 runs its first round alone and the rest beside the block
-    packing at both counts, FP32 factors run all 1,100 lines at once and
-    1,400 in two held rounds.  FP64 against the oracle's same sweeps; FP32
-    factors to 1e-9 of the FP64 oracle's converged solution."""
+    with FP64 factors, 1,280 with FP32 factors): workgroups that start only
+    when others finish, beside the block packing.  FP64 against the
+    oracle's same sweeps; FP32 factors to 1e-9 of the FP64 oracle's
+    converged solution."""
     rng = np.random.default_rng(77)
     lens = rng.integers(1, 91, size=nlines)
     m = tbs.ragged_mesh(lens, nvar=9)

@@ -349,55 +349,19 @@ int launch_factor_wide(int nv, int prec, hipStream_t st, const FactorRec &one, i
 }
 
 int launch_factor(int nv, int prec, hipStream_t st, const FactorRec &one, int nline, const FactorRec *recs,
-                  const int2 *ltab, size_t lds = 0) {
+                  const int2 *ltab) {
     if (nline == 0) return TBS_OK;
     if (nv > 16) TBS_TRY(launch_factor_wide(nv, prec, st, one, nline, recs, ltab));
     switch (nv) {
 #define X(v) case v: \
-        if (prec == TBS_FP32) launch_factor_nv<v, float>(st, one, nline, recs, ltab, lds); \
-        else launch_factor_nv<v, double>(st, one, nline, recs, ltab, lds); \
+        if (prec == TBS_FP32) launch_factor_nv<v, float>(st, one, nline, recs, ltab); \
+        else launch_factor_nv<v, double>(st, one, nline, recs, ltab); \
         break;
         TBS_PLAN_NVARS(X)
 #undef X
     }
     TBS_LAUNCHED("ffactor_kernel");
     return TBS_OK;
-}
-
-// Workgroups (lines) of the factor kernel resident at once on the context's
-// device (limited: held to four per CU by *lds bytes of dynamic LDS, *lds = 0
-// when the kernel's own resources already hold it there); 0 when not known
-// (block sizes of the wide factor kernel)
-int factor_round(tbs_ctx *ctx, int nv, int prec, bool limited, size_t *lds) {
-    *lds = 0;
-    if (nv < 2 || nv > 16) return 0;
-    const int pi = prec == TBS_FP32 ? 1 : 0;
-    int *c = ctx->factor_round[pi][nv];
-    if (c[0] == 0) {
-        int cus = 0, lds_cu = 0;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) cus = 0;
-        if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, ctx->device) !=
-            hipSuccess)
-            lds_cu = 0;
-        size_t extra = 0;
-        int per = 0, per4 = 0;
-        switch (nv) {
-#define X(v) case v: \
-            if (prec == TBS_FP32) { extra = factor_limit_lds<v, float>(lds_cu); per = factor_blocks_per_cu<v, float>(0); \
-                                    per4 = extra ? factor_blocks_per_cu<v, float>(extra) : per; } \
-            else { extra = factor_limit_lds<v, double>(lds_cu); per = factor_blocks_per_cu<v, double>(0); \
-                   per4 = extra ? factor_blocks_per_cu<v, double>(extra) : per; } \
-            break;
-            TBS_PLAN_NVARS(X)
-#undef X
-        }
-        c[0] = per > 0 && cus > 0 ? per * cus : -1;
-        c[1] = per4 > 0 && cus > 0 ? per4 * cus : -1;
-        ctx->factor_lds[pi][nv] = c[1] > 0 && per4 < per ? (int)extra : 0;
-    }
-    const int r = c[limited ? 1 : 0];
-    if (limited && r > 0) *lds = (size_t)ctx->factor_lds[pi][nv];
-    return r > 0 ? r : 0;
 }
 
 int launch_pack(const tbs_plan *p, hipStream_t pst, const double *jacD, const double *jacO1, const double *jacO2) {
@@ -814,40 +778,16 @@ int tbs_plans_factor(tbs_ctx *ctx, tbs_plan *const *plans, int n, const double *
     }
     hipStream_t pst;
     TBS_TRY(tbs::aux_stream(ctx, &pst));
-    // The factor kernel and the packing slow each other down (both wait on
-    // memory; together they take about what they take one after the other).
-    // When the lines take between one and two rounds of resident factor
-    // workgroups, the first round runs alone and the packing runs beside the
-    // second, short one: at 1,200 lines of 1,280 (1,024 resident) 5.1 ms
-    // against 5.6 ms with the packing beside the whole factorisation.
-    // From 9x9 blocks a step is long enough that more than two waves per
-    // SIMD only pay when every line is resident at once (FP32 factors, 9x9:
-    // 1,200 lines of 1,280 in one round of five workgroups per CU 3.9 ms
-    // against 4.3 ms in two rounds of four; 1,571-1,600 lines in a first
-    // round of 1,280 5-10% slower than in one of 1,024): beyond one round
-    // the factor kernel is held to four workgroups per CU.
-    size_t lds = 0;
-    int R = factor_round(ctx, nv, prec, false, &lds);
-    if (nv >= 9 && R > 0 && nl > R) R = factor_round(ctx, nv, prec, true, &lds);
-    const long nA = R > 0 && nl > R && nl < 2L * R ? R : 0;
-    if (nA == 0) {
-        TBS_HIP(hipEventRecord(ctx->ev_fork, st));
-        TBS_HIP(hipStreamWaitEvent(pst, ctx->ev_fork, 0));
-    }
+    // The packing starts beside the factor kernel.  (Running a first round of
+    // resident factor workgroups alone and the packing beside the rest was
+    // faster with an earlier, slower factor kernel; with this one it is no
+    // faster at 1,200-1,600 lines of 1,280 and 8-11% slower for 64 systems
+    // of short lines, measured.)
+    TBS_HIP(hipEventRecord(ctx->ev_fork, st));
+    TBS_HIP(hipStreamWaitEvent(pst, ctx->ev_fork, 0));
     if (n == 1) {
         const FactorRec fr = factor_rec(plans[0], jacD[0], jacO1[0], jacO2[0]);
-        if (nA > 0) {
-            TBS_TRY(launch_factor(nv, prec, st, fr, (int)nA, nullptr, nullptr, lds));
-            TBS_HIP(hipEventRecord(ctx->ev_fork, st));
-            TBS_HIP(hipStreamWaitEvent(pst, ctx->ev_fork, 0));
-            FactorRec fb = fr;
-            fb.linepoint += nA;
-            fb.linesize += nA;
-            fb.woff += nA;
-            TBS_TRY(launch_factor(nv, prec, st, fb, (int)(nl - nA), nullptr, nullptr, lds));
-        } else {
-            TBS_TRY(launch_factor(nv, prec, st, fr, (int)nl, nullptr, nullptr, lds));
-        }
+        TBS_TRY(launch_factor(nv, prec, st, fr, (int)nl, nullptr, nullptr));
     } else {
         // one launch over every plan's lines, longest first; the records and
         // the line table go up through the staging ring into a device buffer
@@ -871,20 +811,13 @@ int tbs_plans_factor(tbs_ctx *ctx, tbs_plan *const *plans, int n, const double *
         TBS_TRY(tbs::stage_release(ctx));
         const FactorRec *drec = (const FactorRec *)ctx->dstage;
         const int2 *dlt = (const int2 *)(ctx->dstage + lt_off);
-        if (nA > 0) {
-            TBS_TRY(launch_factor(nv, prec, st, FactorRec{}, (int)nA, drec, dlt, lds));
-            TBS_HIP(hipEventRecord(ctx->ev_fork, st));
-            TBS_HIP(hipStreamWaitEvent(pst, ctx->ev_fork, 0));
-            TBS_TRY(launch_factor(nv, prec, st, FactorRec{}, (int)(nl - nA), drec, dlt + nA, lds));
-        } else {
-            TBS_TRY(launch_factor(nv, prec, st, FactorRec{}, (int)nl, drec, dlt, lds));
-        }
+        TBS_TRY(launch_factor(nv, prec, st, FactorRec{}, (int)nl, drec, dlt));
     }
     // residual blocks, packed on the context's second stream while the
     // factor kernel (two waves per line) runs on the main one: queued after
     // the factor kernel (its waves take their CUs first) and forked after the
-    // memsets, or after the first round; the main stream waits for the
-    // packing before anything after the call
+    // memsets; the main stream waits for the packing before anything after
+    // the call
     for (int i = 0; i < n; ++i) TBS_TRY(launch_pack(plans[i], pst, jacD[i], jacO1[i], jacO2[i]));
     TBS_HIP(hipEventRecord(ctx->ev_join, pst));
     TBS_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));

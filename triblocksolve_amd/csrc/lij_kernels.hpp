@@ -1879,44 +1879,12 @@ int launch_final(hipStream_t st, const void *sys, int nsys, int max_sweeps, doub
 }
 
 // One lane per column of [S | C | A | I] up to NV = 16, two columns per lane above.
-// lds: dynamic LDS reserved per workgroup (unused by the kernel; it limits
-// how many workgroups share a CU)
 template <int NV, typename TF>
-void launch_factor_nv(hipStream_t st, const FactorRec &one, int nline, const FactorRec *recs, const int2 *ltab,
-                      size_t lds) {
+void launch_factor_nv(hipStream_t st, const FactorRec &one, int nline, const FactorRec *recs, const int2 *ltab) {
     if constexpr (4 * NV <= 64)
-        hipLaunchKernelGGL((ffactor_kernel<NV, TF>), dim3(nline), dim3(128), lds, st, one, nline, recs, ltab);
+        hipLaunchKernelGGL((ffactor_kernel<NV, TF>), dim3(nline), dim3(128), 0, st, one, nline, recs, ltab);
     else
         hipLaunchKernelGGL((ffactor_wide_kernel<NV, TF>), dim3(nline), dim3(128), 0, st, one, nline, recs, ltab);
-}
-
-// Workgroups of ffactor_kernel<NV, TF> resident on one CU with lds bytes of
-// dynamic LDS each (0: not known, or a block size the wide factor kernel
-// takes)
-template <int NV, typename TF>
-int factor_blocks_per_cu(size_t lds) {
-    int nb = 0;
-    if constexpr (4 * NV <= 64) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void *>(&ffactor_kernel<NV, TF>),
-                                                         128, lds) != hipSuccess)
-            nb = 0;
-    }
-    return nb;
-}
-
-// Dynamic LDS that holds ffactor_kernel<NV, TF> to at most four workgroups
-// (two waves per SIMD) on a CU with lds_cu bytes of LDS; 0 when its own LDS
-// already does, or not known
-template <int NV, typename TF>
-size_t factor_limit_lds(int lds_cu) {
-    if constexpr (4 * NV <= 64) {
-        hipFuncAttributes fa;
-        if (lds_cu <= 0 || hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&ffactor_kernel<NV, TF>)) != hipSuccess)
-            return 0;
-        const size_t need = (size_t)lds_cu / 5 + 1;   // more than a fifth of the CU's LDS
-        return need > fa.sharedSizeBytes ? (need - fa.sharedSizeBytes + 255) / 256 * 256 : 0;
-    }
-    return 0;
 }
 
 // The LDS packing holds 16 whole blocks per workgroup (32 KiB at NV = 16);

@@ -35,8 +35,8 @@ int factor(int nv, int prec, hipStream_t st, const FactorRec &one, int nline, co
            const int2 *ltab) {
     switch (nv) {
 #define X(v) case v: \
-        if (prec == TBS_FP32) launch_factor_nv<v, float>(st, one, nline, recs, ltab, 0); \
-        else launch_factor_nv<v, double>(st, one, nline, recs, ltab, 0); \
+        if (prec == TBS_FP32) launch_factor_nv<v, float>(st, one, nline, recs, ltab); \
+        else launch_factor_nv<v, double>(st, one, nline, recs, ltab); \
         return TBS_OK;
         TBS_WIDE_NVARS(X)
 #undef X

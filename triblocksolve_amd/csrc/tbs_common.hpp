@@ -44,12 +44,6 @@ struct tbs_ctx {
     // stream order on the context's stream).
     char *dstage = nullptr;
     size_t dstage_cap = 0;
-    // Workgroups of the factor kernel resident at once on the device, per
-    // (FP32 factors?, nvar): [0] as built, [1] held to four per CU by
-    // factor_lds bytes of dynamic LDS (factor_lds 0: no such limit needed);
-    // 0 until asked, -1 not known
-    int factor_round[2][17][2] = {};
-    int factor_lds[2][17] = {};
 };
 
 namespace tbs {
