@@ -839,7 +839,8 @@ __global__ __launch_bounds__(128) void ffactor_kernel(FactorRec one, int nline, 
     // How the S_k update gets A_k's entries (below), by what was measured
     // fastest (factorisation at 384,000 elements, and at 1,024-1,600 lines of
     // 1,280 for 9x9; the LDS form = 1): 0 LDS reads at a uniform address
-    // (2x2 to 8x8, 10x10, 11x11, 16x16; the other forms 1.01-1.10 there);
+    // (2x2 to 8x8, 10x10, 11x11, 16x16: readlane 1.01-1.10, DPP 0.93-1.08
+    // with no consistent gain);
     // 2 DPP row broadcasts (9x9 FP64, 12x12 to 15x15: 0.80-0.96); 1 readlane
     // broadcasts (9x9 FP32 factors: the fewest VGPRs, 168, so three waves
     // share a SIMD and 1,200 lines fit one round, 0.90 against 0.97 with
